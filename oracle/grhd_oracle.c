@@ -1,0 +1,1746 @@
+/*
+ * grhd_oracle.c -- CPU restatement (FP64, scalar C) of the GRHD2 per-step
+ * finite-volume update.
+ *
+ * TEST INFRASTRUCTURE ONLY: this is the checker the HIP path is compared
+ * against; nothing in the product (libgrhd_hip.so, the Fortran driver, the
+ * serving loop) links or calls it.
+ *
+ * Every function names the reference routine it restates (file:line into the
+ * reference checkout).  Expressions keep the reference's evaluation order so
+ * that, built with -ffp-contract=off, results can be compared bit-for-bit with
+ * the reference compiled the same way (oracle/_ref, see oracle/Makefile).
+ *
+ * Differences from the reference that are deliberate:
+ *  - the metric is evaluated where needed instead of read from a stored cache
+ *    (the cache holds the values of the same functions: initialization.f90:265-391);
+ *  - the wave-speed diagnostic probe that stops the program at n_steps == 0
+ *    (fluxes.f90:790-802) is not reproduced.
+ *
+ * Parity status of each part (how it is pinned) is listed in the test files
+ * tests/test_oracle_*.py.
+ */
+#include "grhd_oracle.h"
+
+#include <float.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define NEQ 5
+#define EQ_DE 0
+#define EQ_PR 1
+#define EQ_VX 2
+#define EQ_VY 3
+#define EQ_VZ 4
+
+/* variables.f90:157-158 */
+static const double V_MAX = 1.0 - 1.0e-5;
+
+struct orc_problem {
+    orc_config c;
+    int ng;
+    double g1, rho_floor, p_floor;
+    double x_min, x_max, dx, dy, dz;
+    double *x, *y, *z;            /* centres, index -ng..n+ng stored at [i+ng] */
+    double *x_face, *y_face, *z_face; /* faces 0..n */
+    double *injector;             /* Michel outer ghosts (5, ng, ny, nz) */
+};
+
+/* ---------------------------------------------------------------- helpers */
+static inline double dmax(double a, double b) { return a > b ? a : b; }
+static inline double dmin(double a, double b) { return a < b ? a : b; }
+static inline double sgn1(double a) { return copysign(1.0, a); } /* sign(1d0, a) */
+
+static inline size_t pidx(const orc_problem *P, int v, int i, int j, int k)
+{
+    /* i, j, k are reference indices in [-ng, n+ng] */
+    const int ng = P->ng;
+    const size_t NXg = (size_t)P->c.nx + 2 * ng + 1, NYg = (size_t)P->c.ny + 2 * ng + 1;
+    return ((((size_t)(k + ng) * NYg + (size_t)(j + ng)) * NXg + (size_t)(i + ng)) * NEQ) + v;
+}
+static inline size_t uidx(const orc_problem *P, int v, int i, int j, int k)
+{
+    /* 1-based interior indices */
+    return ((((size_t)(k - 1) * P->c.ny + (size_t)(j - 1)) * P->c.nx + (size_t)(i - 1)) * NEQ) + v;
+}
+#define XC(i) (P->x[(i) + P->ng])
+#define YC(j) (P->y[(j) + P->ng])
+#define ZC(k) (P->z[(k) + P->ng])
+
+/* ------------------------------------------------------------------ metric */
+typedef struct {
+    double alpha, beta[3], g[3][3], det, gm[4][4], dlna[4];
+} Met;
+
+/* Minkowski, Cartesian (metrics.f90:209-257). */
+static void met_mink_cart(Met *m)
+{
+    memset(m, 0, sizeof(*m));
+    m->alpha = 1.0;
+    m->g[0][0] = m->g[1][1] = m->g[2][2] = 1.0;
+    m->det = 1.0;
+    m->gm[0][0] = -1.0;
+    m->gm[1][1] = m->gm[2][2] = m->gm[3][3] = 1.0;
+}
+
+/* Minkowski, cylindrical (x1 = rho, x2 = phi, x3 = z) (metrics.f90:419-481). */
+static void met_mink_cyl(double xp, Met *m, double ch[4][4][4], double dg[4][4][3])
+{
+    memset(m, 0, sizeof(*m));
+    m->alpha = 1.0;
+    m->g[0][0] = 1.0;
+    m->g[1][1] = xp * xp;
+    m->g[2][2] = 1.0;
+    m->det = xp * xp;
+    m->gm[0][0] = -1.0;
+    m->gm[1][1] = 1.0;
+    m->gm[2][2] = 1.0 / dmax(xp * xp, 1.0e-20);
+    m->gm[3][3] = 1.0;
+    if (ch) {
+        memset(ch, 0, sizeof(double) * 64);
+        ch[1][2][2] = -xp;
+        ch[2][1][2] = 1.0 / dmax(xp, 1.0e-15);
+        ch[2][2][1] = 1.0 / dmax(xp, 1.0e-15);
+    }
+    if (dg) {
+        memset(dg, 0, sizeof(double) * 48);
+        dg[2][2][0] = 2.0 * xp;
+    }
+}
+
+/* Minkowski, spherical, physical r (metrics.f90:265-333). */
+static void met_mink_sph_phys(const orc_problem *P, double r, double th, Met *m,
+                              double ch[4][4][4], double dg[4][4][3])
+{
+    const double r2 = r * r, s = sin(th), s2 = s * s;
+    memset(m, 0, sizeof(*m));
+    m->alpha = 1.0;
+    m->g[0][0] = 1.0;
+    m->g[1][1] = r2;
+    m->g[2][2] = r2 * s2;
+    m->det = r2 * r2 * s2;
+    m->gm[0][0] = -1.0;
+    m->gm[1][1] = 1.0;
+    m->gm[2][2] = 1.0 / r2;
+    m->gm[3][3] = 1.0 / dmax(r2 * s2, 1.0e-30);
+    if (dg) {
+        const double c = cos(th);
+        memset(dg, 0, sizeof(double) * 48);
+        dg[2][2][0] = 2.0 * r;
+        dg[3][3][0] = 2.0 * r * (s * s);
+        dg[3][3][1] = 2.0 * (r * r) * s * c;
+        if (P->c.ny == 1) { for (int a = 0; a < 4; a++) for (int b = 0; b < 4; b++) dg[a][b][1] = 0.0; }
+    }
+    if (ch) {
+        const double c = cos(th), ss = copysign(dmax(fabs(s), 1.0e-20), s);
+        memset(ch, 0, sizeof(double) * 64);
+        ch[1][2][2] = -r;
+        ch[1][3][3] = -r * (s * s);
+        ch[2][1][2] = 1.0 / r;
+        ch[2][2][1] = ch[2][1][2];
+        ch[2][3][3] = -s * c;
+        ch[3][1][3] = 1.0 / r;
+        ch[3][3][1] = ch[3][1][3];
+        ch[3][2][3] = c / ss;
+        ch[3][3][2] = ch[3][2][3];
+    }
+}
+
+/* Minkowski, spherical, x = ln r (metrics.f90:338-411). */
+static void met_mink_sph_log(const orc_problem *P, double xp, double th, Met *m,
+                             double ch[4][4][4], double dg[4][4][3])
+{
+    const double r = exp(xp), r2 = r * r, s = sin(th), s2 = s * s;
+    memset(m, 0, sizeof(*m));
+    m->alpha = 1.0;
+    m->g[0][0] = r2;
+    m->g[1][1] = r2;
+    m->g[2][2] = r2 * s2;
+    m->det = r2 * r2 * r2 * s2;
+    m->gm[0][0] = -1.0;
+    m->gm[1][1] = 1.0 / r2;
+    m->gm[2][2] = 1.0 / r2;
+    m->gm[3][3] = 1.0 / dmax(r2 * s2, 1.0e-30);
+    if (dg) {
+        const double e2 = exp(2.0 * xp), c = cos(th);
+        memset(dg, 0, sizeof(double) * 48);
+        dg[1][1][0] = 2.0 * e2;
+        dg[2][2][0] = 2.0 * e2;
+        dg[3][3][0] = 2.0 * e2 * (s * s);
+        dg[3][3][1] = 2.0 * e2 * s * c;
+        if (P->c.ny == 1) { for (int a = 0; a < 4; a++) for (int b = 0; b < 4; b++) dg[a][b][1] = 0.0; }
+    }
+    if (ch) {
+        const double c = cos(th), ss = copysign(dmax(fabs(s), 1.0e-20), s);
+        memset(ch, 0, sizeof(double) * 64);
+        ch[1][1][1] = 1.0;
+        ch[1][2][2] = -1.0;
+        ch[1][3][3] = -(s * s);
+        ch[2][1][2] = 1.0;
+        ch[2][2][1] = 1.0;
+        ch[2][3][3] = -s * c;
+        ch[3][1][3] = 1.0;
+        ch[3][3][1] = 1.0;
+        ch[3][2][3] = c / ss;
+        ch[3][3][2] = ch[3][2][3];
+    }
+}
+
+/* Schwarzschild in ingoing Eddington-Finkelstein, physical r (metrics.f90:492-610). */
+static void met_ef_phys(const orc_problem *P, double r, double th, Met *m, double ch[4][4][4],
+                        double dg[4][4][3])
+{
+    const double M = P->c.mass, s = sin(th);
+    memset(m, 0, sizeof(*m));
+    m->alpha = 1.0 / sqrt(1.0 + 2.0 * M / r);
+    m->beta[0] = 2.0 * M / r / (1.0 + 2.0 * M / r);
+    const double g11 = 1.0 + 2.0 * M / r, g22 = r * r, g33 = (r * r) * (s * s);
+    m->det = g11 * g22 * g33;
+    m->g[0][0] = g11;
+    m->g[1][1] = g22;
+    m->g[2][2] = g33;
+    const double a2 = m->alpha * m->alpha;
+    m->gm[0][0] = -1.0 / a2;
+    for (int i = 0; i < 3; i++) { m->gm[0][i + 1] = m->beta[i] / a2; m->gm[i + 1][0] = m->gm[0][i + 1]; }
+    m->gm[1][1] = 1.0 / g11 - (m->beta[0] * m->beta[0]) / a2;
+    m->gm[2][2] = 1.0 / g22 - (m->beta[1] * m->beta[1]) / a2;
+    m->gm[3][3] = 1.0 / g33 - (m->beta[2] * m->beta[2]) / a2;
+    m->dlna[1] = M / (r * r + 2.0 * M * r);
+    if (ch) {
+        const double c = cos(th), ss = copysign(dmax(fabs(s), 1.0e-20), s);
+        memset(ch, 0, sizeof(double) * 64);
+        ch[0][0][0] = 2.0 * (M * M) / (r * r * r);
+        ch[0][0][1] = M * (1.0 + 2.0 * M / r) / (r * r);
+        ch[0][1][0] = ch[0][0][1];
+        ch[0][1][1] = 2.0 * M * (1.0 + M / r) / (r * r);
+        ch[0][2][2] = -2.0 * M;
+        ch[0][3][3] = ch[0][2][2] * (s * s);
+        ch[1][0][0] = M / (r * r) * (1.0 - 2.0 * M / r);
+        ch[1][0][1] = -ch[0][0][0];
+        ch[1][1][0] = ch[1][0][1];
+        ch[1][1][1] = -ch[0][0][1];
+        ch[1][2][2] = 2.0 * M - r;
+        ch[1][3][3] = ch[1][2][2] * (s * s);
+        ch[2][1][2] = 1.0 / r;
+        ch[2][2][1] = ch[2][1][2];
+        ch[2][3][3] = -s * c;
+        ch[3][1][3] = 1.0 / r;
+        ch[3][3][1] = ch[3][1][3];
+        ch[3][2][3] = c / ss;
+        ch[3][3][2] = ch[3][2][3];
+    }
+    if (dg) {
+        const double c = cos(th);
+        memset(dg, 0, sizeof(double) * 48);
+        dg[0][0][0] = -2.0 * M / (r * r);
+        dg[0][1][0] = -2.0 * M / (r * r);
+        dg[1][0][0] = dg[0][1][0];
+        dg[1][1][0] = -2.0 * M / (r * r);
+        dg[2][2][0] = 2.0 * r;
+        dg[3][3][0] = 2.0 * r * (s * s);
+        dg[3][3][1] = (r * r) * 2.0 * s * c;
+        if (P->c.ny == 1) { for (int a = 0; a < 4; a++) for (int b = 0; b < 4; b++) dg[a][b][1] = 0.0; }
+    }
+}
+
+/* Schwarzschild in ingoing EF, x = ln r (metrics.f90:612-746). */
+static void met_ef_log(const orc_problem *P, double xp, double th, Met *m, double ch[4][4][4],
+                       double dg[4][4][3])
+{
+    const double M = P->c.mass, r = exp(xp), s = sin(th);
+    memset(m, 0, sizeof(*m));
+    m->alpha = 1.0 / sqrt(1.0 + 2.0 * M / r);
+    m->beta[0] = (2.0 * M / r / (1.0 + 2.0 * M / r)) / r;
+    const double g11 = (1.0 + 2.0 * M / r) * (r * r), g22 = r * r, g33 = (r * r) * (s * s);
+    m->det = g11 * g22 * g33;
+    m->g[0][0] = g11;
+    m->g[1][1] = g22;
+    m->g[2][2] = g33;
+    const double a2 = m->alpha * m->alpha;
+    m->gm[0][0] = -1.0 / a2;
+    for (int i = 0; i < 3; i++) { m->gm[0][i + 1] = m->beta[i] / a2; m->gm[i + 1][0] = m->gm[0][i + 1]; }
+    m->gm[1][1] = 1.0 / g11 - (m->beta[0] * m->beta[0]) / a2;
+    m->gm[2][2] = 1.0 / g22 - (m->beta[1] * m->beta[1]) / a2;
+    m->gm[3][3] = 1.0 / g33 - (m->beta[2] * m->beta[2]) / a2;
+    m->dlna[1] = M / (r + 2.0 * M);
+    if (ch) {
+        const double c = cos(th), ss = copysign(dmax(fabs(s), 1.0e-20), s);
+        memset(ch, 0, sizeof(double) * 64);
+        ch[0][0][0] = 2.0 * (M * M) / (r * r * r);
+        ch[0][0][1] = M * (1.0 + 2.0 * M / r) / r;
+        ch[0][1][0] = ch[0][0][1];
+        ch[0][1][1] = 2.0 * M * (1.0 + M / r);
+        ch[0][2][2] = -2.0 * M;
+        ch[0][3][3] = ch[0][2][2] * (s * s);
+        ch[1][0][0] = M / (r * r * r) * (1.0 - 2.0 * M / r);
+        ch[1][0][1] = -2.0 * (M * M) / (r * r * r);
+        ch[1][1][0] = ch[1][0][1];
+        ch[1][1][1] = 1.0 - M / r - 2.0 * (M * M) / (r * r);
+        ch[1][2][2] = 2.0 * M / r - 1.0;
+        ch[1][3][3] = ch[1][2][2] * (s * s);
+        ch[2][1][2] = 1.0;
+        ch[2][2][1] = ch[2][1][2];
+        ch[2][3][3] = -s * c;
+        ch[3][1][3] = 1.0;
+        ch[3][3][1] = ch[3][1][3];
+        ch[3][2][3] = c / ss;
+        ch[3][3][2] = ch[3][2][3];
+    }
+    if (dg) {
+        const double c = cos(th);
+        memset(dg, 0, sizeof(double) * 48);
+        dg[0][0][0] = -2.0 * M / r;
+        dg[0][1][0] = 0.0;
+        dg[1][0][0] = 0.0;
+        dg[1][1][0] = 2.0 * (r * r) + 2.0 * M * r;
+        dg[2][2][0] = 2.0 * (r * r);
+        dg[3][3][0] = 2.0 * (r * r) * (s * s);
+        dg[3][3][1] = 2.0 * (r * r) * s * c;
+        if (P->c.ny == 1) { for (int a = 0; a < 4; a++) for (int b = 0; b < 4; b++) dg[a][b][1] = 0.0; }
+    }
+}
+
+/* Kerr-Schild, physical r or x = ln r.  metric: metrics.f90:753-838 (phys),
+ * 998-1085 (log); derivatives: 844-921 (phys), 1091-1180 (log). */
+static void met_ks(const orc_problem *P, double xp, double th, int logr, Met *m,
+                   double dg[4][4][3])
+{
+    const double M = P->c.mass, a = P->c.spin;
+    const double r = logr ? exp(xp) : xp;
+    const double s = sin(th), c = cos(th), s2 = s * s, c2 = c * c;
+    const double Sig = r * r + (a * a) * c2;
+    const double H = M * r / Sig;
+    const double Del = r * r - 2.0 * M * r + a * a;
+    memset(m, 0, sizeof(*m));
+    m->alpha = 1.0 / sqrt(1.0 + 2.0 * H);
+    if (logr) {
+        m->beta[0] = (2.0 * H / (1.0 + 2.0 * H)) / r;
+        m->g[0][0] = (1.0 + 2.0 * H) * (r * r);
+        m->g[0][2] = -a * (1.0 + 2.0 * H) * s2 * r;
+        m->det = (Sig * Sig * s2 * (1.0 + 2.0 * H)) * (r * r);
+    } else {
+        m->beta[0] = 2.0 * H / (1.0 + 2.0 * H);
+        m->g[0][0] = 1.0 + 2.0 * H;
+        m->g[0][2] = -a * (1.0 + 2.0 * H) * s2;
+        m->det = (Sig * Sig) * s2 * (1.0 + 2.0 * H);
+    }
+    m->g[2][0] = m->g[0][2];
+    m->g[1][1] = Sig;
+    m->g[2][2] = (r * r + a * a + 2.0 * H * (a * a) * s2) * s2;
+    m->gm[0][0] = -(1.0 + 2.0 * H);
+    if (logr) {
+        m->gm[0][1] = (2.0 * H) / r;
+        m->gm[1][1] = (Del / Sig) / (r * r);
+        m->gm[1][3] = (a / Sig) / r;
+    } else {
+        m->gm[0][1] = 2.0 * H;
+        m->gm[1][1] = Del / Sig;
+        m->gm[1][3] = a / Sig;
+    }
+    m->gm[1][0] = m->gm[0][1];
+    m->gm[3][1] = m->gm[1][3];
+    m->gm[2][2] = 1.0 / Sig;
+    if (fabs(s) > 1.0e-15) m->gm[3][3] = 1.0 / (Sig * s2);
+    else m->gm[3][3] = 1.0 / (Sig * 1.0e-30);
+    {
+        const double dH_dr = M * ((a * a) * c2 - r * r) / (Sig * Sig);
+        const double dH_dth = 2.0 * M * r * (a * a) * s * c / (Sig * Sig);
+        if (logr) m->dlna[1] = -(dH_dr * r) / (1.0 + 2.0 * H);
+        else m->dlna[1] = -dH_dr / (1.0 + 2.0 * H);
+        m->dlna[2] = -dH_dth / (1.0 + 2.0 * H);
+    }
+    if (dg) {
+        const double dS_dr = 2.0 * r, dS_dth = -2.0 * (a * a) * s * c;
+        const double dH_dr = M * ((a * a) * c2 - r * r) / (Sig * Sig);
+        const double dH_dth = 2.0 * M * r * (a * a) * s * c / (Sig * Sig);
+        memset(dg, 0, sizeof(double) * 48);
+        if (logr) {
+            const double gtr = 2.0 * H, grr = 1.0 + 2.0 * H, grp = -a * (1.0 + 2.0 * H) * s2;
+            dg[0][0][0] = r * (2.0 * dH_dr);
+            dg[0][1][0] = r * gtr + (r * r) * (2.0 * dH_dr);
+            dg[1][0][0] = dg[0][1][0];
+            dg[0][3][0] = r * (-2.0 * a * s2 * dH_dr);
+            dg[3][0][0] = dg[0][3][0];
+            dg[1][1][0] = 2.0 * (r * r) * grr + (r * r * r) * (2.0 * dH_dr);
+            dg[1][3][0] = r * grp + (r * r) * (-2.0 * a * s2 * dH_dr);
+            dg[3][1][0] = dg[1][3][0];
+            dg[2][2][0] = 2.0 * (r * r);
+            dg[3][3][0] = r * (2.0 * r * s2 + 2.0 * (a * a) * (s2 * s2) * dH_dr);
+            dg[0][0][1] = 2.0 * dH_dth;
+            dg[0][1][1] = r * (2.0 * dH_dth);
+            dg[1][0][1] = dg[0][1][1];
+            dg[0][3][1] = -2.0 * a * (s2 * dH_dth + 2.0 * H * s * c);
+            dg[3][0][1] = dg[0][3][1];
+            dg[1][1][1] = (r * r) * (2.0 * dH_dth);
+            dg[1][3][1] = r * (-a * ((1.0 + 2.0 * H) * 2.0 * s * c + s2 * 2.0 * dH_dth));
+            dg[3][1][1] = dg[1][3][1];
+            dg[2][2][1] = dS_dth;
+            dg[3][3][1] = 2.0 * (r * r + a * a) * s * c + 2.0 * (a * a) * (s2 * s2) * dH_dth +
+                          8.0 * (a * a) * H * s * c * s2;
+        } else {
+            dg[0][0][0] = 2.0 * dH_dr;
+            dg[0][1][0] = 2.0 * dH_dr;
+            dg[1][0][0] = dg[0][1][0];
+            dg[0][3][0] = -2.0 * a * s2 * dH_dr;
+            dg[3][0][0] = dg[0][3][0];
+            dg[1][1][0] = 2.0 * dH_dr;
+            dg[1][3][0] = -a * s2 * (2.0 * dH_dr);
+            dg[3][1][0] = dg[1][3][0];
+            dg[2][2][0] = dS_dr;
+            dg[3][3][0] = 2.0 * r * s2 + 2.0 * (a * a) * (s2 * s2) * dH_dr;
+            dg[0][0][1] = 2.0 * dH_dth;
+            dg[0][1][1] = 2.0 * dH_dth;
+            dg[1][0][1] = dg[0][1][1];
+            dg[0][3][1] = -2.0 * a * (s2 * dH_dth + 2.0 * H * s * c);
+            dg[3][0][1] = dg[0][3][1];
+            dg[1][1][1] = 2.0 * dH_dth;
+            dg[1][3][1] = -a * ((1.0 + 2.0 * H) * 2.0 * s * c + s2 * 2.0 * dH_dth);
+            dg[3][1][1] = dg[1][3][1];
+            dg[2][2][1] = dS_dth;
+            dg[3][3][1] = 2.0 * (r * r + a * a) * s * c + 2.0 * (a * a) * (s2 * s2) * dH_dth +
+                          8.0 * (a * a) * H * s * c * s2;
+        }
+        if (P->c.ny == 1) { for (int i = 0; i < 4; i++) for (int j = 0; j < 4; j++) dg[i][j][1] = 0.0; }
+    }
+}
+
+/* Christoffel symbols of Kerr-Schild by contraction over the non-zero
+ * derivative directions r (1) and theta (2) (metrics.f90:926-990, 1186-1242). */
+static void chris_ks(const double gm[4][4], const double dg[4][4][3], double ch[4][4][4])
+{
+    memset(ch, 0, sizeof(double) * 64);
+    for (int l = 0; l < 4; l++)
+        for (int mu = 0; mu < 4; mu++)
+            for (int nu = mu; nu < 4; nu++) {
+                for (int rho = 0; rho < 4; rho++) {
+                    double dmu = 0.0, dnu = 0.0, drho = 0.0;
+                    if (mu == 1 || mu == 2) dmu = dg[rho][nu][mu - 1];
+                    if (nu == 1 || nu == 2) dnu = dg[rho][mu][nu - 1];
+                    if (rho == 1 || rho == 2) drho = dg[mu][nu][rho - 1];
+                    ch[l][mu][nu] = ch[l][mu][nu] + 0.5 * gm[l][rho] * (dmu + dnu - drho);
+                }
+                if (mu != nu) ch[l][nu][mu] = ch[l][mu][nu];
+            }
+}
+
+/* Dispatcher standing in for the procedure pointers bound by set_metric_type
+ * (metrics.f90:94-201).  ch/dg may be NULL. */
+static void metric_eval(const orc_problem *P, double xp, double yp, Met *m, double ch[4][4][4],
+                        double dg[4][4][3])
+{
+    switch (P->c.metric) {
+    case ORC_MINKOWSKI:
+        if (P->c.geom == ORC_CARTESIAN) {
+            met_mink_cart(m);
+            if (ch) memset(ch, 0, sizeof(double) * 64);
+            if (dg) memset(dg, 0, sizeof(double) * 48);
+        } else if (P->c.geom == ORC_CYLINDRICAL) {
+            met_mink_cyl(xp, m, ch, dg);
+        } else if (P->c.log_r) {
+            met_mink_sph_log(P, xp, yp, m, ch, dg);
+        } else {
+            met_mink_sph_phys(P, xp, yp, m, ch, dg);
+        }
+        break;
+    case ORC_EF:
+        if (P->c.log_r) met_ef_log(P, xp, yp, m, ch, dg);
+        else met_ef_phys(P, xp, yp, m, ch, dg);
+        break;
+    default: {
+        double dgl[4][4][3];
+        met_ks(P, xp, yp, P->c.log_r, m, dgl);
+        if (dg) memcpy(dg, dgl, sizeof(dgl));
+        if (ch) chris_ks(m->gm, (const double(*)[4][3])dgl, ch);
+        break;
+    }
+    }
+}
+
+/* Centre quantities as stored by precalculate_metric_cache
+ * (initialization.f90:283-317): gamma^ij by the ADM relation, sqrt(det). */
+typedef struct {
+    Met m;
+    double ginv[3][3];
+    double sqg;
+} Centre;
+
+static void centre_eval(const orc_problem *P, int i, int j, Centre *C, double ch[4][4][4],
+                        double dg[4][4][3])
+{
+    metric_eval(P, XC(i), YC(j), &C->m, ch, dg);
+    for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++)
+            C->ginv[a][b] = C->m.gm[a + 1][b + 1] +
+                            (C->m.beta[a] * C->m.beta[b]) / (C->m.alpha * C->m.alpha);
+    C->sqg = sqrt(C->m.det);
+}
+
+/* ------------------------------------------------------------- equations */
+/* Shared prelude of prim_to_cons, calc_fluxes, prim_to_cons_face and the wave
+ * speeds: v^2 = gamma_ij v^i v^j, causal clamp to v_max, v_i = gamma_ij v^j
+ * (equations.f90:49-71). */
+static double vsq_clamp(const double g[3][3], double v[3])
+{
+    double vsq = 0.0;
+    for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++) vsq = vsq + g[a][b] * v[a] * v[b];
+    if (vsq >= V_MAX) {
+        v[0] = v[0] * sqrt(V_MAX / vsq);
+        v[1] = v[1] * sqrt(V_MAX / vsq);
+        v[2] = v[2] * sqrt(V_MAX / vsq);
+        vsq = V_MAX;
+    }
+    return vsq;
+}
+static void lower(const double g[3][3], const double v[3], double vc[3])
+{
+    for (int a = 0; a < 3; a++) {
+        vc[a] = 0.0;
+        for (int b = 0; b < 3; b++) vc[a] = vc[a] + g[a][b] * v[b];
+    }
+}
+
+/* prim_to_cons (equations.f90:30-88) and prim_to_cons_face (fluxes.f90:76-122). */
+static void p2c(const orc_problem *P, const double *q, const double g[3][3], double sqg,
+                double *U)
+{
+    double v[3] = {q[EQ_VX], q[EQ_VY], q[EQ_VZ]}, vc[3];
+    const double vsq = vsq_clamp(g, v);
+    lower(g, v, vc);
+    const double h = 1.0 + P->g1 * q[EQ_PR] / q[EQ_DE];
+    const double W = 1.0 / sqrt(1.0 - vsq);
+    U[EQ_DE] = q[EQ_DE] * W;
+    U[EQ_PR] = q[EQ_DE] * h * (W * W) - q[EQ_PR] - q[EQ_DE] * W;
+    U[EQ_VX] = q[EQ_DE] * h * (W * W) * vc[0];
+    U[EQ_VY] = q[EQ_DE] * h * (W * W) * vc[1];
+    U[EQ_VZ] = q[EQ_DE] * h * (W * W) * vc[2];
+    for (int e = 0; e < NEQ; e++) U[e] = U[e] * sqg;
+}
+
+/* calc_fluxes (equations.f90:434-502). dir is 1..3. */
+static void phys_flux(const orc_problem *P, const double *q, int dir, double alpha,
+                      const double beta[3], const double g[3][3], double sqg, double *F)
+{
+    double v[3] = {q[EQ_VX], q[EQ_VY], q[EQ_VZ]}, vc[3];
+    const double vsq = vsq_clamp(g, v);
+    lower(g, v, vc);
+    const double vn = v[dir - 1], bn = beta[dir - 1];
+    const double h = 1.0 + P->g1 * q[EQ_PR] / q[EQ_DE];
+    const double W = 1.0 / sqrt(1.0 - vsq);
+    const double veff = alpha * vn - bn;
+    F[EQ_DE] = q[EQ_DE] * W * veff;
+    F[EQ_PR] = q[EQ_DE] * h * (W * W) * veff + q[EQ_PR] * bn - F[EQ_DE];
+    F[EQ_VX] = q[EQ_DE] * h * (W * W) * vc[0] * veff + (dir == 1 ? alpha * q[EQ_PR] : 0.0);
+    F[EQ_VY] = q[EQ_DE] * h * (W * W) * vc[1] * veff + (dir == 2 ? alpha * q[EQ_PR] : 0.0);
+    F[EQ_VZ] = q[EQ_DE] * h * (W * W) * vc[2] * veff + (dir == 3 ? alpha * q[EQ_PR] : 0.0);
+    for (int e = 0; e < NEQ; e++) F[e] = F[e] * sqg;
+}
+
+/* Wave speeds: calc_wavespeeds_srhd (fluxes.f90:669-720) and
+ * calc_wavespeeds_grhd (fluxes.f90:724-788). */
+static void wave_speeds(const orc_problem *P, const double *q, int dir, double alpha,
+                        const double beta[3], const double g[3][3], double *wmin, double *wmax)
+{
+    double v[3] = {q[EQ_VX], q[EQ_VY], q[EQ_VZ]};
+    const double vsq = vsq_clamp(g, v);
+    const double enth = 1.0 + P->g1 * q[EQ_PR] / q[EQ_DE];
+    const double cs2 = P->c.gam * q[EQ_PR] / (q[EQ_DE] * enth);
+    const double cs = sqrt(cs2);
+    const double vn = v[dir - 1];
+    double e1, e2, e3;
+    if (P->c.metric == ORC_MINKOWSKI && P->c.geom == ORC_CARTESIAN) {
+        const double t1 = 1.0 / (1.0 - vsq * cs2);
+        const double t2 = cs * sqrt((1.0 - vsq) * (1.0 / t1 - (vn * vn) * (1.0 - cs2)));
+        e1 = vn;
+        e2 = t1 * (vn * (1.0 - cs2) + t2);
+        e3 = t1 * (vn * (1.0 - cs2) - t2);
+    } else {
+        const double det = g[0][0] * (g[1][1] * g[2][2] - g[1][2] * g[2][1]) -
+                           g[0][1] * (g[1][0] * g[2][2] - g[1][2] * g[2][0]) +
+                           g[0][2] * (g[1][0] * g[2][1] - g[1][1] * g[2][0]);
+        double gii;
+        if (dir == 1) gii = (g[1][1] * g[2][2] - g[1][2] * g[2][1]) / det;
+        else if (dir == 2) gii = (g[0][0] * g[2][2] - g[0][2] * g[2][0]) / det;
+        else gii = (g[0][0] * g[1][1] - g[0][1] * g[1][0]) / det;
+        const double bn = beta[dir - 1];
+        const double t1 = alpha / (1.0 - vsq * cs2);
+        const double t2 =
+            cs * sqrt(dmax(0.0, (1.0 - vsq) * (gii * (1.0 - vsq * cs2) - (vn * vn) * (1.0 - cs2))));
+        e1 = alpha * vn - bn;
+        e2 = t1 * (vn * (1.0 - cs2) + t2) - bn;
+        e3 = t1 * (vn * (1.0 - cs2) - t2) - bn;
+    }
+    *wmax = dmax(dmax(dmax(0.0, e1), e2), e3);
+    *wmin = dmin(dmin(dmin(0.0, e1), e2), e3);
+}
+
+/* calc_hlle_fluxes (fluxes.f90:125-156). */
+static void hlle(const orc_problem *P, const double *qL, const double *qR, int dir, double alpha,
+                 const double beta[3], const double g[3][3], double sqg, double *F)
+{
+    double fL[NEQ], fR[NEQ], uL[NEQ], uR[NEQ], lLmin, lLmax, lRmin, lRmax;
+    phys_flux(P, qL, dir, alpha, beta, g, sqg, fL);
+    phys_flux(P, qR, dir, alpha, beta, g, sqg, fR);
+    p2c(P, qL, g, sqg, uL);
+    p2c(P, qR, g, sqg, uR);
+    wave_speeds(P, qL, dir, alpha, beta, g, &lLmin, &lLmax);
+    wave_speeds(P, qR, dir, alpha, beta, g, &lRmin, &lRmax);
+    const double ap = dmax(dmax(0.0, lLmax), lRmax);
+    const double am = dmin(dmin(0.0, lLmin), lRmin);
+    if (fabs(ap - am) < 1.0e-14) {
+        for (int e = 0; e < NEQ; e++) F[e] = 0.0;
+    } else {
+        for (int e = 0; e < NEQ; e++)
+            F[e] = (ap * fL[e] - am * fR[e] + ap * am * (uR[e] - uL[e])) / (ap - am);
+    }
+}
+
+/* ------------------------------------------------------------------ HLLC */
+static double metric_dot(const double a[3], const double b[3], const double g[3][3])
+{
+    double s = 0.0;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) s = s + g[i][j] * a[i] * b[j];
+    return s;
+}
+
+/* build_face_orthonormal_frame (fluxes.f90:402-473).  frame[i][a] = e_(a)^i,
+ * coframe[a][i] = e^(a)_i. */
+static int face_frame(const double g[3][3], int dir, double frame[3][3], double coframe[3][3],
+                      double *gdd)
+{
+    double ginv[3][3];
+    memset(frame, 0, sizeof(double) * 9);
+    memset(coframe, 0, sizeof(double) * 9);
+    *gdd = 0.0;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++)
+            if (!isfinite(g[i][j])) return 0;
+    const double det = g[0][0] * (g[1][1] * g[2][2] - g[1][2] * g[2][1]) -
+                       g[0][1] * (g[1][0] * g[2][2] - g[1][2] * g[2][0]) +
+                       g[0][2] * (g[1][0] * g[2][1] - g[1][1] * g[2][0]);
+    if (!isfinite(det) || det <= 1.0e-30) return 0;
+    ginv[0][0] = (g[1][1] * g[2][2] - g[1][2] * g[2][1]) / det;
+    ginv[0][1] = (g[0][2] * g[2][1] - g[0][1] * g[2][2]) / det;
+    ginv[0][2] = (g[0][1] * g[1][2] - g[0][2] * g[1][1]) / det;
+    ginv[1][0] = (g[1][2] * g[2][0] - g[1][0] * g[2][2]) / det;
+    ginv[1][1] = (g[0][0] * g[2][2] - g[0][2] * g[2][0]) / det;
+    ginv[1][2] = (g[0][2] * g[1][0] - g[0][0] * g[1][2]) / det;
+    ginv[2][0] = (g[1][0] * g[2][1] - g[1][1] * g[2][0]) / det;
+    ginv[2][1] = (g[0][1] * g[2][0] - g[0][0] * g[2][1]) / det;
+    ginv[2][2] = (g[0][0] * g[1][1] - g[0][1] * g[1][0]) / det;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++)
+            if (!isfinite(ginv[i][j])) return 0;
+    const int d = dir - 1;
+    *gdd = ginv[d][d];
+    if (*gdd <= 1.0e-30 || !isfinite(*gdd)) return 0;
+    for (int i = 0; i < 3; i++) frame[i][0] = ginv[i][d] / sqrt(*gdd);
+    int tang[2];
+    if (dir == 1) { tang[0] = 1; tang[1] = 2; }
+    else if (dir == 2) { tang[0] = 2; tang[1] = 0; }
+    else { tang[0] = 0; tang[1] = 1; }
+    for (int b = 1; b < 3; b++) {
+        double seed[3] = {0.0, 0.0, 0.0};
+        seed[tang[b - 1]] = 1.0;
+        for (int a = 0; a < b; a++) {
+            double fa[3] = {frame[0][a], frame[1][a], frame[2][a]};
+            const double proj = metric_dot(seed, fa, g);
+            for (int i = 0; i < 3; i++) seed[i] = seed[i] - proj * fa[i];
+        }
+        const double nsq = metric_dot(seed, seed, g);
+        if (!isfinite(nsq) || nsq <= 1.0e-28) return 0;
+        for (int i = 0; i < 3; i++) frame[i][b] = seed[i] / sqrt(nsq);
+    }
+    for (int a = 0; a < 3; a++)
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) coframe[a][i] = coframe[a][i] + g[i][j] * frame[j][a];
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++)
+            if (!isfinite(frame[i][j]) || !isfinite(coframe[i][j])) return 0;
+    return 1;
+}
+
+/* calc_srhd_local_state (fluxes.f90:509-568). */
+static int srhd_local(const orc_problem *P, const double *ql, double *U, double *F, double *wmin,
+                      double *wmax)
+{
+    for (int e = 0; e < NEQ; e++) { U[e] = 0.0; F[e] = 0.0; }
+    *wmin = 0.0;
+    *wmax = 0.0;
+    const double rho = ql[EQ_DE], pr = ql[EQ_PR];
+    const double v[3] = {ql[EQ_VX], ql[EQ_VY], ql[EQ_VZ]};
+    if (!isfinite(rho) || !isfinite(pr) || !isfinite(v[0]) || !isfinite(v[1]) || !isfinite(v[2]) ||
+        rho <= 0.0 || pr < 0.0)
+        return 0;
+    const double vsq = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+    if (!isfinite(vsq) || vsq < 0.0) return 0;
+    if (vsq >= V_MAX) return 0;
+    const double h = 1.0 + P->g1 * pr / rho;
+    if (!isfinite(h) || h <= 0.0) return 0;
+    const double cs2 = P->c.gam * pr / (rho * h);
+    if (!isfinite(cs2) || cs2 < 0.0 || cs2 >= 1.0) return 0;
+    const double W = 1.0 / sqrt(1.0 - vsq);
+    const double D = rho * W;
+    const double tau = rho * h * (W * W) - pr - D;
+    double mom[3];
+    for (int i = 0; i < 3; i++) mom[i] = rho * h * (W * W) * v[i];
+    U[EQ_DE] = D;
+    U[EQ_PR] = tau;
+    U[EQ_VX] = mom[0];
+    U[EQ_VY] = mom[1];
+    U[EQ_VZ] = mom[2];
+    F[EQ_DE] = D * v[0];
+    F[EQ_PR] = mom[0] - D * v[0];
+    F[EQ_VX] = mom[0] * v[0] + pr;
+    F[EQ_VY] = mom[1] * v[0];
+    F[EQ_VZ] = mom[2] * v[0];
+    const double den = 1.0 - vsq * cs2;
+    const double rad = (1.0 - vsq) * (1.0 - vsq * cs2 - (v[0] * v[0]) * (1.0 - cs2));
+    if (den <= 1.0e-14 || rad < -1.0e-12) return 0;
+    const double ac = sqrt(cs2) * sqrt(dmax(0.0, rad));
+    *wmin = (v[0] * (1.0 - cs2) - ac) / den;
+    *wmax = (v[0] * (1.0 - cs2) + ac) / den;
+    for (int e = 0; e < NEQ; e++)
+        if (!isfinite(U[e]) || !isfinite(F[e])) return 0;
+    if (!isfinite(*wmin) || !isfinite(*wmax)) return 0;
+    return 1;
+}
+
+/* conservative_state_is_admissible (fluxes.f90:571-591). */
+static int admissible(const double *U)
+{
+    for (int e = 0; e < NEQ; e++)
+        if (!isfinite(U[e])) return 0;
+    if (U[EQ_DE] <= 0.0) return 0;
+    const double E = U[EQ_PR] + U[EQ_DE];
+    const double msq = U[EQ_VX] * U[EQ_VX] + U[EQ_VY] * U[EQ_VY] + U[EQ_VZ] * U[EQ_VZ];
+    const double scale = dmax(dmax(DBL_MIN, E * E), U[EQ_DE] * U[EQ_DE] + msq);
+    if (E <= 0.0) return 0;
+    if (E * E - U[EQ_DE] * U[EQ_DE] - msq <= 1.0e-13 * scale) return 0;
+    return 1;
+}
+
+/* local_flux_to_coordinate (fluxes.f90:596-624). */
+static int local_to_coord(const double *Ul, const double *Fl, double w, double alpha, double sqg,
+                          double gdd, const double coframe[3][3], double *F)
+{
+    double mf[NEQ];
+    for (int e = 0; e < NEQ; e++) { F[e] = 0.0; mf[e] = Fl[e] - w * Ul[e]; }
+    const double fac = sqg * alpha * sqrt(gdd);
+    if (!isfinite(fac)) return 0;
+    for (int e = 0; e < NEQ; e++)
+        if (!isfinite(mf[e])) return 0;
+    F[EQ_DE] = fac * mf[EQ_DE];
+    F[EQ_PR] = fac * mf[EQ_PR];
+    for (int i = 0; i < 3; i++)
+        for (int a = 0; a < 3; a++) F[EQ_VX + i] = F[EQ_VX + i] + fac * coframe[a][i] * mf[EQ_VX + a];
+    for (int e = 0; e < NEQ; e++)
+        if (!isfinite(F[e])) return 0;
+    return 1;
+}
+
+/* calc_hllc_fluxes (fluxes.f90:174-397): Mignone-Bodo HLLC in a local
+ * orthonormal face frame, HLLE fallback on every degeneracy. */
+static void hllc(const orc_problem *P, const double *qL, const double *qR, int dir, double alpha,
+                 const double beta[3], const double g[3][3], double sqg, double *F)
+{
+    double frame[3][3], cof[3][3], gdd;
+    double qLh[NEQ], qRh[NEQ], uL[NEQ], uR[NEQ], fL[NEQ], fR[NEQ];
+    double lLmin, lLmax, lRmin, lRmax;
+    if (dir < 1 || dir > 3 || alpha <= 0.0 || sqg <= 0.0 || !isfinite(alpha) || !isfinite(sqg))
+        goto fallback;
+    if (!face_frame(g, dir, frame, cof, &gdd)) goto fallback;
+    /* primitives_to_local_frame (fluxes.f90:490-504) */
+    for (int e = 0; e < NEQ; e++) { qLh[e] = qL[e]; qRh[e] = qR[e]; }
+    for (int a = 0; a < 3; a++) {
+        qLh[EQ_VX + a] = 0.0;
+        qRh[EQ_VX + a] = 0.0;
+        for (int i = 0; i < 3; i++) {
+            qLh[EQ_VX + a] = qLh[EQ_VX + a] + cof[a][i] * qL[EQ_VX + i];
+            qRh[EQ_VX + a] = qRh[EQ_VX + a] + cof[a][i] * qR[EQ_VX + i];
+        }
+    }
+    if (!srhd_local(P, qLh, uL, fL, &lLmin, &lLmax)) goto fallback;
+    if (!srhd_local(P, qRh, uR, fR, &lRmin, &lRmax)) goto fallback;
+    {
+        const double ll = dmin(lLmin, lRmin), lr = dmax(lLmax, lRmax);
+        const double w = beta[dir - 1] / (alpha * sqrt(gdd));
+        if (!isfinite(w) || lr - ll <= 1.0e-13) goto fallback;
+        if (w <= ll) {
+            if (!local_to_coord(uL, fL, w, alpha, sqg, gdd, cof, F)) goto fallback;
+            return;
+        } else if (w >= lr) {
+            if (!local_to_coord(uR, fR, w, alpha, sqg, gdd, cof, F)) goto fallback;
+            return;
+        }
+        double uh[NEQ], fh[NEQ];
+        double den = lr - ll;
+        for (int e = 0; e < NEQ; e++) {
+            uh[e] = (lr * uR[e] - ll * uL[e] + fL[e] - fR[e]) / den;
+            fh[e] = (lr * fL[e] - ll * fR[e] + lr * ll * (uR[e] - uL[e])) / den;
+        }
+        for (int e = 0; e < NEQ; e++)
+            if (!isfinite(uh[e]) || !isfinite(fh[e])) goto fallback;
+        const double Eh = uh[EQ_PR] + uh[EQ_DE], FEh = fh[EQ_PR] + fh[EQ_DE];
+        const double Mh = uh[EQ_VX], FMh = fh[EQ_VX];
+        const double A = FEh, B = -(Eh + FMh), Cq = Mh;
+        double disc = B * B - 4.0 * A * Cq;
+        const double dscale = dmax(dmax(DBL_MIN, B * B), fabs(4.0 * A * Cq));
+        if (!isfinite(disc) || disc < -1.0e-12 * dscale) goto fallback;
+        disc = dmax(0.0, disc);
+        const double cscale = dmax(dmax(DBL_MIN, fabs(B)), fabs(Cq));
+        const int quad = fabs(A) > 1.0e-13 * cscale;
+        double ls;
+        if (!quad) {
+            if (fabs(B) <= 1.0e-14 * dmax(DBL_MIN, fabs(Cq))) goto fallback;
+            ls = -Cq / B;
+        } else {
+            den = -B - sqrt(disc);
+            if (fabs(den) > 1.0e-14 * dmax(dmax(DBL_MIN, fabs(B)), sqrt(disc))) {
+                ls = den / (2.0 * A);
+            } else {
+                den = -B + sqrt(disc);
+                if (fabs(den) <= 1.0e-14 * dmax(dmax(DBL_MIN, fabs(B)), sqrt(disc))) goto fallback;
+                ls = (2.0 * Cq) / den;
+            }
+        }
+        const double rtol = 1.0e-10 * dmax(dmax(1.0, fabs(ll)), fabs(lr));
+        int valid = isfinite(ls) && ls >= ll - rtol && ls <= lr + rtol && fabs(ls) <= 1.0 + rtol;
+        if (!valid && quad) {
+            const double ra = (-B + sqrt(disc)) / (2.0 * A);
+            if (isfinite(ra) && ra >= ll - rtol && ra <= lr + rtol && fabs(ra) <= 1.0 + rtol) {
+                ls = ra;
+                valid = 1;
+            }
+        }
+        if (!valid) goto fallback;
+        ls = dmin(lr, dmax(ll, ls));
+        const double AL = ll * (uL[EQ_PR] + uL[EQ_DE]) - uL[EQ_VX];
+        const double BL = uL[EQ_VX] * (ll - qLh[EQ_VX]) - qLh[EQ_PR];
+        const double AR = lr * (uR[EQ_PR] + uR[EQ_DE]) - uR[EQ_VX];
+        const double BR = uR[EQ_VX] * (lr - qRh[EQ_VX]) - qRh[EQ_PR];
+        den = 1.0 - ll * ls;
+        if (fabs(den) <= 1.0e-13) goto fallback;
+        const double psL = (AL * ls - BL) / den;
+        den = 1.0 - lr * ls;
+        if (fabs(den) <= 1.0e-13) goto fallback;
+        const double psR = (AR * ls - BR) / den;
+        const double sscale = dmax(dmax(dmax(dmax(DBL_MIN, fabs(uL[EQ_PR] + uL[EQ_DE])),
+                                             fabs(uR[EQ_PR] + uR[EQ_DE])),
+                                        fabs(uL[EQ_VX])),
+                                   fabs(uR[EQ_VX]));
+        const double ptol = 1.0e-8 * dmax(dmax(dmax(dmax(DBL_MIN, fabs(qLh[EQ_PR])), fabs(qRh[EQ_PR])),
+                                              fabs(psL)),
+                                         fabs(psR)) +
+                            1.0e2 * DBL_EPSILON * sscale;
+        if (!isfinite(psL) || !isfinite(psR) || fabs(psL - psR) > ptol) goto fallback;
+        const double ps = 0.5 * (psL + psR);
+        if (!isfinite(ps) || fabs(ll - ls) <= 1.0e-13 || fabs(lr - ls) <= 1.0e-13) goto fallback;
+        double pv[NEQ] = {0.0, ls, 1.0, 0.0, 0.0};
+        double usL[NEQ], usR[NEQ], fsL[NEQ], fsR[NEQ];
+        for (int e = 0; e < NEQ; e++) {
+            usL[e] = (ll * uL[e] - fL[e] + ps * pv[e]) / (ll - ls);
+            usR[e] = (lr * uR[e] - fR[e] + ps * pv[e]) / (lr - ls);
+        }
+        for (int e = 0; e < NEQ; e++) {
+            fsL[e] = fL[e] + ll * (usL[e] - uL[e]);
+            fsR[e] = fR[e] + lr * (usR[e] - uR[e]);
+        }
+        if (!admissible(usL) || !admissible(usR)) goto fallback;
+        for (int e = 0; e < NEQ; e++)
+            if (!isfinite(fsL[e]) || !isfinite(fsR[e])) goto fallback;
+        const double *us = (w <= ls) ? usL : usR;
+        const double *fs = (w <= ls) ? fsL : fsR;
+        if (!local_to_coord(us, fs, w, alpha, sqg, gdd, cof, F)) goto fallback;
+        return;
+    }
+fallback:
+    hlle(P, qL, qR, dir, alpha, beta, g, sqg, F);
+}
+
+/* resolve_riemann_problem (fluxes.f90:56-72). */
+static void riemann(const orc_problem *P, const double *qL, const double *qR, int dir,
+                    double alpha, const double beta[3], const double g[3][3], double sqg, double *F)
+{
+    if (P->c.rs == ORC_RS_HLLC) hllc(P, qL, qR, dir, alpha, beta, g, sqg, F);
+    else hlle(P, qL, qR, dir, alpha, beta, g, sqg, F);
+}
+
+/* ----------------------------------------------------------- reconstruction */
+static inline double minmod2(double a, double b)
+{   /* reconstruction.f90:28-32 */
+    return 0.5 * (sgn1(a) + sgn1(b)) * dmin(fabs(a), fabs(b));
+}
+static inline double minmod3(double a, double b, double c)
+{   /* reconstruction.f90:34-38 */
+    return sgn1(a) * dmax(0.0, dmin(dmin(fabs(a), sgn1(a) * b), sgn1(a) * c));
+}
+static inline double minmod4(double a, double b, double c, double d)
+{   /* reconstruction.f90:40-45 */
+    return 0.125 * (sgn1(a) + sgn1(b)) * fabs(sgn1(a) + sgn1(c)) * fabs(sgn1(a) + sgn1(d)) *
+           dmin(dmin(dmin(fabs(a), fabs(b)), fabs(c)), fabs(d));
+}
+/* weno3_interface, "dynamic epsilon" variant (reconstruction.f90:141-181).
+ * q points at the centre value: q[-1], q[0], q[1]. */
+static double weno3(double qm1, double q0, double qp1)
+{
+    const double d0 = 2.0 / 3.0, d1 = 1.0 / 3.0, eps_min = 1.0e-14;
+    const double p0 = 0.5 * q0 + 0.5 * qp1;
+    const double p1 = -0.5 * qm1 + 1.5 * q0;
+    const double b0 = (q0 - qp1) * (q0 - qp1);
+    const double b1 = (qm1 - q0) * (qm1 - q0);
+    const double sens = fabs(b0 - b1) / (dmax(b0, b1) + eps_min);
+    const double epsh = eps_min + (qm1 * qm1 + q0 * q0 + qp1 * qp1) * exp(-10.0 * sens);
+    const double a0 = d0 / ((epsh + b0) * (epsh + b0));
+    const double a1 = d1 / ((epsh + b1) * (epsh + b1));
+    const double w0 = a0 / (a0 + a1), w1 = a1 / (a0 + a1);
+    return w0 * p0 + w1 * p1;
+}
+/* weno5_interface, WENO5-Z (reconstruction.f90:187-215). */
+static double weno5(double qm2, double qm1, double q0, double qp1, double qp2)
+{
+    const double eps = 1.0e-14, d0 = 0.3, d1 = 0.6, d2 = 0.1;
+    const double t0 = q0 - 2.0 * qp1 + qp2, u0 = 3.0 * q0 - 4.0 * qp1 + qp2;
+    const double t1 = qm1 - 2.0 * q0 + qp1, u1 = qm1 - qp1;
+    const double t2 = qm2 - 2.0 * qm1 + q0, u2 = qm2 - 4.0 * qm1 + 3.0 * q0;
+    const double b0 = 13.0 / 12.0 * (t0 * t0) + 0.25 * (u0 * u0);
+    const double b1 = 13.0 / 12.0 * (t1 * t1) + 0.25 * (u1 * u1);
+    const double b2 = 13.0 / 12.0 * (t2 * t2) + 0.25 * (u2 * u2);
+    const double tau5 = fabs(b0 - b2);
+    const double a0 = d0 * (1.0 + (tau5 / (eps + b0)));
+    const double a1 = d1 * (1.0 + (tau5 / (eps + b1)));
+    const double a2 = d2 * (1.0 + (tau5 / (eps + b2)));
+    const double w0 = a0 / (a0 + a1 + a2), w1 = a1 / (a0 + a1 + a2), w2 = a2 / (a0 + a1 + a2);
+    const double p0 = (1.0 / 3.0) * q0 + (5.0 / 6.0) * qp1 - (1.0 / 6.0) * qp2;
+    const double p1 = -(1.0 / 6.0) * qm1 + (5.0 / 6.0) * q0 + (1.0 / 3.0) * qp1;
+    const double p2 = (1.0 / 3.0) * qm2 - (7.0 / 6.0) * qm1 + (11.0 / 6.0) * q0;
+    return w0 * p0 + w1 * p1 + w2 * p2;
+}
+/* mp5_interface, Suresh-Huynh MP5 (reconstruction.f90:222-254). */
+static double mp5(double qm2, double qm1, double q0, double qp1, double qp2)
+{
+    const double alpha = 4.0, eps = 1.0e-10;
+    const double vL = (2.0 * qm2 - 13.0 * qm1 + 47.0 * q0 + 27.0 * qp1 - 3.0 * qp2) / 60.0;
+    const double vmp = q0 + minmod2(qp1 - q0, alpha * (q0 - qm1));
+    if ((vL - q0) * (vL - vmp) < eps) return vL;
+    const double djm1 = qm2 - 2.0 * qm1 + q0;
+    const double dj = qm1 - 2.0 * q0 + qp1;
+    const double djp1 = q0 - 2.0 * qp1 + qp2;
+    const double dm4p = minmod4(4.0 * dj - djp1, 4.0 * djp1 - dj, dj, djp1);
+    const double dm4m = minmod4(4.0 * dj - djm1, 4.0 * djm1 - dj, dj, djm1);
+    const double vul = q0 + alpha * (q0 - qm1);
+    const double vav = 0.5 * (q0 + qp1);
+    const double vmd = vav - 0.5 * dm4p;
+    const double vlc = q0 + 0.5 * (q0 - qm1) + 4.0 * dm4m / 3.0;
+    const double vmin = dmax(dmin(dmin(q0, qp1), vmd), dmin(dmin(q0, vul), vlc));
+    const double vmax = dmin(dmax(dmax(q0, qp1), vmd), dmax(dmax(q0, vul), vlc));
+    return vL + minmod2(vmin - vL, vmax - vL);
+}
+/* tvd_slope (reconstruction.f90:259-278). */
+static double tvd_slope(double a, double b, int lim)
+{
+    switch (lim) {
+    case ORC_LIM_SUPERBEE:
+        if (a * b <= 0.0) return 0.0;
+        return sgn1(a) * dmax(dmin(2.0 * fabs(a), fabs(b)), dmin(fabs(a), 2.0 * fabs(b)));
+    case ORC_LIM_MC:
+        return minmod3(2.0 * b, 0.5 * (a + b), 2.0 * a);
+    default:
+        return minmod2(a, b);
+    }
+}
+
+/* reconstruct_1d_core (reconstruction.f90:285-657).  s(v, i) for i in
+ * [-ng, nodes+ng]; qL/qR for faces 0..nodes.  Faces are independent, so this
+ * also serves a single face. */
+#define S(v, i) s[((size_t)((i) + ng)) * NEQ + (v)]
+static void recon_face(const orc_problem *P, const double *s, int ng, int nodes, int dir, int i,
+                       double *qL, double *qR)
+{
+    const int rec = P->c.rec, mink = (P->c.metric == ORC_MINKOWSKI);
+    const double rf = P->rho_floor, pf = P->p_floor;
+    if (rec == ORC_REC_GODUNOV) {
+        for (int v = 0; v < NEQ; v++) { qL[v] = S(v, i); qR[v] = S(v, i + 1); }
+    } else if (P->c.sensor && rec != ORC_REC_GODUNOV) {
+        const double pt_tvd = 0.80, rt_tvd = 0.50, pt_weno = 0.40, rt_weno = 0.15;
+        const double pt_god = 10.0, rt_god = 10.0, atm = 1.0e2;
+        double pmax = S(EQ_PR, i - 2), pmin = S(EQ_PR, i - 2);
+        double rmax = S(EQ_DE, i - 2), rmin = S(EQ_DE, i - 2);
+        for (int o = -1; o <= 2; o++) {
+            pmax = dmax(pmax, S(EQ_PR, i + o)); pmin = dmin(pmin, S(EQ_PR, i + o));
+            rmax = dmax(rmax, S(EQ_DE, i + o)); rmin = dmin(rmin, S(EQ_DE, i + o));
+        }
+        const double pj = (pmax > 1.0e3 * pf) ? (pmax - pmin) / dmax(pmin, pf) : 0.0;
+        const double rj = (rmax > 1.0e3 * rf) ? (rmax - rmin) / dmax(rmin, rf) : 0.0;
+        for (int v = 0; v < NEQ; v++) {
+            if (!mink && dir == 1 && i <= 0) {
+                qL[v] = S(v, i); qR[v] = S(v, i + 1);
+            } else if (S(EQ_DE, i) <= atm * rf || S(EQ_DE, i + 1) <= atm * rf) {
+                qL[v] = S(v, i); qR[v] = S(v, i + 1);
+            } else if (pj >= pt_god || rj >= rt_god) {
+                qL[v] = S(v, i); qR[v] = S(v, i + 1);
+            } else if (rec == ORC_REC_TVD) {
+                qL[v] = S(v, i) + 0.5 * tvd_slope(S(v, i) - S(v, i - 1), S(v, i + 1) - S(v, i), P->c.lim);
+                qR[v] = S(v, i + 1) - 0.5 * tvd_slope(S(v, i + 1) - S(v, i), S(v, i + 2) - S(v, i + 1), P->c.lim);
+            } else if (pj >= pt_tvd || rj >= rt_tvd) {
+                qL[v] = S(v, i) + 0.5 * tvd_slope(S(v, i) - S(v, i - 1), S(v, i + 1) - S(v, i), ORC_LIM_MC);
+                qR[v] = S(v, i + 1) - 0.5 * tvd_slope(S(v, i + 1) - S(v, i), S(v, i + 2) - S(v, i + 1), ORC_LIM_MC);
+            } else if (rec == ORC_REC_WENO3) {
+                qL[v] = weno3(S(v, i - 1), S(v, i), S(v, i + 1));
+                qR[v] = weno3(S(v, i + 2), S(v, i + 1), S(v, i));
+            } else if (rec == ORC_REC_MP5) {
+                if (pj >= pt_weno || rj >= rt_weno || (!mink && dir == 1 && i <= 3) ||
+                    (!mink && dir == 2 && (i <= 2 || i >= nodes - 2))) {
+                    qL[v] = weno3(S(v, i - 1), S(v, i), S(v, i + 1));
+                    qR[v] = weno3(S(v, i + 2), S(v, i + 1), S(v, i));
+                } else {
+                    qL[v] = mp5(S(v, i - 2), S(v, i - 1), S(v, i), S(v, i + 1), S(v, i + 2));
+                    qR[v] = mp5(S(v, i + 3), S(v, i + 2), S(v, i + 1), S(v, i), S(v, i - 1));
+                }
+            } else { /* WENO5 */
+                if (pj >= pt_weno || rj >= rt_weno) {
+                    qL[v] = weno3(S(v, i - 1), S(v, i), S(v, i + 1));
+                    qR[v] = weno3(S(v, i + 2), S(v, i + 1), S(v, i));
+                } else {
+                    qL[v] = weno5(S(v, i - 2), S(v, i - 1), S(v, i), S(v, i + 1), S(v, i + 2));
+                    qR[v] = weno5(S(v, i + 3), S(v, i + 2), S(v, i + 1), S(v, i), S(v, i - 1));
+                }
+            }
+        }
+    } else {
+        for (int v = 0; v < NEQ; v++) {
+            if (rec == ORC_REC_TVD) {
+                qL[v] = S(v, i) + 0.5 * tvd_slope(S(v, i) - S(v, i - 1), S(v, i + 1) - S(v, i), P->c.lim);
+                qR[v] = S(v, i + 1) - 0.5 * tvd_slope(S(v, i + 1) - S(v, i), S(v, i + 2) - S(v, i + 1), P->c.lim);
+            } else if (!mink && dir == 1 && i <= 0) {
+                qL[v] = S(v, i); qR[v] = S(v, i + 1);
+            } else if (rec == ORC_REC_WENO3) {
+                qL[v] = weno3(S(v, i - 1), S(v, i), S(v, i + 1));
+                qR[v] = weno3(S(v, i + 2), S(v, i + 1), S(v, i));
+            } else if (rec == ORC_REC_MP5) {
+                qL[v] = mp5(S(v, i - 2), S(v, i - 1), S(v, i), S(v, i + 1), S(v, i + 2));
+                qR[v] = mp5(S(v, i + 3), S(v, i + 2), S(v, i + 1), S(v, i), S(v, i - 1));
+            } else {
+                qL[v] = weno5(S(v, i - 2), S(v, i - 1), S(v, i), S(v, i + 1), S(v, i + 2));
+                qR[v] = weno5(S(v, i + 3), S(v, i + 2), S(v, i + 1), S(v, i), S(v, i - 1));
+            }
+        }
+    }
+    /* positivity floors (reconstruction.f90:647-655) */
+    qL[EQ_DE] = dmax(qL[EQ_DE], rf);
+    qR[EQ_DE] = dmax(qR[EQ_DE], rf);
+    qL[EQ_PR] = dmax(qL[EQ_PR], pf);
+    qR[EQ_PR] = dmax(qR[EQ_PR], pf);
+}
+#undef S
+
+/* ----------------------------------------------------------- con2prim */
+/* eval_V_and_dV (equations.f90:212-220). */
+static void eval_V(const orc_problem *P, double W, const double *c, double *V, double *dV)
+{
+    const double den = P->g1 * (W * W) - 1.0;
+    *V = (P->g1 * (W * W) * (c[EQ_PR] + c[EQ_DE]) - c[EQ_DE] * W) / den;
+    *dV = (P->g1 * W * (c[EQ_DE] * W - 2.0 * (c[EQ_PR] + c[EQ_DE])) + c[EQ_DE]) / (den * den);
+}
+/* eval_residual (equations.f90:223-246). */
+static void eval_res(const orc_problem *P, double W, const double *c, const double gi[3][3],
+                     double *f, double *df)
+{
+    double V, dV, Q = 0.0;
+    eval_V(P, W, c, &V, &dV);
+    for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++) Q = Q + gi[a][b] * c[EQ_VX + a] * c[EQ_VX + b];
+    *f = (V * V) * (1.0 - 1.0 / (W * W)) - Q;
+    *df = 2.0 * V * (dV * (W * W * W - W) + V) / (W * W * W);
+}
+static double w_upper(const orc_problem *P, const double *c, const double gi[3][3], double lo)
+{
+    double s = 0.0;
+    for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++) s = s + gi[a][b] * c[EQ_VX + a] * c[EQ_VX + b];
+    const double wg = (sqrt(s) + c[EQ_DE]) / dmax(c[EQ_DE], P->rho_floor);
+    return dmax(lo, wg * 2.0);
+}
+/* root_brent (equations.f90:343-424). */
+static double root_brent(const orc_problem *P, const double *c, const double gi[3][3])
+{
+    const double x1 = 1.0, xacc = 1.0e-8, EPS = 3.0e-8;
+    const double x2 = w_upper(P, c, gi, 1.0e4);
+    double a, b, cc, d = 0.0, e = 0.0, fa, fb, fc, df;
+    eval_res(P, x1, c, gi, &fa, &df);
+    eval_res(P, x2, c, gi, &fb, &df);
+    if (sgn1(fa) * sgn1(fb) > 0.0) return -1.0;
+    a = x1; b = x2; cc = b; fc = fb;
+    for (int it = 1; it <= 100; it++) {
+        if (sgn1(fb) * sgn1(fc) > 0.0) { cc = a; fc = fa; d = b - a; e = d; }
+        if (fabs(fc) < fabs(fb)) { a = b; b = cc; cc = a; fa = fb; fb = fc; fc = fa; }
+        const double tol1 = 2.0 * EPS * fabs(b) + 0.5 * xacc;
+        const double xm = 0.5 * (cc - b);
+        if (fabs(xm) <= tol1 || fabs(fb) < 1.0e-14) return b;
+        if (fabs(e) >= tol1 && fabs(fa) > fabs(fb)) {
+            double o, q, r;
+            const double y1 = fb / fa;
+            if (fabs(a - cc) < 1.0e-14) {
+                o = 2.0 * xm * y1; q = 1.0 - y1;
+            } else {
+                q = fa / fc; r = fb / fc;
+                o = y1 * (2.0 * xm * q * (q - r) - (b - a) * (r - 1.0));
+                q = (q - 1.0) * (r - 1.0) * (y1 - 1.0);
+            }
+            if (o > 0.0) q = -q;
+            o = fabs(o);
+            if (2.0 * o < dmin(3.0 * xm * q - fabs(tol1 * q), fabs(e * q))) { e = d; d = o / q; }
+            else { d = xm; e = d; }
+        } else {
+            d = xm; e = d;
+        }
+        a = b; fa = fb;
+        if (fabs(d) > tol1) b = b + d;
+        else b = b + copysign(tol1, xm);
+        eval_res(P, b, c, gi, &fb, &df);
+    }
+    return -1.0;
+}
+/* root_safe_newton (equations.f90:249-340): rtsafe on [1, max(1e3, 2 W_guess)]. */
+static double root_newton(const orc_problem *P, const double *c, const double gi[3][3])
+{
+    const double x1 = 1.0, xacc = 1.0e-10;
+    const double x2 = w_upper(P, c, gi, 1.0e3);
+    double xl, xh, f, df, flo, fhi, dx, dxold, root, tmp;
+    int conv = 0;
+    eval_res(P, x1, c, gi, &flo, &df);
+    eval_res(P, x2, c, gi, &fhi, &df);
+    if (fabs(flo) < 1.0e-14) return x1;
+    else if (fabs(fhi) < 1.0e-14) return x2;
+    else if (sgn1(flo) * sgn1(fhi) > 0.0) return -1.0;
+    if (flo < 0.0) { xl = x1; xh = x2; }
+    else { xh = x1; xl = x2; }
+    root = 0.5 * (x1 + x2);
+    dxold = fabs(x2 - x1);
+    dx = dxold;
+    eval_res(P, root, c, gi, &f, &df);
+    for (int it = 1; it <= 100; it++) {
+        if (((root - xh) * df - f) * ((root - xl) * df - f) > 0.0 || fabs(2.0 * f) > fabs(dxold * df)) {
+            dxold = dx;
+            dx = 0.5 * (xh - xl);
+            root = xl + dx;
+            if (fabs(xl - root) < 1.0e-14) return root;
+        } else {
+            dxold = dx;
+            dx = f / df;
+            tmp = root;
+            root = root - dx;
+            if (fabs(tmp - root) < 1.0e-14) return root;
+        }
+        if (fabs(dx) < xacc) { conv = 1; return root; }
+        eval_res(P, root, c, gi, &f, &df);
+        if (f < 0.0) xl = root;
+        else xh = root;
+    }
+    if (!conv || fabs(f) > 1.0e-6) root = root_brent(P, c, gi);
+    return root;
+}
+
+/* cons_to_prim (equations.f90:94-205) with the centre metric of the cell. */
+static void c2p(const orc_problem *P, const Centre *C, double *c, double *q)
+{
+    const double sqg = C->sqg;
+    const double Df = P->rho_floor * sqg, tf = P->p_floor * (P->g1 - 1.0) * sqg;
+    const double(*gi)[3] = C->ginv;
+    const double(*g)[3] = C->m.g;
+    if (c[EQ_DE] <= Df) goto atmosphere;
+    {
+        const double S1 = c[EQ_VX], S2 = c[EQ_VY], S3 = c[EQ_VZ];
+        const double Ssq = S1 * (gi[0][0] * S1 + gi[0][1] * S2 + gi[0][2] * S3) +
+                           S2 * (gi[1][0] * S1 + gi[1][1] * S2 + gi[1][2] * S3) +
+                           S3 * (gi[2][0] * S1 + gi[2][1] * S2 + gi[2][2] * S3);
+        const double E = c[EQ_PR] + c[EQ_DE];
+        const double Smax = dmax(0.0, E * E - c[EQ_DE] * c[EQ_DE]) * V_MAX;
+        if (Ssq > Smax && Ssq > 1.0e-20) {
+            const double sf = sqrt(Smax / Ssq);
+            c[EQ_VX] = c[EQ_VX] * sf;
+            c[EQ_VY] = c[EQ_VY] * sf;
+            c[EQ_VZ] = c[EQ_VZ] * sf;
+        }
+        const double W = root_newton(P, c, gi);
+        if (W < 1.0) goto atmosphere;
+        double V, dV;
+        eval_V(P, W, c, &V, &dV);
+        if (V <= 0.0 || isnan(V)) goto atmosphere;
+        q[EQ_DE] = c[EQ_DE] / (W * sqg);
+        q[EQ_PR] = (V - c[EQ_PR] - c[EQ_DE]) / sqg;
+        q[EQ_VX] = (gi[0][0] * c[EQ_VX] + gi[0][1] * c[EQ_VY] + gi[0][2] * c[EQ_VZ]) / V;
+        q[EQ_VY] = (gi[1][0] * c[EQ_VX] + gi[1][1] * c[EQ_VY] + gi[1][2] * c[EQ_VZ]) / V;
+        q[EQ_VZ] = (gi[2][0] * c[EQ_VX] + gi[2][1] * c[EQ_VY] + gi[2][2] * c[EQ_VZ]) / V;
+        double vf = 0.0;
+        for (int a = 0; a < 3; a++)
+            for (int b = 0; b < 3; b++) vf = vf + g[a][b] * q[EQ_VX + a] * q[EQ_VX + b];
+        if (q[EQ_DE] < P->rho_floor || q[EQ_PR] < P->p_floor || isnan(q[EQ_DE]) || isnan(q[EQ_PR]) ||
+            isnan(q[EQ_VX]) || isnan(q[EQ_VY]) || isnan(q[EQ_VZ]) || isnan(vf) || vf >= V_MAX)
+            goto atmosphere;
+        return;
+    }
+atmosphere:
+    q[EQ_DE] = P->rho_floor;
+    q[EQ_PR] = P->p_floor;
+    q[EQ_VX] = q[EQ_VY] = q[EQ_VZ] = 0.0;
+    c[EQ_DE] = Df;
+    c[EQ_PR] = tf;
+    c[EQ_VX] = c[EQ_VY] = c[EQ_VZ] = 0.0;
+}
+
+/* ----------------------------------------------------------- sources */
+/* calculate_stress_energy_tensor (metrics.f90:1251-1310). */
+static void stress_energy(const orc_problem *P, const double *q, const Met *m, double T[4][4])
+{
+    double v[3] = {q[EQ_VX], q[EQ_VY], q[EQ_VZ]}, u4[4];
+    const double vsq = vsq_clamp(m->g, v);
+    const double h = 1.0 + P->g1 * q[EQ_PR] / q[EQ_DE];
+    const double W = 1.0 / sqrt(1.0 - vsq);
+    u4[0] = W / m->alpha;
+    for (int i = 0; i < 3; i++) u4[i + 1] = W * (v[i] - m->beta[i] / m->alpha);
+    for (int mu = 0; mu < 4; mu++)
+        for (int nu = mu; nu < 4; nu++) {
+            T[mu][nu] = (q[EQ_DE] * h * u4[mu] * u4[nu]) + (q[EQ_PR] * m->gm[mu][nu]);
+            if (mu != nu) T[nu][mu] = T[mu][nu];
+        }
+}
+/* calc_sources (equations.f90:509-552). */
+static void sources(const orc_problem *P, const double *q, int i, int j, double *src)
+{
+    for (int e = 0; e < NEQ; e++) src[e] = 0.0;
+    if (P->c.metric == ORC_MINKOWSKI && P->c.geom == ORC_CARTESIAN) return;
+    Centre C;
+    double ch[4][4][4], dg[4][4][3], T[4][4];
+    centre_eval(P, i, j, &C, ch, dg);
+    stress_energy(P, q, &C.m, T);
+    for (int d = 0; d < 3; d++)
+        for (int mu = 0; mu < 4; mu++)
+            for (int nu = 0; nu < 4; nu++)
+                src[EQ_VX + d] = src[EQ_VX + d] + 0.5 * T[mu][nu] * dg[mu][nu][d];
+    for (int mu = 0; mu < 4; mu++) {
+        src[EQ_PR] = src[EQ_PR] + T[mu][0] * C.m.dlna[mu];
+        for (int nu = 0; nu < 4; nu++) src[EQ_PR] = src[EQ_PR] - T[mu][nu] * ch[0][mu][nu];
+    }
+    for (int e = 0; e < NEQ; e++) src[e] = src[e] * C.m.alpha * C.sqg;
+    src[EQ_PR] = src[EQ_PR] * C.m.alpha;
+}
+
+/* ----------------------------------------------------------- problem */
+orc_problem *orc_create(const orc_config *cfg)
+{
+    orc_problem *P = (orc_problem *)calloc(1, sizeof(*P));
+    if (!P) return NULL;
+    P->c = *cfg;
+    /* choose_numerical_architecture (initialization.f90:23-55) */
+    switch (cfg->rec) {
+    case ORC_REC_GODUNOV: P->ng = 1; break;
+    case ORC_REC_TVD: case ORC_REC_WENO3: P->ng = 2; break;
+    default: P->ng = 3; break;
+    }
+    /* initialization.f90:452-457 */
+    P->g1 = cfg->gam / (cfg->gam - 1.0);
+    P->rho_floor = 1.0e-10;
+    P->p_floor = P->rho_floor * 1.0e-3;
+    /* allocate_and_grid (initialization.f90:89-206) */
+    if (cfg->log_r) { P->x_min = log(cfg->r_min); P->x_max = log(cfg->r_max); }
+    else { P->x_min = cfg->r_min; P->x_max = cfg->r_max; }
+    const int nx = cfg->nx, ny = cfg->ny, nz = cfg->nz, ng = P->ng;
+    P->dx = (P->x_max - P->x_min) / (double)(nx > 1 ? nx : 1);
+    P->dy = (cfg->y_max - cfg->y_min) / (double)(ny > 1 ? ny : 1);
+    P->dz = (cfg->z_max - cfg->z_min) / (double)(nz > 1 ? nz : 1);
+    P->x = (double *)malloc(sizeof(double) * (nx + 2 * ng + 1));
+    P->y = (double *)malloc(sizeof(double) * (ny + 2 * ng + 1));
+    P->z = (double *)malloc(sizeof(double) * (nz + 2 * ng + 1));
+    P->x_face = (double *)malloc(sizeof(double) * (nx + 1));
+    P->y_face = (double *)malloc(sizeof(double) * (ny + 1));
+    P->z_face = (double *)malloc(sizeof(double) * (nz + 1));
+    for (int i = 0; i <= nx; i++) P->x_face[i] = P->x_min + (double)i * P->dx;
+    for (int i = -ng; i <= nx + ng; i++) P->x[i + ng] = P->x_min + ((double)i - 0.5) * P->dx;
+    for (int j = 0; j <= ny; j++) P->y_face[j] = cfg->y_min + (double)j * P->dy;
+    for (int j = -ng; j <= ny + ng; j++) P->y[j + ng] = cfg->y_min + ((double)j - 0.5) * P->dy;
+    for (int k = 0; k <= nz; k++) P->z_face[k] = cfg->z_min + (double)k * P->dz;
+    for (int k = -ng; k <= nz + ng; k++) P->z[k + ng] = cfg->z_min + ((double)k - 0.5) * P->dz;
+    return P;
+}
+
+void orc_destroy(orc_problem *P)
+{
+    if (!P) return;
+    free(P->x); free(P->y); free(P->z);
+    free(P->x_face); free(P->y_face); free(P->z_face);
+    free(P->injector);
+    free(P);
+}
+
+void orc_dims(const orc_problem *P, int *dims)
+{
+    dims[0] = P->c.nx; dims[1] = P->c.ny; dims[2] = P->c.nz; dims[3] = P->ng;
+}
+
+void orc_grid(const orc_problem *P, double *grid)
+{
+    grid[0] = P->dx; grid[1] = P->dy; grid[2] = P->dz;
+    grid[3] = P->x_min; grid[4] = P->x_max;
+    grid[5] = P->c.y_min; grid[6] = P->c.y_max; grid[7] = P->c.z_min; grid[8] = P->c.z_max;
+}
+
+void orc_cache_injector(orc_problem *P, const double *p)
+{
+    const int nx = P->c.nx, ny = P->c.ny, nz = P->c.nz, ng = P->ng;
+    free(P->injector);
+    P->injector = (double *)malloc(sizeof(double) * NEQ * ng * ny * nz);
+    for (int k = 1; k <= nz; k++)
+        for (int j = 1; j <= ny; j++)
+            for (int g = 1; g <= ng; g++)
+                for (int v = 0; v < NEQ; v++)
+                    P->injector[((((size_t)(k - 1) * ny + (j - 1)) * ng + (g - 1)) * NEQ) + v] =
+                        p[pidx(P, v, nx + g, j, k)];
+}
+
+void orc_prim_to_cons_all(const orc_problem *P, const double *p, double *u)
+{
+    for (int k = 1; k <= P->c.nz; k++)
+        for (int j = 1; j <= P->c.ny; j++)
+            for (int i = 1; i <= P->c.nx; i++) {
+                Centre C;
+                centre_eval(P, i, j, &C, NULL, NULL);
+                p2c(P, &p[pidx(P, 0, i, j, k)], C.m.g, C.sqg, &u[uidx(P, 0, i, j, k)]);
+            }
+}
+
+void orc_cons_to_prim_all(const orc_problem *P, double *u, double *p)
+{
+    for (int k = 1; k <= P->c.nz; k++)
+        for (int j = 1; j <= P->c.ny; j++)
+            for (int i = 1; i <= P->c.nx; i++) {
+                Centre C;
+                centre_eval(P, i, j, &C, NULL, NULL);
+                c2p(P, &C, &u[uidx(P, 0, i, j, k)], &p[pidx(P, 0, i, j, k)]);
+            }
+}
+
+/* Face metric helper: alpha, beta, gamma, sqrt(det) at (xp, yp). */
+static void face_metric(const orc_problem *P, double xp, double yp, double *alpha, double beta[3],
+                        double g[3][3], double *sqg)
+{
+    Met m;
+    metric_eval(P, xp, yp, &m, NULL, NULL);
+    *alpha = m.alpha;
+    memcpy(beta, m.beta, sizeof(m.beta));
+    memcpy(g, m.g, sizeof(m.g));
+    *sqg = sqrt(m.det);
+}
+
+void orc_rhs(const orc_problem *P, const double *p, double *rhs)
+{
+    const int nx = P->c.nx, ny = P->c.ny, nz = P->c.nz, ng = P->ng;
+    const int nmax = (nx > ny ? (nx > nz ? nx : nz) : (ny > nz ? ny : nz));
+    double *strip = (double *)malloc(sizeof(double) * NEQ * (nmax + 2 * ng + 1));
+    double *flux = (double *)malloc(sizeof(double) * NEQ * (nmax + 1));
+    const double eps64 = 64.0 * DBL_EPSILON;
+    const int polar = (P->c.geom == ORC_SPHERICAL || P->c.geom == ORC_SPHEROIDAL);
+    /* x sweep (evolution.f90:48-94) */
+    for (int k = 1; k <= nz; k++)
+        for (int j = 1; j <= ny; j++) {
+            for (int i = -ng; i <= nx + ng; i++)
+                for (int v = 0; v < NEQ; v++) strip[(size_t)(i + ng) * NEQ + v] = p[pidx(P, v, i, j, k)];
+            for (int i = 0; i <= nx; i++) {
+                double *F = &flux[(size_t)i * NEQ];
+                if (P->c.geom == ORC_CYLINDRICAL && fabs(P->x_face[i]) <= eps64) {
+                    for (int v = 0; v < NEQ; v++) F[v] = 0.0;
+                } else {
+                    double qL[NEQ], qR[NEQ], a, b[3], g[3][3], sqg;
+                    recon_face(P, strip, ng, nx, 1, i, qL, qR);
+                    face_metric(P, P->x_face[i], YC(j), &a, b, g, &sqg);
+                    riemann(P, qL, qR, 1, a, b, g, sqg, F);
+                }
+            }
+            for (int i = 1; i <= nx; i++) {
+                double src[NEQ];
+                sources(P, &p[pidx(P, 0, i, j, k)], i, j, src);
+                for (int v = 0; v < NEQ; v++)
+                    rhs[uidx(P, v, i, j, k)] = -(flux[(size_t)i * NEQ + v] - flux[(size_t)(i - 1) * NEQ + v]) / P->dx + src[v];
+            }
+        }
+    /* y sweep (evolution.f90:100-146) */
+    if (ny > 1)
+        for (int k = 1; k <= nz; k++)
+            for (int i = 1; i <= nx; i++) {
+                for (int j = -ng; j <= ny + ng; j++)
+                    for (int v = 0; v < NEQ; v++) strip[(size_t)(j + ng) * NEQ + v] = p[pidx(P, v, i, j, k)];
+                for (int j = 0; j <= ny; j++) {
+                    double *F = &flux[(size_t)j * NEQ];
+                    if (polar && fabs(sin(P->y_face[j])) <= eps64) {
+                        for (int v = 0; v < NEQ; v++) F[v] = 0.0;
+                    } else {
+                        double qL[NEQ], qR[NEQ], a, b[3], g[3][3], sqg;
+                        recon_face(P, strip, ng, ny, 2, j, qL, qR);
+                        face_metric(P, XC(i), P->y_face[j], &a, b, g, &sqg);
+                        riemann(P, qL, qR, 2, a, b, g, sqg, F);
+                    }
+                }
+                for (int j = 1; j <= ny; j++)
+                    for (int v = 0; v < NEQ; v++)
+                        rhs[uidx(P, v, i, j, k)] = rhs[uidx(P, v, i, j, k)] -
+                            (flux[(size_t)j * NEQ + v] - flux[(size_t)(j - 1) * NEQ + v]) / P->dy;
+            }
+    /* z sweep (evolution.f90:152-189); z faces use the centre metric */
+    if (nz > 1)
+        for (int j = 1; j <= ny; j++)
+            for (int i = 1; i <= nx; i++) {
+                double a, b[3], g[3][3], sqg;
+                face_metric(P, XC(i), YC(j), &a, b, g, &sqg);
+                for (int k = -ng; k <= nz + ng; k++)
+                    for (int v = 0; v < NEQ; v++) strip[(size_t)(k + ng) * NEQ + v] = p[pidx(P, v, i, j, k)];
+                for (int k = 0; k <= nz; k++) {
+                    double qL[NEQ], qR[NEQ];
+                    recon_face(P, strip, ng, nz, 3, k, qL, qR);
+                    riemann(P, qL, qR, 3, a, b, g, sqg, &flux[(size_t)k * NEQ]);
+                }
+                for (int k = 1; k <= nz; k++)
+                    for (int v = 0; v < NEQ; v++)
+                        rhs[uidx(P, v, i, j, k)] = rhs[uidx(P, v, i, j, k)] -
+                            (flux[(size_t)k * NEQ + v] - flux[(size_t)(k - 1) * NEQ + v]) / P->dz;
+            }
+    free(strip);
+    free(flux);
+}
+
+double orc_dt(const orc_problem *P)
+{
+    const int nx = P->c.nx, ny = P->c.ny, nz = P->c.nz;
+    double mx = 0.0, my = 0.0, mz = 0.0;
+    long bad = 0;
+    if (!isfinite(P->dx) || P->dx <= 0.0 || (ny > 1 && (!isfinite(P->dy) || P->dy <= 0.0)) ||
+        (nz > 1 && (!isfinite(P->dz) || P->dz <= 0.0)))
+        return -1.0;
+    /* the metric is independent of the third coordinate: one k suffices */
+    for (int j = 1; j <= ny; j++)
+        for (int i = 1; i <= nx; i++) {
+            Centre C;
+            centre_eval(P, i, j, &C, NULL, NULL);
+            const double a = C.m.alpha;
+            if (!isfinite(a) || a <= 0.0 || !isfinite(C.m.beta[0]) || !isfinite(C.m.beta[1]) ||
+                !isfinite(C.m.beta[2]) || !isfinite(C.ginv[0][0]) || C.ginv[0][0] <= 0.0 ||
+                (ny > 1 && (!isfinite(C.ginv[1][1]) || C.ginv[1][1] <= 0.0)) ||
+                (nz > 1 && (!isfinite(C.ginv[2][2]) || C.ginv[2][2] <= 0.0))) {
+                bad++;
+                continue;
+            }
+            mx = dmax(mx, a * sqrt(C.ginv[0][0]) + fabs(C.m.beta[0]));
+            if (ny > 1) my = dmax(my, a * sqrt(C.ginv[1][1]) + fabs(C.m.beta[1]));
+            if (nz > 1) mz = dmax(mz, a * sqrt(C.ginv[2][2]) + fabs(C.m.beta[2]));
+        }
+    if (bad > 0) return -1.0;
+    double inv = mx / P->dx;
+    if (ny > 1) inv = inv + my / P->dy;
+    if (nz > 1) inv = inv + mz / P->dz;
+    if (!isfinite(inv) || inv <= 0.0) return -1.0;
+    return P->c.cfl / inv;
+}
+
+/* dust free-fall state (conditions.f90:249-278) for the Dust outer injector. */
+static void dust_state(const orc_problem *P, double xr, double *q)
+{
+    double r, jac;
+    if (P->c.log_r) { r = exp(xr); jac = r; } else { r = xr; jac = 1.0; }
+    const double M = P->c.mass;
+    const double ff = sqrt(2.0 * M / r);
+    q[EQ_DE] = -P->c.dust_accretion_constant / ((r * r) * ff);
+    q[EQ_PR] = P->p_floor;
+    q[EQ_VX] = -1.0 / (sqrt(1.0 + r / (2.0 * M)) * (1.0 + ff + 2.0 * M / r));
+    q[EQ_VX] = q[EQ_VX] / jac;
+    q[EQ_VY] = 0.0;
+    q[EQ_VZ] = 0.0;
+}
+
+void orc_boundaries(const orc_problem *P, double *p)
+{
+    const int nx = P->c.nx, ny = P->c.ny, nz = P->c.nz, ng = P->ng, cid = P->c.case_id;
+#define Q(v, i, j, k) p[pidx(P, v, i, j, k)]
+#define ALLV for (int v = 0; v < NEQ; v++)
+    /* periodic y then z over the full ghosted planes (conditions.f90:800-806) */
+#define PERIODIC_Y()                                                                      \
+    for (int g = 1; g <= ng; g++)                                                          \
+        for (int k = -ng; k <= nz + ng; k++)                                               \
+            for (int i = -ng; i <= nx + ng; i++) ALLV {                                    \
+                Q(v, i, 1 - g, k) = Q(v, i, ny + 1 - g, k);                                \
+                Q(v, i, ny + g, k) = Q(v, i, g, k);                                        \
+            }
+#define PERIODIC_Z()                                                                      \
+    for (int g = 1; g <= ng; g++)                                                          \
+        for (int j = -ng; j <= ny + ng; j++)                                               \
+            for (int i = -ng; i <= nx + ng; i++) ALLV {                                    \
+                Q(v, i, j, 1 - g) = Q(v, i, j, nz + 1 - g);                                \
+                Q(v, i, j, nz + g) = Q(v, i, j, g);                                        \
+            }
+    /* The reference interleaves y and z inside one g loop; for g the y planes
+     * are written before the z planes of the same g.  Reproduce that order. */
+#define PERIODIC_YZ_INTERLEAVED()                                                         \
+    for (int g = 1; g <= ng; g++) {                                                        \
+        for (int k = -ng; k <= nz + ng; k++)                                               \
+            for (int i = -ng; i <= nx + ng; i++) ALLV {                                    \
+                Q(v, i, 1 - g, k) = Q(v, i, ny + 1 - g, k);                                \
+            }                                                                              \
+        for (int k = -ng; k <= nz + ng; k++)                                               \
+            for (int i = -ng; i <= nx + ng; i++) ALLV {                                    \
+                Q(v, i, ny + g, k) = Q(v, i, g, k);                                        \
+            }                                                                              \
+        for (int j = -ng; j <= ny + ng; j++)                                               \
+            for (int i = -ng; i <= nx + ng; i++) ALLV {                                    \
+                Q(v, i, j, 1 - g) = Q(v, i, j, nz + 1 - g);                                \
+            }                                                                              \
+        for (int j = -ng; j <= ny + ng; j++)                                               \
+            for (int i = -ng; i <= nx + ng; i++) ALLV {                                    \
+                Q(v, i, j, nz + g) = Q(v, i, j, g);                                        \
+            }                                                                              \
+    }
+    switch (cid) {
+    case 1: case 2: case 3: /* conditions.f90:794-806 */
+        for (int g = 1; g <= ng; g++) {
+            for (int k = -ng; k <= nz + ng; k++)
+                for (int j = -ng; j <= ny + ng; j++) ALLV Q(v, 1 - g, j, k) = Q(v, 1, j, k);
+            for (int k = -ng; k <= nz + ng; k++)
+                for (int j = -ng; j <= ny + ng; j++) ALLV Q(v, nx + g, j, k) = Q(v, nx, j, k);
+        }
+        PERIODIC_YZ_INTERLEAVED();
+        break;
+    case 4: case 6: /* conditions.f90:808-818 */
+        for (int g = 1; g <= ng; g++) {
+            for (int k = -ng; k <= nz + ng; k++)
+                for (int j = -ng; j <= ny + ng; j++) ALLV Q(v, 1 - g, j, k) = Q(v, nx + 1 - g, j, k);
+            for (int k = -ng; k <= nz + ng; k++)
+                for (int j = -ng; j <= ny + ng; j++) ALLV Q(v, nx + g, j, k) = Q(v, g, j, k);
+            for (int k = -ng; k <= nz + ng; k++)
+                for (int i = -ng; i <= nx + ng; i++) ALLV Q(v, i, 1 - g, k) = Q(v, i, ny + 1 - g, k);
+            for (int k = -ng; k <= nz + ng; k++)
+                for (int i = -ng; i <= nx + ng; i++) ALLV Q(v, i, ny + g, k) = Q(v, i, g, k);
+            for (int j = -ng; j <= ny + ng; j++)
+                for (int i = -ng; i <= nx + ng; i++) ALLV Q(v, i, j, 1 - g) = Q(v, i, j, nz + 1 - g);
+            for (int j = -ng; j <= ny + ng; j++)
+                for (int i = -ng; i <= nx + ng; i++) ALLV Q(v, i, j, nz + g) = Q(v, i, j, g);
+        }
+        break;
+    case 5: /* conditions.f90:820-870 */
+        for (int k = -ng; k <= nz + ng; k++)
+            for (int j = -ng; j <= ny + ng; j++)
+                for (int i = -ng; i <= nx + ng; i++)
+                    if (i < 1 || i > nx || j < 1 || j > ny || k < 1 || k > nz) {
+                        Q(EQ_DE, i, j, k) = P->c.jet_ambient_density;
+                        Q(EQ_PR, i, j, k) = P->c.jet_ambient_pressure;
+                        Q(EQ_VX, i, j, k) = 0.0; Q(EQ_VY, i, j, k) = 0.0; Q(EQ_VZ, i, j, k) = 0.0;
+                        Q(EQ_VZ, i, j, k) = P->c.jet_ambient_velocity;
+                    }
+        for (int k = -ng; k <= nz + ng; k++)
+            for (int j = -ng; j <= ny + ng; j++)
+                for (int g = 1; g <= ng; g++) {
+                    ALLV Q(v, 1 - g, j, k) = Q(v, g, j, k);
+                    Q(EQ_VX, 1 - g, j, k) = -Q(EQ_VX, g, j, k);
+                }
+        for (int j = -ng; j <= ny + ng; j++)
+            for (int i = -ng; i <= nx + ng; i++)
+                for (int g = 1; g <= ng; g++) {
+                    ALLV Q(v, i, j, nz + g) = Q(v, i, j, nz);
+                    Q(EQ_VZ, i, j, nz + g) = dmax(0.0, Q(EQ_VZ, i, j, nz));
+                    if (XC(i) <= P->c.jet_nozzle_radius) {
+                        Q(EQ_DE, i, j, 1 - g) = P->c.jet_density;
+                        Q(EQ_PR, i, j, 1 - g) = P->c.jet_pressure;
+                        Q(EQ_VX, i, j, 1 - g) = 0.0; Q(EQ_VY, i, j, 1 - g) = 0.0;
+                        Q(EQ_VZ, i, j, 1 - g) = P->c.jet_velocity;
+                    } else {
+                        ALLV Q(v, i, j, 1 - g) = Q(v, i, j, g);
+                        Q(EQ_VZ, i, j, 1 - g) = -Q(EQ_VZ, i, j, g);
+                    }
+                }
+        break;
+    case 7: /* conditions.f90:872-891 */
+        for (int k = 1; k <= nz; k++)
+            for (int j = 1; j <= ny; j++)
+                for (int g = 1; g <= ng; g++) ALLV {
+                    Q(v, 1 - g, j, k) = Q(v, 1, j, k);
+                    Q(v, nx + g, j, k) = P->injector ?
+                        P->injector[((((size_t)(k - 1) * ny + (j - 1)) * ng + (g - 1)) * NEQ) + v] : 0.0;
+                }
+        PERIODIC_YZ_INTERLEAVED();
+        break;
+    case 8: /* conditions.f90:893-914 */
+        for (int g = 1; g <= ng; g++) {
+            double ds[NEQ];
+            dust_state(P, XC(nx + g), ds);
+            for (int k = 1; k <= nz; k++)
+                for (int j = 1; j <= ny; j++) {
+                    ALLV Q(v, 1 - g, j, k) = Q(v, 1, j, k);
+                    Q(EQ_DE, 1 - g, j, k) = dmax(Q(EQ_DE, 1, j, k), P->rho_floor);
+                    Q(EQ_PR, 1 - g, j, k) = dmax(Q(EQ_PR, 1, j, k), P->p_floor);
+                    Q(EQ_VX, 1 - g, j, k) = dmin(0.0, Q(EQ_VX, 1, j, k));
+                    ALLV Q(v, nx + g, j, k) = ds[v];
+                }
+        }
+        PERIODIC_YZ_INTERLEAVED();
+        break;
+    case 9: case 10: case 11: case 12: {
+        /* shift-aware radial outflow (conditions.f90:918-938, 951-973, 1007-1026) */
+        for (int k = 1; k <= nz; k++)
+            for (int j = 1; j <= ny; j++) {
+                Centre C1, Cn;
+                centre_eval(P, 1, j, &C1, NULL, NULL);
+                centre_eval(P, nx, j, &Cn, NULL, NULL);
+                const double vin = C1.m.beta[0] / C1.m.alpha, vout = Cn.m.beta[0] / Cn.m.alpha;
+                for (int g = 1; g <= ng; g++) {
+                    ALLV Q(v, 1 - g, j, k) = Q(v, 1, j, k);
+                    Q(EQ_DE, 1 - g, j, k) = dmax(Q(EQ_DE, 1, j, k), P->rho_floor);
+                    Q(EQ_PR, 1 - g, j, k) = dmax(Q(EQ_PR, 1, j, k), P->p_floor);
+                    Q(EQ_VX, 1 - g, j, k) = dmin(vin, Q(EQ_VX, 1, j, k));
+                    if (cid <= 10) {
+                        if (Q(EQ_DE, 1, j, k) < 1.0e-6) { Q(EQ_VY, 1 - g, j, k) = 0.0; Q(EQ_VZ, 1 - g, j, k) = 0.0; }
+                        Q(EQ_DE, nx + g, j, k) = dmax(Q(EQ_DE, nx, j, k), P->rho_floor);
+                        Q(EQ_PR, nx + g, j, k) = dmax(Q(EQ_PR, nx, j, k), P->p_floor);
+                        Q(EQ_VX, nx + g, j, k) = dmax(vout, Q(EQ_VX, nx, j, k));
+                        Q(EQ_VY, nx + g, j, k) = Q(EQ_VY, nx, j, k);
+                        Q(EQ_VZ, nx + g, j, k) = Q(EQ_VZ, nx, j, k);
+                    } else {
+                        ALLV Q(v, nx + g, j, k) = Q(v, nx, j, k);
+                        Q(EQ_DE, nx + g, j, k) = dmax(Q(EQ_DE, nx, j, k), P->rho_floor);
+                        Q(EQ_PR, nx + g, j, k) = dmax(Q(EQ_PR, nx, j, k), P->p_floor);
+                        Q(EQ_VX, nx + g, j, k) = dmax(vout, Q(EQ_VX, nx, j, k));
+                    }
+                }
+            }
+        if (cid <= 10) {
+            PERIODIC_YZ_INTERLEAVED();
+        } else if (cid == 11) { /* reflective theta, then periodic phi (conditions.f90:976-994) */
+            for (int k = 1; k <= nz; k++)
+                for (int i = -ng; i <= nx + ng; i++)
+                    for (int g = 1; g <= ng; g++) {
+                        ALLV Q(v, i, 1 - g, k) = Q(v, i, g, k);
+                        Q(EQ_VY, i, 1 - g, k) = -Q(EQ_VY, i, g, k);
+                        ALLV Q(v, i, ny + g, k) = Q(v, i, ny + 1 - g, k);
+                        Q(EQ_VY, i, ny + g, k) = -Q(EQ_VY, i, ny + 1 - g, k);
+                    }
+            PERIODIC_Z();
+        } else { /* trans-polar theta with phi -> phi + pi, then periodic phi (conditions.f90:1032-1059) */
+            for (int k = 1; k <= nz; k++) {
+                int ks = k + nz / 2;
+                if (ks > nz) ks -= nz;
+                for (int i = -ng; i <= nx + ng; i++)
+                    for (int g = 1; g <= ng; g++) {
+                        ALLV Q(v, i, 1 - g, k) = Q(v, i, g, ks);
+                        Q(EQ_VY, i, 1 - g, k) = -Q(EQ_VY, i, g, ks);
+                        Q(EQ_VZ, i, 1 - g, k) = Q(EQ_VZ, i, g, ks);
+                        ALLV Q(v, i, ny + g, k) = Q(v, i, ny + 1 - g, ks);
+                        Q(EQ_VY, i, ny + g, k) = -Q(EQ_VY, i, ny + 1 - g, ks);
+                        Q(EQ_VZ, i, ny + g, k) = Q(EQ_VZ, i, ny + 1 - g, ks);
+                    }
+            }
+            PERIODIC_Z();
+        }
+        break;
+    }
+    default:
+        break;
+    }
+#undef Q
+#undef ALLV
+#undef PERIODIC_Y
+#undef PERIODIC_Z
+#undef PERIODIC_YZ_INTERLEAVED
+}
+
+void orc_step(const orc_problem *P, double dt, double *u, double *p, double *u0, double *rhs)
+{
+    const size_t n = (size_t)NEQ * P->c.nx * P->c.ny * P->c.nz;
+    /* grhd2.f90:232-265: u = up, BC, then three stages */
+    memcpy(u0, u, n * sizeof(double));
+    orc_boundaries(P, p);
+    for (int rk = 1; rk <= 3; rk++) {
+        orc_rhs(P, p, rhs);
+        if (rk == 1) {
+            for (size_t e = 0; e < n; e++) u[e] = u0[e] + rhs[e] * dt;
+        } else if (rk == 2) {
+            for (size_t e = 0; e < n; e++) u[e] = 0.75 * u0[e] + 0.25 * (u[e] + rhs[e] * dt);
+        } else {
+            for (size_t e = 0; e < n; e++) u[e] = (u0[e] + 2.0 * (u[e] + rhs[e] * dt)) / 3.0;
+        }
+        orc_cons_to_prim_all(P, u, p);
+        orc_boundaries(P, p);
+    }
+}
+
+/* ----------------------------------------------------------- point probes */
+void orc_metric_point(const orc_problem *P, double x, double y, double *out, double *chris,
+                      double *dg)
+{
+    Met m;
+    double ch[4][4][4], d[4][4][3];
+    metric_eval(P, x, y, &m, ch, d);
+    out[0] = m.alpha;
+    for (int i = 0; i < 3; i++) out[1 + i] = m.beta[i];
+    for (int b = 0; b < 3; b++)
+        for (int a = 0; a < 3; a++) out[4 + b * 3 + a] = m.g[a][b];
+    out[13] = m.det;
+    for (int b = 0; b < 4; b++)
+        for (int a = 0; a < 4; a++) out[14 + b * 4 + a] = m.gm[a][b];
+    for (int i = 0; i < 4; i++) out[30 + i] = m.dlna[i];
+    /* Fortran column-major: chris(l, mu, nu) -> l + 4 mu + 16 nu */
+    for (int nu = 0; nu < 4; nu++)
+        for (int mu = 0; mu < 4; mu++)
+            for (int l = 0; l < 4; l++) chris[l + 4 * mu + 16 * nu] = ch[l][mu][nu];
+    for (int k = 0; k < 3; k++)
+        for (int nu = 0; nu < 4; nu++)
+            for (int mu = 0; mu < 4; mu++) dg[mu + 4 * nu + 16 * k] = d[mu][nu][k];
+}
+
+static void unpack_g(const double *gf, double g[3][3])
+{   /* column-major gamma(3,3) */
+    for (int b = 0; b < 3; b++)
+        for (int a = 0; a < 3; a++) g[a][b] = gf[b * 3 + a];
+}
+
+void orc_riemann(const orc_problem *P, const double *qL, const double *qR, int dir, double alpha,
+                 const double *beta, const double *gf, double sqg, double *flux)
+{
+    double g[3][3];
+    unpack_g(gf, g);
+    riemann(P, qL, qR, dir, alpha, beta, g, sqg, flux);
+}
+
+void orc_physical_flux(const orc_problem *P, const double *q, int dir, double alpha,
+                       const double *beta, const double *gf, double sqg, double *flux)
+{
+    double g[3][3];
+    unpack_g(gf, g);
+    phys_flux(P, q, dir, alpha, beta, g, sqg, flux);
+}
+
+void orc_reconstruct(const orc_problem *P, const double *prim, int nodes, int dir, double *qL,
+                     double *qR)
+{
+    for (int i = 0; i <= nodes; i++)
+        recon_face(P, prim, P->ng, nodes, dir, i, &qL[(size_t)i * NEQ], &qR[(size_t)i * NEQ]);
+}
+
+void orc_cons_to_prim_cell(const orc_problem *P, double *cons, double *prim, int i, int j, int k)
+{
+    (void)k;
+    Centre C;
+    centre_eval(P, i, j, &C, NULL, NULL);
+    c2p(P, &C, cons, prim);
+}
+
+void orc_prim_to_cons_cell(const orc_problem *P, const double *prim, double *cons, int i, int j,
+                           int k)
+{
+    (void)k;
+    Centre C;
+    centre_eval(P, i, j, &C, NULL, NULL);
+    p2c(P, prim, C.m.g, C.sqg, cons);
+}
+
+void orc_sources_cell(const orc_problem *P, const double *prim, double *src, int i, int j, int k)
+{
+    (void)k;
+    sources(P, prim, i, j, src);
+}
