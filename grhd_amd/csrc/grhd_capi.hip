@@ -1,0 +1,675 @@
+// grhd_capi.hip -- the C ABI (include/grhd_c.h): contexts, validation, layout
+// conversion between the reference's host layout and the device SoA layout,
+// and the RK3 step driver.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/grhd_c.h"
+#include "grhd_initial.hpp"
+#include "grhd_kernels.hpp"
+
+using namespace grhd;
+
+struct grhd_ctx {
+    grhd_config cfg;
+    Params P;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    bool mixed = false;
+    double *p = nullptr;      // ghosted primitives, SoA [5][NZs][NYs][NXs]
+    double *u[2] = {nullptr, nullptr};  // conserved, SoA [5][nz][ny][nx]
+    int ucur = 0;             // u[ucur] holds U^n
+    void *F[3] = {nullptr, nullptr, nullptr};
+    int64_t fstr[3] = {0, 0, 0};
+    double *injector = nullptr;
+    double *partial = nullptr;
+    int partial_n = 0;
+    unsigned long long *counters = nullptr;
+    int64_t steps = 0;
+    double dt_cfl = 0.0;
+    bool dt_ready = false;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double last_ms = 0.0;
+    int64_t last_launches = 0;
+    std::string err;
+};
+
+namespace {
+
+std::mutex g_err_mu;
+std::string g_create_err;
+
+int fail(grhd_ctx *ctx, int code, const std::string &msg)
+{
+    if (ctx) {
+        ctx->err = msg;
+    } else {
+        std::lock_guard<std::mutex> lk(g_err_mu);
+        g_create_err = msg;
+    }
+    return code;
+}
+
+#define HIPCHK(ctx, call)                                                                      \
+    do {                                                                                       \
+        hipError_t e_ = (call);                                                                \
+        if (e_ != hipSuccess)                                                                  \
+            return fail(ctx, e_ == hipErrorOutOfMemory ? GRHD_ERR_OUT_OF_MEMORY : GRHD_ERR_HIP, \
+                        std::string(#call) + ": " + hipGetErrorString(e_));                    \
+    } while (0)
+
+const double kPi = 3.141592653589793;
+
+// validate_parameters (parameters.f90:359-485) for the fields the update reads.
+int validate(const grhd_config &c, std::string &msg)
+{
+    const double eps = DBL_EPSILON;
+    const bool polar = (c.geometry == GRHD_GEOM_SPHERICAL || c.geometry == GRHD_GEOM_SPHEROIDAL);
+    const double reals[] = {c.r_min, c.r_max, c.y_min, c.y_max, c.z_min, c.z_max,
+                            c.adiabatic_index, c.bh_mass, c.spin, c.cfl};
+    for (double v : reals)
+        if (!std::isfinite(v)) return msg = "real parameters must be finite", 1;
+    if (c.problem < 1 || c.problem > 12) return msg = "invalid problem id", 1;
+    if (c.nx < 1 || c.ny < 1 || c.nz < 1) return msg = "nx, ny and nz must be positive", 1;
+    if (c.r_max <= c.r_min) return msg = "r_max must exceed r_min", 1;
+    if (c.log_r && !polar) return msg = "logarithmic r needs spherical or spheroidal geometry", 1;
+    if (polar && c.r_min <= 0.0) return msg = "spherical/spheroidal geometry needs r_min > 0", 1;
+    if (c.adiabatic_index <= 1.0) return msg = "adiabatic_index must exceed 1", 1;
+    if (c.bh_mass < 0.0) return msg = "bh_mass cannot be negative", 1;
+    if (c.cfl <= 0.0) return msg = "cfl must be positive", 1;
+    if (c.reconstruction < GRHD_REC_GODUNOV || c.reconstruction > GRHD_REC_WENO5)
+        return msg = "invalid reconstruction identifier", 1;
+    if (c.riemann_solver != GRHD_RS_HLLE && c.riemann_solver != GRHD_RS_HLLC)
+        return msg = "invalid Riemann solver identifier", 1;
+    if (c.reconstruction == GRHD_REC_TVD &&
+        (c.tvd_limiter < GRHD_LIM_MINMOD || c.tvd_limiter > GRHD_LIM_MC))
+        return msg = "invalid TVD limiter identifier", 1;
+    if (c.precision != GRHD_PRECISION_FP64 && c.precision != GRHD_PRECISION_MIXED)
+        return msg = "invalid precision", 1;
+    switch (c.metric) {
+    case GRHD_METRIC_MINKOWSKI:
+        if (std::fabs(c.spin) > 0.0 || std::fabs(c.bh_mass) > 0.0)
+            return msg = "Minkowski needs spin = 0 and bh_mass = 0", 1;
+        if (c.geometry == GRHD_GEOM_SPHEROIDAL)
+            return msg = "Minkowski needs cartesian, cylindrical or spherical geometry", 1;
+        break;
+    case GRHD_METRIC_EDDINGTON_FINKELSTEIN:
+        if (std::fabs(c.spin) > 0.0) return msg = "Eddington-Finkelstein needs spin = 0", 1;
+        if (c.bh_mass <= 0.0 || c.geometry != GRHD_GEOM_SPHERICAL)
+            return msg = "Eddington-Finkelstein needs bh_mass > 0 and spherical geometry", 1;
+        break;
+    case GRHD_METRIC_KERR_SCHILD:
+        if (c.bh_mass <= 0.0 || !(c.geometry == GRHD_GEOM_SPHEROIDAL || c.geometry == GRHD_GEOM_SPHERICAL))
+            return msg = "Kerr-Schild needs bh_mass > 0 and spheroidal geometry", 1;
+        if (std::fabs(c.spin) > c.bh_mass + 64.0 * eps * std::max(1.0, c.bh_mass))
+            return msg = "Kerr-Schild needs |spin| <= bh_mass", 1;
+        break;
+    default:
+        return msg = "invalid metric", 1;
+    }
+    if (polar) {
+        if (c.y_min < 0.0 || c.y_max > kPi || c.y_max <= c.y_min)
+            return msg = "need 0 <= theta_min < theta_max <= pi", 1;
+        if (c.z_max <= c.z_min || c.z_max - c.z_min > 2.0 * kPi + 64.0 * eps * kPi)
+            return msg = "the azimuthal interval must have length in (0, 2 pi]", 1;
+    } else if (c.y_max <= c.y_min || c.z_max <= c.z_min) {
+        return msg = "coordinate bounds must increase", 1;
+    }
+    if (c.problem == 12) {
+        const double tol = 128.0 * eps * kPi;
+        if (c.ny <= 1 || c.nz <= 1 || c.nz % 2 != 0)
+            return msg = "fishbone_3d needs ny, nz > 1 and even nz", 1;
+        if (!c.allow_open_theta &&
+            (std::fabs(c.y_min) > tol || std::fabs(c.y_max - kPi) > tol ||
+             std::fabs((c.z_max - c.z_min) - 2.0 * kPi) > tol))
+            return msg = "fishbone_3d needs theta = [0, pi] and phi length 2 pi", 1;
+    }
+    return 0;
+}
+
+int metric_kind(const grhd_config &c)
+{
+    if (c.metric == GRHD_METRIC_MINKOWSKI) {
+        if (c.geometry == GRHD_GEOM_CARTESIAN) return MK_MINK_CART;
+        if (c.geometry == GRHD_GEOM_CYLINDRICAL) return MK_MINK_CYL;
+        return c.log_r ? MK_MINK_SPH_LOG : MK_MINK_SPH;
+    }
+    if (c.metric == GRHD_METRIC_EDDINGTON_FINKELSTEIN) return c.log_r ? MK_EF_LOG : MK_EF;
+    return c.log_r ? MK_KS_LOG : MK_KS;
+}
+
+Params make_params(const grhd_config &c)
+{
+    Params P;
+    std::memset(&P, 0, sizeof(P));
+    P.problem = c.problem;
+    P.mk = metric_kind(c);
+    P.nx = c.nx;
+    P.ny = c.ny;
+    P.nz = c.nz;
+    P.ng = (c.reconstruction == GRHD_REC_GODUNOV) ? 1
+           : (c.reconstruction == GRHD_REC_TVD || c.reconstruction == GRHD_REC_WENO3) ? 2 : 3;
+    P.rec = c.reconstruction;
+    P.rs = c.riemann_solver;
+    P.lim = c.tvd_limiter;
+    P.sensor = c.shock_sensor ? 1 : 0;
+    P.polar = (c.geometry == GRHD_GEOM_SPHERICAL || c.geometry == GRHD_GEOM_SPHEROIDAL);
+    P.cyl = (c.geometry == GRHD_GEOM_CYLINDRICAL);
+    P.gam = c.adiabatic_index;
+    P.g1 = c.adiabatic_index / (c.adiabatic_index - 1.0);
+    P.mass = c.bh_mass;
+    P.spin = c.spin;
+    P.rho_floor = 1.0e-10;           // initialization.f90:454-455
+    P.p_floor = P.rho_floor * 1.0e-3;
+    const double x_min = c.log_r ? std::log(c.r_min) : c.r_min;
+    const double x_max = c.log_r ? std::log(c.r_max) : c.r_max;
+    P.x_min = x_min;
+    P.dx = (x_max - x_min) / (double)std::max(1, c.nx);
+    P.y_min = c.y_min;
+    P.dy = (c.y_max - c.y_min) / (double)std::max(1, c.ny);
+    P.z_min = c.z_min;
+    P.dz = (c.z_max - c.z_min) / (double)std::max(1, c.nz);
+    P.inv_dx = 1.0 / P.dx;
+    P.inv_dy = 1.0 / P.dy;
+    P.inv_dz = 1.0 / P.dz;
+    // ghosts only on active axes; an inactive axis stores its single cell
+    P.ox = P.ng;
+    P.NXs = c.nx + 2 * P.ng + 1;
+    P.oy = (c.ny > 1) ? P.ng : -1;
+    P.NYs = (c.ny > 1) ? c.ny + 2 * P.ng + 1 : 1;
+    P.oz = (c.nz > 1) ? P.ng : -1;
+    P.NZs = (c.nz > 1) ? c.nz + 2 * P.ng + 1 : 1;
+    P.vstride = (int64_t)P.NXs * P.NYs * P.NZs;
+    P.jet_ambient_density = c.jet_ambient_density;
+    P.jet_ambient_pressure = c.jet_ambient_pressure;
+    P.jet_ambient_velocity = c.jet_ambient_velocity;
+    P.jet_nozzle_radius = c.jet_nozzle_radius;
+    P.jet_density = c.jet_density;
+    P.jet_pressure = c.jet_pressure;
+    P.jet_velocity = c.jet_velocity;
+    P.dust_accretion_constant = c.dust_accretion_constant;
+    return P;
+}
+
+// Reference-layout index helpers (host arrays).
+struct RefLayout {
+    int nx, ny, nz, ng, NXg, NYg, NZg;
+    explicit RefLayout(const Params &P)
+        : nx(P.nx), ny(P.ny), nz(P.nz), ng(P.ng), NXg(P.nx + 2 * P.ng + 1), NYg(P.ny + 2 * P.ng + 1),
+          NZg(P.nz + 2 * P.ng + 1) {}
+    size_t p(int v, int i, int j, int k) const
+    {
+        return ((((size_t)(k + ng) * NYg + (size_t)(j + ng)) * NXg + (size_t)(i + ng)) * NEQ) + v;
+    }
+    size_t u(int v, int i, int j, int k) const
+    {
+        return ((((size_t)(k - 1) * ny + (size_t)(j - 1)) * nx + (size_t)(i - 1)) * NEQ) + v;
+    }
+    size_t psize() const { return (size_t)NXg * NYg * NZg * NEQ; }
+    size_t usize() const { return (size_t)nx * ny * nz * NEQ; }
+};
+
+// Storage range of reference index along an axis: active axes keep [-ng, n+ng],
+// inactive axes keep index 1 only.
+inline void axis_range(int n, int ng, int &lo, int &hi)
+{
+    if (n > 1) {
+        lo = -ng;
+        hi = n + ng;
+    } else {
+        lo = 1;
+        hi = 1;
+    }
+}
+
+void host_to_soa_p(const Params &P, const double *href, std::vector<double> &soa)
+{
+    RefLayout L(P);
+    soa.assign((size_t)P.vstride * NEQ, 0.0);
+    int ilo, ihi, jlo, jhi, klo, khi;
+    axis_range(P.nx, P.ng, ilo, ihi);
+    if (P.nx == 1) { ilo = -P.ng; ihi = 1 + P.ng; }  // x keeps ghosts always
+    axis_range(P.ny, P.ng, jlo, jhi);
+    axis_range(P.nz, P.ng, klo, khi);
+    for (int v = 0; v < NEQ; v++)
+        for (int k = klo; k <= khi; k++)
+            for (int j = jlo; j <= jhi; j++)
+                for (int i = ilo; i <= ihi; i++)
+                    soa[(size_t)v * P.vstride + (size_t)pindex(P, i, j, k)] = href[L.p(v, i, j, k)];
+}
+
+void soa_to_host_p(const Params &P, const std::vector<double> &soa, double *href)
+{
+    RefLayout L(P);
+    const int ng = P.ng;
+    for (int k = -ng; k <= P.nz + ng; k++) {
+        const int ks = (P.nz > 1) ? k : 1;
+        for (int j = -ng; j <= P.ny + ng; j++) {
+            const int js = (P.ny > 1) ? j : 1;
+            for (int i = -ng; i <= P.nx + ng; i++)
+                for (int v = 0; v < NEQ; v++)
+                    href[L.p(v, i, j, k)] = soa[(size_t)v * P.vstride + (size_t)pindex(P, i, js, ks)];
+        }
+    }
+}
+
+int ensure_dt(grhd_ctx *c)
+{
+    if (c->dt_ready) return GRHD_OK;
+    const Params &P = c->P;
+    if (!std::isfinite(P.dx) || P.dx <= 0.0 || (P.ny > 1 && (!std::isfinite(P.dy) || P.dy <= 0.0)) ||
+        (P.nz > 1 && (!std::isfinite(P.dz) || P.dz <= 0.0)))
+        return fail(c, GRHD_ERR_INVALID_METRIC, "invalid grid spacing in the CFL condition");
+    const int nb = launch_cfl(P, c->stream, c->partial);
+    std::vector<double> h((size_t)nb * 4);
+    HIPCHK(c, hipMemcpyAsync(h.data(), c->partial, h.size() * sizeof(double), hipMemcpyDeviceToHost,
+                             c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    double mx = 0.0, my = 0.0, mz = 0.0, bad = 0.0;
+    for (int b = 0; b < nb; b++) {
+        mx = std::max(mx, h[4 * b + 0]);
+        my = std::max(my, h[4 * b + 1]);
+        mz = std::max(mz, h[4 * b + 2]);
+        bad += h[4 * b + 3];
+    }
+    if (bad > 0.0) return fail(c, GRHD_ERR_INVALID_METRIC, "invalid metric data in CFL cells");
+    double inv = mx / P.dx;
+    if (P.ny > 1) inv = inv + my / P.dy;
+    if (P.nz > 1) inv = inv + mz / P.dz;
+    if (!std::isfinite(inv) || inv <= 0.0)
+        return fail(c, GRHD_ERR_INVALID_METRIC, "invalid multidimensional CFL propagation rate");
+    c->dt_cfl = c->cfg.cfl / inv;
+    c->dt_ready = true;
+    return GRHD_OK;
+}
+
+void cache_injector_from(grhd_ctx *c, const double *href)
+{
+    if (c->P.problem != 7) return;
+    const Params &P = c->P;
+    RefLayout L(P);
+    std::vector<double> inj((size_t)NEQ * P.ng * P.ny * P.nz);
+    for (int k = 1; k <= P.nz; k++)
+        for (int j = 1; j <= P.ny; j++)
+            for (int g = 1; g <= P.ng; g++)
+                for (int v = 0; v < NEQ; v++)
+                    inj[((((size_t)(k - 1) * P.ny + (j - 1)) * P.ng + (g - 1)) * NEQ) + v] =
+                        href[L.p(v, P.nx + g, j, k)];
+    (void)hipMemcpyAsync(c->injector, inj.data(), inj.size() * sizeof(double), hipMemcpyHostToDevice,
+                         c->stream);
+    (void)hipStreamSynchronize(c->stream);
+}
+
+int upload_p(grhd_ctx *c, const double *href)
+{
+    std::vector<double> soa;
+    host_to_soa_p(c->P, href, soa);
+    HIPCHK(c, hipMemcpyAsync(c->p, soa.data(), soa.size() * sizeof(double), hipMemcpyHostToDevice,
+                             c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    cache_injector_from(c, href);
+    return GRHD_OK;
+}
+
+int run_steps(grhd_ctx *c, const double *dts, int n)
+{
+    const Params &P = c->P;
+    HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    int64_t launches = 0;
+    launch_boundaries(P, c->stream, c->p, c->injector);  // grhd2.f90:233
+    launches++;
+    const int per_stage = 1 + (P.ny > 1) + (P.nz > 1) + 1 + 1;
+    for (int s = 0; s < n; s++) {
+        double *u0 = c->u[c->ucur], *u1 = c->u[1 - c->ucur];
+        for (int rk = 1; rk <= 3; rk++) {
+            launch_stage(P, c->stream, c->p, c->F, c->fstr, u0, u1, rk, dts[s], c->counters, c->mixed,
+                         c->injector, true);
+            launches += per_stage;
+        }
+        c->ucur = 1 - c->ucur;
+    }
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+    c->steps += n;
+    c->last_launches = launches;
+    return GRHD_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *grhd_last_error(const grhd_ctx *ctx)
+{
+    if (ctx) return ctx->err.c_str();
+    std::lock_guard<std::mutex> lk(g_err_mu);
+    return g_create_err.c_str();
+}
+
+int grhd_create(const grhd_config *cfg, grhd_ctx **out)
+{
+    if (!cfg || !out) return fail(nullptr, GRHD_ERR_INVALID_ARGUMENT, "null argument");
+    *out = nullptr;
+    std::string msg;
+    if (validate(*cfg, msg)) return fail(nullptr, GRHD_ERR_INVALID_CONFIG, msg);
+    grhd_ctx *c = new grhd_ctx();
+    c->cfg = *cfg;
+    if (c->cfg.metric == GRHD_METRIC_KERR_SCHILD) c->cfg.geometry = GRHD_GEOM_SPHEROIDAL;
+    c->P = make_params(c->cfg);
+    c->mixed = (cfg->precision == GRHD_PRECISION_MIXED);
+    const Params &P = c->P;
+    // Boundary rules that reflect or shift need as many interior cells as ghosts.
+    const bool needs = (P.problem == 5 && (P.nx < P.ng || (P.nz > 1 && P.nz < P.ng))) || (P.problem == 11 && P.ny < P.ng) ||
+                       (P.problem == 12 && P.ny < P.ng);
+    if (needs) {
+        delete c;
+        return fail(nullptr, GRHD_ERR_INVALID_CONFIG, "grid smaller than the ghost depth");
+    }
+    auto bail = [&](hipError_t e, const char *what) {
+        std::string m = std::string(what) + ": " + hipGetErrorString(e);
+        const int code = (e == hipErrorOutOfMemory) ? GRHD_ERR_OUT_OF_MEMORY : GRHD_ERR_HIP;
+        grhd_destroy(c);
+        return fail(nullptr, code, m);
+    };
+    hipError_t e;
+    if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess)
+        return bail(e, "hipStreamCreate");
+    c->own_stream = true;
+    const int64_t ncell = (int64_t)P.nx * P.ny * P.nz;
+    const size_t rsz = c->mixed ? sizeof(float) : sizeof(double);
+    c->fstr[0] = (int64_t)(P.nx + 1) * P.ny * P.nz;
+    c->fstr[1] = (int64_t)P.nx * (P.ny + 1) * P.nz;
+    c->fstr[2] = (int64_t)P.nx * P.ny * (P.nz + 1);
+    if ((e = hipMalloc(&c->p, (size_t)P.vstride * NEQ * sizeof(double))) != hipSuccess) return bail(e, "hipMalloc p");
+    for (int b = 0; b < 2; b++)
+        if ((e = hipMalloc(&c->u[b], (size_t)ncell * NEQ * sizeof(double))) != hipSuccess)
+            return bail(e, "hipMalloc u");
+    for (int d = 0; d < 3; d++) {
+        const bool active = (d == 0) || (d == 1 && P.ny > 1) || (d == 2 && P.nz > 1);
+        const size_t n = active ? (size_t)c->fstr[d] * NEQ : 1;
+        if ((e = hipMalloc(&c->F[d], n * rsz)) != hipSuccess) return bail(e, "hipMalloc flux");
+    }
+    if ((e = hipMalloc(&c->injector, (size_t)NEQ * P.ng * P.ny * P.nz * sizeof(double) + 8)) != hipSuccess)
+        return bail(e, "hipMalloc injector");
+    c->partial_n = (int)(((int64_t)P.nx * P.ny + 255) / 256);
+    if ((e = hipMalloc(&c->partial, (size_t)c->partial_n * 4 * sizeof(double))) != hipSuccess)
+        return bail(e, "hipMalloc partial");
+    if ((e = hipMalloc(&c->counters, 4 * sizeof(unsigned long long))) != hipSuccess)
+        return bail(e, "hipMalloc counters");
+    if ((e = hipMemsetAsync(c->counters, 0, 4 * sizeof(unsigned long long), c->stream)) != hipSuccess)
+        return bail(e, "hipMemset");
+    if ((e = hipMemsetAsync(c->p, 0, (size_t)P.vstride * NEQ * sizeof(double), c->stream)) != hipSuccess)
+        return bail(e, "hipMemset");
+    if ((e = hipEventCreate(&c->ev0)) != hipSuccess) return bail(e, "hipEventCreate");
+    if ((e = hipEventCreate(&c->ev1)) != hipSuccess) return bail(e, "hipEventCreate");
+    if ((e = hipStreamSynchronize(c->stream)) != hipSuccess) return bail(e, "hipStreamSynchronize");
+    *out = c;
+    return GRHD_OK;
+}
+
+int grhd_destroy(grhd_ctx *c)
+{
+    if (!c) return GRHD_OK;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    (void)hipFree(c->p);
+    (void)hipFree(c->u[0]);
+    (void)hipFree(c->u[1]);
+    for (int d = 0; d < 3; d++) (void)hipFree(c->F[d]);
+    (void)hipFree(c->injector);
+    (void)hipFree(c->partial);
+    (void)hipFree(c->counters);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return GRHD_OK;
+}
+
+int grhd_dims(const grhd_ctx *c, int32_t *dims)
+{
+    if (!c || !dims) return GRHD_ERR_INVALID_ARGUMENT;
+    dims[0] = c->P.nx;
+    dims[1] = c->P.ny;
+    dims[2] = c->P.nz;
+    dims[3] = c->P.ng;
+    return GRHD_OK;
+}
+
+int grhd_initial_conditions(grhd_ctx *c)
+{
+    if (!c) return GRHD_ERR_INVALID_ARGUMENT;
+    std::vector<double> h;
+    const int rc = initial_conditions(c->P, c->cfg, h, c->err);
+    if (rc != GRHD_OK) return rc;
+    return grhd_set_primitives(c, h.data());
+}
+
+int grhd_perturb(grhd_ctx *c, int32_t type, double amplitude, uint64_t seed)
+{
+    if (!c || (type != 1 && type != 2) || !(amplitude >= 0.0 && amplitude < 1.0))
+        return c ? fail(c, GRHD_ERR_INVALID_ARGUMENT, "perturbation type must be 1 or 2 and 0 <= A < 1")
+                 : GRHD_ERR_INVALID_ARGUMENT;
+    RefLayout L(c->P);
+    std::vector<double> h(L.psize());
+    int rc = grhd_get_primitives(c, h.data());
+    if (rc != GRHD_OK) return rc;
+    perturb(c->P, type, amplitude, seed, h);
+    rc = upload_p(c, h.data());
+    if (rc != GRHD_OK) return rc;
+    launch_p2c(c->P, c->stream, c->p, c->u[c->ucur]);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return GRHD_OK;
+}
+
+int grhd_fishbone_unit_density_k(const grhd_config *cfg, double *K)
+{
+    if (!cfg || !K) return GRHD_ERR_INVALID_ARGUMENT;
+    grhd_config c = *cfg;
+    if (c.metric == GRHD_METRIC_KERR_SCHILD) c.geometry = GRHD_GEOM_SPHEROIDAL;
+    const Params P = make_params(c);
+    const double k = fishbone_K_for_unit_density(P, c.fm_inner_radius * c.bh_mass,
+                                                 c.fm_pressure_max_radius * c.bh_mass);
+    if (!(k > 0.0) || !std::isfinite(k)) return fail(nullptr, GRHD_ERR_INVALID_CONFIG, "invalid torus");
+    *K = k;
+    return GRHD_OK;
+}
+
+int grhd_put_primitives(grhd_ctx *c, const double *p)
+{
+    if (!c || !p) return GRHD_ERR_INVALID_ARGUMENT;
+    return upload_p(c, p);
+}
+
+int grhd_set_primitives(grhd_ctx *c, const double *p)
+{
+    if (!c || !p) return GRHD_ERR_INVALID_ARGUMENT;
+    int rc = upload_p(c, p);
+    if (rc != GRHD_OK) return rc;
+    launch_p2c(c->P, c->stream, c->p, c->u[c->ucur]);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return GRHD_OK;
+}
+
+int grhd_put_conserved(grhd_ctx *c, const double *u)
+{
+    if (!c || !u) return GRHD_ERR_INVALID_ARGUMENT;
+    const Params &P = c->P;
+    RefLayout L(P);
+    const int64_t ncell = (int64_t)P.nx * P.ny * P.nz;
+    std::vector<double> soa((size_t)ncell * NEQ);
+    for (int64_t t = 0; t < ncell; t++)
+        for (int v = 0; v < NEQ; v++) soa[(size_t)v * ncell + t] = u[(size_t)t * NEQ + v];
+    HIPCHK(c, hipMemcpyAsync(c->u[c->ucur], soa.data(), soa.size() * sizeof(double),
+                             hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return GRHD_OK;
+}
+
+int grhd_set_conserved(grhd_ctx *c, const double *u)
+{
+    int rc = grhd_put_conserved(c, u);
+    if (rc != GRHD_OK) return rc;
+    if (c->P.problem == 7) {  // prepare_case_boundary_data (conditions.f90:217-226)
+        std::vector<double> h;
+        rc = initial_conditions(c->P, c->cfg, h, c->err);
+        if (rc != GRHD_OK) return rc;
+        cache_injector_from(c, h.data());
+    }
+    rc = grhd_con2prim(c);
+    if (rc != GRHD_OK) return rc;
+    return grhd_boundaries(c);
+}
+
+int grhd_get_primitives(grhd_ctx *c, double *p)
+{
+    if (!c || !p) return GRHD_ERR_INVALID_ARGUMENT;
+    std::vector<double> soa((size_t)c->P.vstride * NEQ);
+    HIPCHK(c, hipMemcpyAsync(soa.data(), c->p, soa.size() * sizeof(double), hipMemcpyDeviceToHost,
+                             c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    soa_to_host_p(c->P, soa, p);
+    return GRHD_OK;
+}
+
+int grhd_get_conserved(grhd_ctx *c, double *u)
+{
+    if (!c || !u) return GRHD_ERR_INVALID_ARGUMENT;
+    const Params &P = c->P;
+    const int64_t ncell = (int64_t)P.nx * P.ny * P.nz;
+    std::vector<double> soa((size_t)ncell * NEQ);
+    HIPCHK(c, hipMemcpyAsync(soa.data(), c->u[c->ucur], soa.size() * sizeof(double),
+                             hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (int64_t t = 0; t < ncell; t++)
+        for (int v = 0; v < NEQ; v++) u[(size_t)t * NEQ + v] = soa[(size_t)v * ncell + t];
+    return GRHD_OK;
+}
+
+int grhd_compute_dt(grhd_ctx *c, double *dt)
+{
+    if (!c || !dt) return GRHD_ERR_INVALID_ARGUMENT;
+    const int rc = ensure_dt(c);
+    if (rc != GRHD_OK) return rc;
+    *dt = c->dt_cfl;
+    return GRHD_OK;
+}
+
+int grhd_step(grhd_ctx *c, double dt, int32_t nsteps)
+{
+    if (!c || nsteps < 0 || !std::isfinite(dt)) return GRHD_ERR_INVALID_ARGUMENT;
+    std::vector<double> dts((size_t)nsteps, dt);
+    return run_steps(c, dts.data(), nsteps);
+}
+
+int grhd_advance(grhd_ctx *c, double t_final, int32_t max_steps, double *t, int32_t *steps_done,
+                 double *last_dt)
+{
+    if (!c || !t || max_steps < 0) return GRHD_ERR_INVALID_ARGUMENT;
+    int rc = ensure_dt(c);
+    if (rc != GRHD_OK) return rc;
+    // grhd2.f90:179-184,229-230: dt, clamp to the final time, t advances first.
+    std::vector<double> dts;
+    double tt = *t, dt = 0.0;
+    while ((int)dts.size() < max_steps && tt < t_final) {
+        dt = c->dt_cfl;
+        if (tt + dt > t_final) dt = t_final - tt;
+        tt = tt + dt;
+        dts.push_back(dt);
+    }
+    rc = run_steps(c, dts.data(), (int)dts.size());
+    if (rc != GRHD_OK) return rc;
+    *t = tt;
+    if (steps_done) *steps_done = (int32_t)dts.size();
+    if (last_dt) *last_dt = dt;
+    return GRHD_OK;
+}
+
+int grhd_boundaries(grhd_ctx *c)
+{
+    if (!c) return GRHD_ERR_INVALID_ARGUMENT;
+    launch_boundaries(c->P, c->stream, c->p, c->injector);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return GRHD_OK;
+}
+
+int grhd_rhs(grhd_ctx *c, double *rhs)
+{
+    if (!c || !rhs) return GRHD_ERR_INVALID_ARGUMENT;
+    const Params &P = c->P;
+    const int64_t ncell = (int64_t)P.nx * P.ny * P.nz;
+    double *tmp = nullptr;
+    HIPCHK(c, hipMalloc(&tmp, (size_t)ncell * NEQ * sizeof(double)));
+    launch_boundaries(P, c->stream, c->p, c->injector);
+    launch_stage(P, c->stream, c->p, c->F, c->fstr, c->u[c->ucur], tmp, 0, 0.0, c->counters, c->mixed,
+                 c->injector, false);
+    std::vector<double> soa((size_t)ncell * NEQ);
+    hipError_t e = hipMemcpyAsync(soa.data(), tmp, soa.size() * sizeof(double), hipMemcpyDeviceToHost,
+                                  c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    (void)hipFree(tmp);
+    if (e != hipSuccess) return fail(c, GRHD_ERR_HIP, hipGetErrorString(e));
+    for (int64_t t = 0; t < ncell; t++)
+        for (int v = 0; v < NEQ; v++) rhs[(size_t)t * NEQ + v] = soa[(size_t)v * ncell + t];
+    return GRHD_OK;
+}
+
+int grhd_con2prim(grhd_ctx *c)
+{
+    if (!c) return GRHD_ERR_INVALID_ARGUMENT;
+    launch_c2p(c->P, c->stream, c->p, c->u[c->ucur], c->counters);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return GRHD_OK;
+}
+
+int grhd_counters(grhd_ctx *c, int64_t *out)
+{
+    if (!c || !out) return GRHD_ERR_INVALID_ARGUMENT;
+    unsigned long long h[4];
+    HIPCHK(c, hipMemcpyAsync(h, c->counters, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    out[0] = (int64_t)h[0];
+    out[1] = (int64_t)h[1];
+    out[2] = c->steps;
+    return GRHD_OK;
+}
+
+int grhd_set_stream(grhd_ctx *c, void *s)
+{
+    if (!c) return GRHD_ERR_INVALID_ARGUMENT;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (c->own_stream) (void)hipStreamDestroy(c->stream);
+    c->stream = (hipStream_t)s;
+    c->own_stream = false;
+    return GRHD_OK;
+}
+
+int grhd_synchronize(grhd_ctx *c)
+{
+    if (!c) return GRHD_ERR_INVALID_ARGUMENT;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return GRHD_OK;
+}
+
+int grhd_last_timing(grhd_ctx *c, double *ms, int64_t *launches)
+{
+    if (!c) return GRHD_ERR_INVALID_ARGUMENT;
+    HIPCHK(c, hipEventSynchronize(c->ev1));
+    float f = 0.0f;
+    HIPCHK(c, hipEventElapsedTime(&f, c->ev0, c->ev1));
+    if (ms) *ms = f;
+    if (launches) *launches = c->last_launches;
+    return GRHD_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,21 @@
+// grhd_kernels.hpp -- launch interface of the device kernels (grhd_kernels.hip).
+#pragma once
+#include "grhd_physics.hpp"
+
+namespace grhd {
+
+template <int MK>
+void launch_boundaries_mk(const Params &P, hipStream_t s, double *p, const double *injector);
+
+// One RK stage: face fluxes, cell update (stage 1..3; 0 = rhs only into u),
+// then the ghost fill when with_boundaries.
+void launch_stage(const Params &P, hipStream_t s, double *p, void *const F[3], const int64_t fstr[3],
+                  const double *u0, double *u, int stage, double dt, unsigned long long *counters,
+                  bool mixed, const double *injector, bool with_boundaries);
+void launch_boundaries(const Params &P, hipStream_t s, double *p, const double *injector);
+void launch_p2c(const Params &P, hipStream_t s, const double *p, double *u);
+void launch_c2p(const Params &P, hipStream_t s, double *p, double *u, unsigned long long *counters);
+// Returns the number of partial records (4 doubles each) written.
+int launch_cfl(const Params &P, hipStream_t s, double *partial);
+
+}  // namespace grhd

@@ -1,0 +1,1177 @@
+// grhd_physics.hpp -- point-wise physics of the GRHD2 update for gfx950.
+//
+// Everything here is a pure function of its arguments (no global state), usable
+// on the device and on the host (initial data).  The metric is evaluated
+// analytically where it is needed instead of being read from a stored cache:
+// the reference precomputes the same functions into ~1.7 KB/cell of cache
+// (initialization.f90:265-391); here each value costs a few flops in registers.
+//
+// Face quantities (reconstruction, Riemann solver) are templated on the
+// arithmetic type R (double for the FP64 path, float for the mixed path);
+// con2prim, sources and metric determinants are always double.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <float.h>
+#include <math.h>
+#include <stdint.h>
+
+#define GRHD_HD __host__ __device__ __forceinline__
+
+namespace grhd {
+
+enum : int { EQ_DE = 0, EQ_PR = 1, EQ_VX = 2, EQ_VY = 3, EQ_VZ = 4, NEQ = 5 };
+
+// Metric kinds: metric x geometry x radial map (metrics.f90:94-201).
+enum MetricKind : int {
+    MK_MINK_CART = 0,
+    MK_MINK_CYL = 1,
+    MK_MINK_SPH = 2,
+    MK_MINK_SPH_LOG = 3,
+    MK_EF = 4,
+    MK_EF_LOG = 5,
+    MK_KS = 6,
+    MK_KS_LOG = 7,
+};
+
+// Problem constants broadcast to every kernel (passed by value).
+struct Params {
+    int problem;
+    int mk;
+    int nx, ny, nz, ng;
+    int rec, rs, lim, sensor;
+    int polar;  // spherical / spheroidal geometry
+    int cyl;    // cylindrical geometry
+    double gam, g1, mass, spin;
+    double rho_floor, p_floor;
+    double x_min, dx, y_min, dy, z_min, dz;
+    double inv_dx, inv_dy, inv_dz;
+    // device storage of the ghosted primitives: storage index of reference
+    // index (i, j, k) is ((k + oz) * NYs + (j + oy)) * NXs + (i + ox)
+    int ox, oy, oz;
+    int NXs, NYs, NZs;
+    int64_t vstride;  // NXs * NYs * NZs
+    // boundary data
+    double jet_ambient_density, jet_ambient_pressure, jet_ambient_velocity;
+    double jet_nozzle_radius, jet_density, jet_pressure, jet_velocity;
+    double dust_accretion_constant;
+};
+
+GRHD_HD int64_t pindex(const Params &P, int i, int j, int k)
+{
+    return ((int64_t)(k + P.oz) * P.NYs + (j + P.oy)) * P.NXs + (i + P.ox);
+}
+
+// Coordinates of the cell-centred grid (initialization.f90:110-115, 191-206).
+GRHD_HD double xc(const Params &P, int i) { return P.x_min + ((double)i - 0.5) * P.dx; }
+GRHD_HD double xf(const Params &P, int i) { return P.x_min + (double)i * P.dx; }
+GRHD_HD double yc(const Params &P, int j) { return P.y_min + ((double)j - 0.5) * P.dy; }
+GRHD_HD double yf(const Params &P, int j) { return P.y_min + (double)j * P.dy; }
+GRHD_HD double zc(const Params &P, int k) { return P.z_min + ((double)k - 0.5) * P.dz; }
+
+template <typename R> struct Lim;
+template <> struct Lim<double> {
+    static GRHD_HD double tiny() { return DBL_MIN; }
+    static GRHD_HD double eps() { return DBL_EPSILON; }
+};
+template <> struct Lim<float> {
+    static GRHD_HD float tiny() { return FLT_MIN; }
+    static GRHD_HD float eps() { return FLT_EPSILON; }
+};
+
+template <typename R> GRHD_HD R rmax(R a, R b) { return a > b ? a : b; }
+template <typename R> GRHD_HD R rmin(R a, R b) { return a < b ? a : b; }
+template <typename R> GRHD_HD R sgn1(R a) { return copysign((R)1, a); }
+GRHD_HD bool finite(double a) { return isfinite(a); }
+GRHD_HD bool finite(float a) { return isfinite(a); }
+
+// ---------------------------------------------------------------- metric --
+// The 3+1 quantities on a face: lapse, shift, spatial metric, sqrt(det).
+struct FaceMetric {
+    double alpha, beta[3], g[3][3], det;
+};
+
+// Centre quantities: everything sources and con2prim read from the reference
+// cache (initialization.f90:283-317).  Only the lambda = 0 Christoffels enter
+// the sources (equations.f90:543-548), and the metric depends on (r, theta)
+// only, so derivatives are kept for directions 1 and 2.
+struct CentreMetric {
+    double alpha, beta[3], g[3][3], det, sqg;
+    double gm[4][4];
+    double ginv[3][3];
+    double dlna[4];
+    double ch0[4][4];    // Gamma^0_{mu nu}
+    double dg[4][4][2];  // d_k g_{mu nu}, k = r (0), theta (1)
+};
+
+// Basic metric (alpha, beta, gamma, det) of every kind.  Kerr-Schild:
+// metrics.f90:753-838 (physical r), 998-1085 (x = ln r); Eddington-Finkelstein:
+// 492-551, 612-670; Minkowski: 209-239, 265-294, 338-368, 419-450.
+template <int MK>
+GRHD_HD void metric_face(const Params &P, double xp, double yp, FaceMetric &m)
+{
+    for (int a = 0; a < 3; a++) {
+        m.beta[a] = 0.0;
+        for (int b = 0; b < 3; b++) m.g[a][b] = 0.0;
+    }
+    if (MK == MK_MINK_CART) {
+        m.alpha = 1.0;
+        m.g[0][0] = m.g[1][1] = m.g[2][2] = 1.0;
+        m.det = 1.0;
+    } else if (MK == MK_MINK_CYL) {
+        m.alpha = 1.0;
+        m.g[0][0] = 1.0;
+        m.g[1][1] = xp * xp;
+        m.g[2][2] = 1.0;
+        m.det = xp * xp;
+    } else if (MK == MK_MINK_SPH) {
+        const double r2 = xp * xp, s = sin(yp), s2 = s * s;
+        m.alpha = 1.0;
+        m.g[0][0] = 1.0;
+        m.g[1][1] = r2;
+        m.g[2][2] = r2 * s2;
+        m.det = r2 * r2 * s2;
+    } else if (MK == MK_MINK_SPH_LOG) {
+        const double r = exp(xp), r2 = r * r, s = sin(yp), s2 = s * s;
+        m.alpha = 1.0;
+        m.g[0][0] = r2;
+        m.g[1][1] = r2;
+        m.g[2][2] = r2 * s2;
+        m.det = r2 * r2 * r2 * s2;
+    } else if (MK == MK_EF || MK == MK_EF_LOG) {
+        const double M = P.mass;
+        const double r = (MK == MK_EF_LOG) ? exp(xp) : xp;
+        const double s = sin(yp);
+        m.alpha = 1.0 / sqrt(1.0 + 2.0 * M / r);
+        const double g11 = (MK == MK_EF_LOG) ? (1.0 + 2.0 * M / r) * (r * r) : 1.0 + 2.0 * M / r;
+        m.beta[0] = (MK == MK_EF_LOG) ? (2.0 * M / r / (1.0 + 2.0 * M / r)) / r
+                                      : 2.0 * M / r / (1.0 + 2.0 * M / r);
+        const double g22 = r * r, g33 = (r * r) * (s * s);
+        m.g[0][0] = g11;
+        m.g[1][1] = g22;
+        m.g[2][2] = g33;
+        m.det = g11 * g22 * g33;
+    } else {  // Kerr-Schild
+        const double M = P.mass, a = P.spin;
+        const double r = (MK == MK_KS_LOG) ? exp(xp) : xp;
+        const double s = sin(yp), c = cos(yp), s2 = s * s, c2 = c * c;
+        const double Sig = r * r + (a * a) * c2;
+        const double H = M * r / Sig;
+        m.alpha = 1.0 / sqrt(1.0 + 2.0 * H);
+        if (MK == MK_KS_LOG) {
+            m.beta[0] = (2.0 * H / (1.0 + 2.0 * H)) / r;
+            m.g[0][0] = (1.0 + 2.0 * H) * (r * r);
+            m.g[0][2] = -a * (1.0 + 2.0 * H) * s2 * r;
+            m.det = (Sig * Sig * s2 * (1.0 + 2.0 * H)) * (r * r);
+        } else {
+            m.beta[0] = 2.0 * H / (1.0 + 2.0 * H);
+            m.g[0][0] = 1.0 + 2.0 * H;
+            m.g[0][2] = -a * (1.0 + 2.0 * H) * s2;
+            m.det = (Sig * Sig) * s2 * (1.0 + 2.0 * H);
+        }
+        m.g[2][0] = m.g[0][2];
+        m.g[1][1] = Sig;
+        m.g[2][2] = (r * r + a * a + 2.0 * H * (a * a) * s2) * s2;
+    }
+}
+
+// Full centre metric: adds g^{mu nu}, d ln(alpha), d g_{mu nu} (r, theta),
+// Gamma^0_{mu nu}, gamma^{ij} by the ADM relation and sqrt(det)
+// (initialization.f90:283-317).  ny1: the grid has ny == 1, which zeroes the
+// theta derivatives (metrics.f90:309, 385, 609, 745, 919, 1178).
+template <int MK>
+GRHD_HD void metric_centre(const Params &P, double xp, double yp, CentreMetric &m)
+{
+    FaceMetric f;
+    metric_face<MK>(P, xp, yp, f);
+    m.alpha = f.alpha;
+    for (int a = 0; a < 3; a++) {
+        m.beta[a] = f.beta[a];
+        for (int b = 0; b < 3; b++) m.g[a][b] = f.g[a][b];
+    }
+    m.det = f.det;
+    for (int a = 0; a < 4; a++) {
+        m.dlna[a] = 0.0;
+        for (int b = 0; b < 4; b++) {
+            m.gm[a][b] = 0.0;
+            m.ch0[a][b] = 0.0;
+            m.dg[a][b][0] = 0.0;
+            m.dg[a][b][1] = 0.0;
+        }
+    }
+    const bool ny1 = (P.ny == 1);
+    if (MK == MK_MINK_CART) {
+        m.gm[0][0] = -1.0;
+        m.gm[1][1] = m.gm[2][2] = m.gm[3][3] = 1.0;
+    } else if (MK == MK_MINK_CYL) {  // metrics.f90:443-481
+        m.gm[0][0] = -1.0;
+        m.gm[1][1] = 1.0;
+        m.gm[2][2] = 1.0 / rmax(xp * xp, 1.0e-20);
+        m.gm[3][3] = 1.0;
+        m.dg[2][2][0] = 2.0 * xp;
+    } else if (MK == MK_MINK_SPH) {  // metrics.f90:287-310
+        const double r2 = xp * xp, s = sin(yp), c = cos(yp), s2 = s * s;
+        m.gm[0][0] = -1.0;
+        m.gm[1][1] = 1.0;
+        m.gm[2][2] = 1.0 / r2;
+        m.gm[3][3] = 1.0 / rmax(r2 * s2, 1.0e-30);
+        m.dg[2][2][0] = 2.0 * xp;
+        m.dg[3][3][0] = 2.0 * xp * (s * s);
+        m.dg[3][3][1] = ny1 ? 0.0 : 2.0 * (xp * xp) * s * c;
+    } else if (MK == MK_MINK_SPH_LOG) {  // metrics.f90:361-386
+        const double r = exp(xp), r2 = r * r, s = sin(yp), c = cos(yp), s2 = s * s;
+        const double e2 = exp(2.0 * xp);
+        m.gm[0][0] = -1.0;
+        m.gm[1][1] = 1.0 / r2;
+        m.gm[2][2] = 1.0 / r2;
+        m.gm[3][3] = 1.0 / rmax(r2 * s2, 1.0e-30);
+        m.dg[1][1][0] = 2.0 * e2;
+        m.dg[2][2][0] = 2.0 * e2;
+        m.dg[3][3][0] = 2.0 * e2 * (s * s);
+        m.dg[3][3][1] = ny1 ? 0.0 : 2.0 * e2 * s * c;
+    } else if (MK == MK_EF || MK == MK_EF_LOG) {  // metrics.f90:525-610, 649-746
+        const double M = P.mass;
+        const double r = (MK == MK_EF_LOG) ? exp(xp) : xp;
+        const double s = sin(yp), c = cos(yp);
+        const double a2 = m.alpha * m.alpha;
+        m.gm[0][0] = -1.0 / a2;
+        for (int i = 0; i < 3; i++) {
+            m.gm[0][i + 1] = m.beta[i] / a2;
+            m.gm[i + 1][0] = m.gm[0][i + 1];
+        }
+        m.gm[1][1] = 1.0 / m.g[0][0] - (m.beta[0] * m.beta[0]) / a2;
+        m.gm[2][2] = 1.0 / m.g[1][1] - (m.beta[1] * m.beta[1]) / a2;
+        m.gm[3][3] = 1.0 / m.g[2][2] - (m.beta[2] * m.beta[2]) / a2;
+        if (MK == MK_EF_LOG) {
+            m.dlna[1] = M / (r + 2.0 * M);
+            m.ch0[0][0] = 2.0 * (M * M) / (r * r * r);
+            m.ch0[0][1] = M * (1.0 + 2.0 * M / r) / r;
+            m.ch0[1][0] = m.ch0[0][1];
+            m.ch0[1][1] = 2.0 * M * (1.0 + M / r);
+            m.ch0[2][2] = -2.0 * M;
+            m.ch0[3][3] = m.ch0[2][2] * (s * s);
+            m.dg[0][0][0] = -2.0 * M / r;
+            m.dg[1][1][0] = 2.0 * (r * r) + 2.0 * M * r;
+            m.dg[2][2][0] = 2.0 * (r * r);
+            m.dg[3][3][0] = 2.0 * (r * r) * (s * s);
+            m.dg[3][3][1] = ny1 ? 0.0 : 2.0 * (r * r) * s * c;
+        } else {
+            m.dlna[1] = M / (r * r + 2.0 * M * r);
+            m.ch0[0][0] = 2.0 * (M * M) / (r * r * r);
+            m.ch0[0][1] = M * (1.0 + 2.0 * M / r) / (r * r);
+            m.ch0[1][0] = m.ch0[0][1];
+            m.ch0[1][1] = 2.0 * M * (1.0 + M / r) / (r * r);
+            m.ch0[2][2] = -2.0 * M;
+            m.ch0[3][3] = m.ch0[2][2] * (s * s);
+            m.dg[0][0][0] = -2.0 * M / (r * r);
+            m.dg[0][1][0] = -2.0 * M / (r * r);
+            m.dg[1][0][0] = m.dg[0][1][0];
+            m.dg[1][1][0] = -2.0 * M / (r * r);
+            m.dg[2][2][0] = 2.0 * r;
+            m.dg[3][3][0] = 2.0 * r * (s * s);
+            m.dg[3][3][1] = ny1 ? 0.0 : (r * r) * 2.0 * s * c;
+        }
+    } else {  // Kerr-Schild: metrics.f90:800-829 / 1045-1078, derivatives 844-921 / 1091-1180
+        const double M = P.mass, a = P.spin;
+        const bool lg = (MK == MK_KS_LOG);
+        const double r = lg ? exp(xp) : xp;
+        const double s = sin(yp), c = cos(yp), s2 = s * s, c2 = c * c;
+        const double Sig = r * r + (a * a) * c2;
+        const double H = M * r / Sig;
+        const double Del = r * r - 2.0 * M * r + a * a;
+        m.gm[0][0] = -(1.0 + 2.0 * H);
+        if (lg) {
+            m.gm[0][1] = (2.0 * H) / r;
+            m.gm[1][1] = (Del / Sig) / (r * r);
+            m.gm[1][3] = (a / Sig) / r;
+        } else {
+            m.gm[0][1] = 2.0 * H;
+            m.gm[1][1] = Del / Sig;
+            m.gm[1][3] = a / Sig;
+        }
+        m.gm[1][0] = m.gm[0][1];
+        m.gm[3][1] = m.gm[1][3];
+        m.gm[2][2] = 1.0 / Sig;
+        m.gm[3][3] = (fabs(s) > 1.0e-15) ? 1.0 / (Sig * s2) : 1.0 / (Sig * 1.0e-30);
+        const double dH_dr = M * ((a * a) * c2 - r * r) / (Sig * Sig);
+        const double dH_dth = 2.0 * M * r * (a * a) * s * c / (Sig * Sig);
+        m.dlna[1] = lg ? -(dH_dr * r) / (1.0 + 2.0 * H) : -dH_dr / (1.0 + 2.0 * H);
+        m.dlna[2] = -dH_dth / (1.0 + 2.0 * H);
+        double (*dg)[4][2] = m.dg;
+        const double dS_dth = -2.0 * (a * a) * s * c;
+        if (lg) {
+            const double gtr = 2.0 * H, grr = 1.0 + 2.0 * H, grp = -a * (1.0 + 2.0 * H) * s2;
+            dg[0][0][0] = r * (2.0 * dH_dr);
+            dg[0][1][0] = r * gtr + (r * r) * (2.0 * dH_dr);
+            dg[0][3][0] = r * (-2.0 * a * s2 * dH_dr);
+            dg[1][1][0] = 2.0 * (r * r) * grr + (r * r * r) * (2.0 * dH_dr);
+            dg[1][3][0] = r * grp + (r * r) * (-2.0 * a * s2 * dH_dr);
+            dg[2][2][0] = 2.0 * (r * r);
+            dg[3][3][0] = r * (2.0 * r * s2 + 2.0 * (a * a) * (s2 * s2) * dH_dr);
+            if (!ny1) {
+                dg[0][0][1] = 2.0 * dH_dth;
+                dg[0][1][1] = r * (2.0 * dH_dth);
+                dg[0][3][1] = -2.0 * a * (s2 * dH_dth + 2.0 * H * s * c);
+                dg[1][1][1] = (r * r) * (2.0 * dH_dth);
+                dg[1][3][1] = r * (-a * ((1.0 + 2.0 * H) * 2.0 * s * c + s2 * 2.0 * dH_dth));
+                dg[2][2][1] = dS_dth;
+                dg[3][3][1] = 2.0 * (r * r + a * a) * s * c + 2.0 * (a * a) * (s2 * s2) * dH_dth +
+                              8.0 * (a * a) * H * s * c * s2;
+            }
+        } else {
+            dg[0][0][0] = 2.0 * dH_dr;
+            dg[0][1][0] = 2.0 * dH_dr;
+            dg[0][3][0] = -2.0 * a * s2 * dH_dr;
+            dg[1][1][0] = 2.0 * dH_dr;
+            dg[1][3][0] = -a * s2 * (2.0 * dH_dr);
+            dg[2][2][0] = 2.0 * r;
+            dg[3][3][0] = 2.0 * r * s2 + 2.0 * (a * a) * (s2 * s2) * dH_dr;
+            if (!ny1) {
+                dg[0][0][1] = 2.0 * dH_dth;
+                dg[0][1][1] = 2.0 * dH_dth;
+                dg[0][3][1] = -2.0 * a * (s2 * dH_dth + 2.0 * H * s * c);
+                dg[1][1][1] = 2.0 * dH_dth;
+                dg[1][3][1] = -a * ((1.0 + 2.0 * H) * 2.0 * s * c + s2 * 2.0 * dH_dth);
+                dg[2][2][1] = dS_dth;
+                dg[3][3][1] = 2.0 * (r * r + a * a) * s * c + 2.0 * (a * a) * (s2 * s2) * dH_dth +
+                              8.0 * (a * a) * H * s * c * s2;
+            }
+        }
+        dg[1][0][0] = dg[0][1][0];
+        dg[3][0][0] = dg[0][3][0];
+        dg[3][1][0] = dg[1][3][0];
+        dg[1][0][1] = dg[0][1][1];
+        dg[3][0][1] = dg[0][3][1];
+        dg[3][1][1] = dg[1][3][1];
+        // Gamma^0_{mu nu} = 1/2 g^{0 rho} (d_mu g_{rho nu} + d_nu g_{rho mu} - d_rho g_{mu nu}),
+        // accumulated over rho as in metrics.f90:1203-1241 (lambda = 0 row).
+        for (int mu = 0; mu < 4; mu++)
+            for (int nu = mu; nu < 4; nu++) {
+                double acc = 0.0;
+                for (int rho = 0; rho < 4; rho++) {
+                    const double dmu = (mu == 1 || mu == 2) ? dg[rho][nu][mu - 1] : 0.0;
+                    const double dnu = (nu == 1 || nu == 2) ? dg[rho][mu][nu - 1] : 0.0;
+                    const double drho = (rho == 1 || rho == 2) ? dg[mu][nu][rho - 1] : 0.0;
+                    acc = acc + 0.5 * m.gm[0][rho] * (dmu + dnu - drho);
+                }
+                m.ch0[mu][nu] = acc;
+                m.ch0[nu][mu] = acc;
+            }
+    }
+    for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++)
+            m.ginv[a][b] = m.gm[a + 1][b + 1] + (m.beta[a] * m.beta[b]) / (m.alpha * m.alpha);
+    m.sqg = sqrt(m.det);
+}
+
+// Lighter centre metric for con2prim and prim_to_cons: gamma_ij, gamma^ij, sqrt(det).
+template <int MK>
+GRHD_HD void metric_c2p(const Params &P, double xp, double yp, double g[3][3], double ginv[3][3],
+                        double &sqg, double &alpha, double beta[3])
+{
+    CentreMetric m;
+    metric_centre<MK>(P, xp, yp, m);
+    for (int a = 0; a < 3; a++) {
+        beta[a] = m.beta[a];
+        for (int b = 0; b < 3; b++) {
+            g[a][b] = m.g[a][b];
+            ginv[a][b] = m.ginv[a][b];
+        }
+    }
+    sqg = m.sqg;
+    alpha = m.alpha;
+}
+
+// ------------------------------------------------------- equations (face) --
+// Prelude shared by prim_to_cons, calc_fluxes and the wave speeds:
+// v^2 = gamma_ij v^i v^j with the causal clamp to v_max (equations.f90:49-63).
+template <typename R>
+GRHD_HD R vsq_clamp(const R g[3][3], R v[3])
+{
+    const R vmax = (R)(1.0 - 1.0e-5);
+    R vsq = 0;
+    for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++) vsq = vsq + g[a][b] * v[a] * v[b];
+    if (vsq >= vmax) {
+        v[0] = v[0] * sqrt(vmax / vsq);
+        v[1] = v[1] * sqrt(vmax / vsq);
+        v[2] = v[2] * sqrt(vmax / vsq);
+        vsq = vmax;
+    }
+    return vsq;
+}
+
+template <typename R>
+GRHD_HD void lower(const R g[3][3], const R v[3], R vc[3])
+{
+    for (int a = 0; a < 3; a++) {
+        vc[a] = 0;
+        for (int b = 0; b < 3; b++) vc[a] = vc[a] + g[a][b] * v[b];
+    }
+}
+
+// prim_to_cons (equations.f90:30-88) / prim_to_cons_face (fluxes.f90:76-122).
+template <typename R>
+GRHD_HD void p2c(R g1, const R q[NEQ], const R g[3][3], R sqg, R U[NEQ])
+{
+    R v[3] = {q[EQ_VX], q[EQ_VY], q[EQ_VZ]}, vc[3];
+    const R vsq = vsq_clamp(g, v);
+    lower(g, v, vc);
+    const R h = (R)1 + g1 * q[EQ_PR] / q[EQ_DE];
+    const R W = (R)1 / sqrt((R)1 - vsq);
+    U[EQ_DE] = q[EQ_DE] * W;
+    U[EQ_PR] = q[EQ_DE] * h * (W * W) - q[EQ_PR] - q[EQ_DE] * W;
+    U[EQ_VX] = q[EQ_DE] * h * (W * W) * vc[0];
+    U[EQ_VY] = q[EQ_DE] * h * (W * W) * vc[1];
+    U[EQ_VZ] = q[EQ_DE] * h * (W * W) * vc[2];
+    for (int e = 0; e < NEQ; e++) U[e] = U[e] * sqg;
+}
+
+// Face metric in the face arithmetic.
+template <typename R> struct FaceR {
+    R alpha, beta[3], g[3][3], sqg;
+};
+
+// calc_fluxes (equations.f90:434-502), DIR = 1..3.
+template <typename R, int DIR>
+GRHD_HD void phys_flux(R g1, const R q[NEQ], const FaceR<R> &f, R F[NEQ])
+{
+    R v[3] = {q[EQ_VX], q[EQ_VY], q[EQ_VZ]}, vc[3];
+    const R vsq = vsq_clamp(f.g, v);
+    lower(f.g, v, vc);
+    const R vn = v[DIR - 1], bn = f.beta[DIR - 1];
+    const R h = (R)1 + g1 * q[EQ_PR] / q[EQ_DE];
+    const R W = (R)1 / sqrt((R)1 - vsq);
+    const R veff = f.alpha * vn - bn;
+    F[EQ_DE] = q[EQ_DE] * W * veff;
+    F[EQ_PR] = q[EQ_DE] * h * (W * W) * veff + q[EQ_PR] * bn - F[EQ_DE];
+    F[EQ_VX] = q[EQ_DE] * h * (W * W) * vc[0] * veff + (DIR == 1 ? f.alpha * q[EQ_PR] : (R)0);
+    F[EQ_VY] = q[EQ_DE] * h * (W * W) * vc[1] * veff + (DIR == 2 ? f.alpha * q[EQ_PR] : (R)0);
+    F[EQ_VZ] = q[EQ_DE] * h * (W * W) * vc[2] * veff + (DIR == 3 ? f.alpha * q[EQ_PR] : (R)0);
+    for (int e = 0; e < NEQ; e++) F[e] = F[e] * f.sqg;
+}
+
+// Characteristic speeds: SRHD for Minkowski+Cartesian (fluxes.f90:669-720),
+// GRHD otherwise (fluxes.f90:724-788; the n_steps==0 diagnostic stop is not
+// reproduced).
+template <typename R, int DIR, bool SRHD>
+GRHD_HD void wave_speeds(R g1, R gam, const R q[NEQ], const FaceR<R> &f, R &wmin, R &wmax)
+{
+    R v[3] = {q[EQ_VX], q[EQ_VY], q[EQ_VZ]};
+    const R vsq = vsq_clamp(f.g, v);
+    const R enth = (R)1 + g1 * q[EQ_PR] / q[EQ_DE];
+    const R cs2 = gam * q[EQ_PR] / (q[EQ_DE] * enth);
+    const R cs = sqrt(cs2);
+    const R vn = v[DIR - 1];
+    R e1, e2, e3;
+    if (SRHD) {
+        const R t1 = (R)1 / ((R)1 - vsq * cs2);
+        const R t2 = cs * sqrt(((R)1 - vsq) * ((R)1 / t1 - (vn * vn) * ((R)1 - cs2)));
+        e1 = vn;
+        e2 = t1 * (vn * ((R)1 - cs2) + t2);
+        e3 = t1 * (vn * ((R)1 - cs2) - t2);
+    } else {
+        const R(*g)[3] = f.g;
+        const R det = g[0][0] * (g[1][1] * g[2][2] - g[1][2] * g[2][1]) -
+                      g[0][1] * (g[1][0] * g[2][2] - g[1][2] * g[2][0]) +
+                      g[0][2] * (g[1][0] * g[2][1] - g[1][1] * g[2][0]);
+        R gii;
+        if (DIR == 1) gii = (g[1][1] * g[2][2] - g[1][2] * g[2][1]) / det;
+        else if (DIR == 2) gii = (g[0][0] * g[2][2] - g[0][2] * g[2][0]) / det;
+        else gii = (g[0][0] * g[1][1] - g[0][1] * g[1][0]) / det;
+        const R bn = f.beta[DIR - 1];
+        const R t1 = f.alpha / ((R)1 - vsq * cs2);
+        const R t2 = cs * sqrt(rmax((R)0, ((R)1 - vsq) * (gii * ((R)1 - vsq * cs2) -
+                                                        (vn * vn) * ((R)1 - cs2))));
+        e1 = f.alpha * vn - bn;
+        e2 = t1 * (vn * ((R)1 - cs2) + t2) - bn;
+        e3 = t1 * (vn * ((R)1 - cs2) - t2) - bn;
+    }
+    wmax = rmax(rmax(rmax((R)0, e1), e2), e3);
+    wmin = rmin(rmin(rmin((R)0, e1), e2), e3);
+}
+
+// calc_hlle_fluxes (fluxes.f90:125-156).
+template <typename R, int DIR, bool SRHD>
+GRHD_HD void hlle(R g1, R gam, const R qL[NEQ], const R qR[NEQ], const FaceR<R> &f, R F[NEQ])
+{
+    R fL[NEQ], fR[NEQ], uL[NEQ], uR[NEQ], lLmin, lLmax, lRmin, lRmax;
+    phys_flux<R, DIR>(g1, qL, f, fL);
+    phys_flux<R, DIR>(g1, qR, f, fR);
+    p2c<R>(g1, qL, f.g, f.sqg, uL);
+    p2c<R>(g1, qR, f.g, f.sqg, uR);
+    wave_speeds<R, DIR, SRHD>(g1, gam, qL, f, lLmin, lLmax);
+    wave_speeds<R, DIR, SRHD>(g1, gam, qR, f, lRmin, lRmax);
+    const R ap = rmax(rmax((R)0, lLmax), lRmax);
+    const R am = rmin(rmin((R)0, lLmin), lRmin);
+    if (fabs(ap - am) < (R)1.0e-14) {
+        for (int e = 0; e < NEQ; e++) F[e] = 0;
+    } else {
+        for (int e = 0; e < NEQ; e++)
+            F[e] = (ap * fL[e] - am * fR[e] + ap * am * (uR[e] - uL[e])) / (ap - am);
+    }
+}
+
+// ---------------------------------------------------------------- HLLC --
+template <typename R>
+GRHD_HD R metric_dot(const R a[3], const R b[3], const R g[3][3])
+{
+    R s = 0;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) s = s + g[i][j] * a[i] * b[j];
+    return s;
+}
+
+// build_face_orthonormal_frame (fluxes.f90:402-473): e_(1) normal to the
+// face, Gram-Schmidt in the gamma_ij inner product.  Returns the coframe
+// cof[a][i] = e^(a)_i and gamma^dd.
+template <typename R, int DIR>
+GRHD_HD bool face_frame(const R g[3][3], R cof[3][3], R &gdd)
+{
+    R ginv[3][3], frame[3][3];
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            cof[i][j] = 0;
+            frame[i][j] = 0;
+            if (!finite(g[i][j])) return false;
+        }
+    gdd = 0;
+    const R det = g[0][0] * (g[1][1] * g[2][2] - g[1][2] * g[2][1]) -
+                  g[0][1] * (g[1][0] * g[2][2] - g[1][2] * g[2][0]) +
+                  g[0][2] * (g[1][0] * g[2][1] - g[1][1] * g[2][0]);
+    if (!finite(det) || det <= (R)1.0e-30) return false;
+    ginv[0][0] = (g[1][1] * g[2][2] - g[1][2] * g[2][1]) / det;
+    ginv[0][1] = (g[0][2] * g[2][1] - g[0][1] * g[2][2]) / det;
+    ginv[0][2] = (g[0][1] * g[1][2] - g[0][2] * g[1][1]) / det;
+    ginv[1][0] = (g[1][2] * g[2][0] - g[1][0] * g[2][2]) / det;
+    ginv[1][1] = (g[0][0] * g[2][2] - g[0][2] * g[2][0]) / det;
+    ginv[1][2] = (g[0][2] * g[1][0] - g[0][0] * g[1][2]) / det;
+    ginv[2][0] = (g[1][0] * g[2][1] - g[1][1] * g[2][0]) / det;
+    ginv[2][1] = (g[0][1] * g[2][0] - g[0][0] * g[2][1]) / det;
+    ginv[2][2] = (g[0][0] * g[1][1] - g[0][1] * g[1][0]) / det;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++)
+            if (!finite(ginv[i][j])) return false;
+    const int d = DIR - 1;
+    gdd = ginv[d][d];
+    if (gdd <= (R)1.0e-30 || !finite(gdd)) return false;
+    for (int i = 0; i < 3; i++) frame[i][0] = ginv[i][d] / sqrt(gdd);
+    const int t0 = (DIR == 1) ? 1 : (DIR == 2) ? 2 : 0;
+    const int t1 = (DIR == 1) ? 2 : (DIR == 2) ? 0 : 1;
+    for (int b = 1; b < 3; b++) {
+        R seed[3] = {0, 0, 0};
+        seed[b == 1 ? t0 : t1] = 1;
+        for (int a = 0; a < b; a++) {
+            const R fa[3] = {frame[0][a], frame[1][a], frame[2][a]};
+            const R proj = metric_dot(seed, fa, g);
+            for (int i = 0; i < 3; i++) seed[i] = seed[i] - proj * fa[i];
+        }
+        const R nsq = metric_dot(seed, seed, g);
+        if (!finite(nsq) || nsq <= (R)1.0e-28) return false;
+        for (int i = 0; i < 3; i++) frame[i][b] = seed[i] / sqrt(nsq);
+    }
+    for (int a = 0; a < 3; a++)
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) cof[a][i] = cof[a][i] + g[i][j] * frame[j][a];
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++)
+            if (!finite(frame[i][j]) || !finite(cof[i][j])) return false;
+    return true;
+}
+
+// calc_srhd_local_state (fluxes.f90:509-568).
+template <typename R>
+GRHD_HD bool srhd_local(R g1, R gam, const R ql[NEQ], R U[NEQ], R F[NEQ], R &wmin, R &wmax)
+{
+    const R vmax = (R)(1.0 - 1.0e-5);
+    for (int e = 0; e < NEQ; e++) {
+        U[e] = 0;
+        F[e] = 0;
+    }
+    wmin = 0;
+    wmax = 0;
+    const R rho = ql[EQ_DE], pr = ql[EQ_PR];
+    const R v0 = ql[EQ_VX], v1 = ql[EQ_VY], v2 = ql[EQ_VZ];
+    if (!finite(rho) || !finite(pr) || !finite(v0) || !finite(v1) || !finite(v2) || rho <= (R)0 ||
+        pr < (R)0)
+        return false;
+    const R vsq = v0 * v0 + v1 * v1 + v2 * v2;
+    if (!finite(vsq) || vsq < (R)0) return false;
+    if (vsq >= vmax) return false;
+    const R h = (R)1 + g1 * pr / rho;
+    if (!finite(h) || h <= (R)0) return false;
+    const R cs2 = gam * pr / (rho * h);
+    if (!finite(cs2) || cs2 < (R)0 || cs2 >= (R)1) return false;
+    const R W = (R)1 / sqrt((R)1 - vsq);
+    const R D = rho * W;
+    const R tau = rho * h * (W * W) - pr - D;
+    const R m0 = rho * h * (W * W) * v0, m1 = rho * h * (W * W) * v1, m2 = rho * h * (W * W) * v2;
+    U[EQ_DE] = D;
+    U[EQ_PR] = tau;
+    U[EQ_VX] = m0;
+    U[EQ_VY] = m1;
+    U[EQ_VZ] = m2;
+    F[EQ_DE] = D * v0;
+    F[EQ_PR] = m0 - D * v0;
+    F[EQ_VX] = m0 * v0 + pr;
+    F[EQ_VY] = m1 * v0;
+    F[EQ_VZ] = m2 * v0;
+    const R den = (R)1 - vsq * cs2;
+    const R rad = ((R)1 - vsq) * ((R)1 - vsq * cs2 - (v0 * v0) * ((R)1 - cs2));
+    if (den <= (R)1.0e-14 || rad < (R)-1.0e-12) return false;
+    const R ac = sqrt(cs2) * sqrt(rmax((R)0, rad));
+    wmin = (v0 * ((R)1 - cs2) - ac) / den;
+    wmax = (v0 * ((R)1 - cs2) + ac) / den;
+    for (int e = 0; e < NEQ; e++)
+        if (!finite(U[e]) || !finite(F[e])) return false;
+    if (!finite(wmin) || !finite(wmax)) return false;
+    return true;
+}
+
+// conservative_state_is_admissible (fluxes.f90:571-591).
+template <typename R>
+GRHD_HD bool admissible(const R U[NEQ])
+{
+    for (int e = 0; e < NEQ; e++)
+        if (!finite(U[e])) return false;
+    if (U[EQ_DE] <= (R)0) return false;
+    const R E = U[EQ_PR] + U[EQ_DE];
+    const R msq = U[EQ_VX] * U[EQ_VX] + U[EQ_VY] * U[EQ_VY] + U[EQ_VZ] * U[EQ_VZ];
+    const R scale = rmax(rmax(Lim<R>::tiny(), E * E), U[EQ_DE] * U[EQ_DE] + msq);
+    if (E <= (R)0) return false;
+    if (E * E - U[EQ_DE] * U[EQ_DE] - msq <= (R)1.0e-13 * scale) return false;
+    return true;
+}
+
+// local_flux_to_coordinate (fluxes.f90:596-624).
+template <typename R>
+GRHD_HD bool local_to_coord(const R Ul[NEQ], const R Fl[NEQ], R w, R alpha, R sqg, R gdd,
+                            const R cof[3][3], R F[NEQ])
+{
+    R mf[NEQ];
+    for (int e = 0; e < NEQ; e++) {
+        F[e] = 0;
+        mf[e] = Fl[e] - w * Ul[e];
+    }
+    const R fac = sqg * alpha * sqrt(gdd);
+    if (!finite(fac)) return false;
+    for (int e = 0; e < NEQ; e++)
+        if (!finite(mf[e])) return false;
+    F[EQ_DE] = fac * mf[EQ_DE];
+    F[EQ_PR] = fac * mf[EQ_PR];
+    for (int i = 0; i < 3; i++)
+        for (int a = 0; a < 3; a++) F[EQ_VX + i] = F[EQ_VX + i] + fac * cof[a][i] * mf[EQ_VX + a];
+    for (int e = 0; e < NEQ; e++)
+        if (!finite(F[e])) return false;
+    return true;
+}
+
+// calc_hllc_fluxes (fluxes.f90:174-397).  Returns false where the reference
+// falls back to HLLE; the caller then evaluates HLLE.
+template <typename R, int DIR>
+GRHD_HD bool hllc_try(R g1, R gam, const R qL[NEQ], const R qR[NEQ], const FaceR<R> &f, R F[NEQ])
+{
+    R cof[3][3], gdd, qLh[NEQ], qRh[NEQ], uL[NEQ], uR[NEQ], fL[NEQ], fR[NEQ];
+    R lLmin, lLmax, lRmin, lRmax;
+    const R alpha = f.alpha, sqg = f.sqg;
+    if (alpha <= (R)0 || sqg <= (R)0 || !finite(alpha) || !finite(sqg)) return false;
+    if (!face_frame<R, DIR>(f.g, cof, gdd)) return false;
+    for (int e = 0; e < NEQ; e++) {
+        qLh[e] = qL[e];
+        qRh[e] = qR[e];
+    }
+    for (int a = 0; a < 3; a++) {
+        qLh[EQ_VX + a] = 0;
+        qRh[EQ_VX + a] = 0;
+        for (int i = 0; i < 3; i++) {
+            qLh[EQ_VX + a] = qLh[EQ_VX + a] + cof[a][i] * qL[EQ_VX + i];
+            qRh[EQ_VX + a] = qRh[EQ_VX + a] + cof[a][i] * qR[EQ_VX + i];
+        }
+    }
+    if (!srhd_local(g1, gam, qLh, uL, fL, lLmin, lLmax)) return false;
+    if (!srhd_local(g1, gam, qRh, uR, fR, lRmin, lRmax)) return false;
+    const R ll = rmin(lLmin, lRmin), lr = rmax(lLmax, lRmax);
+    const R w = f.beta[DIR - 1] / (alpha * sqrt(gdd));
+    if (!finite(w) || lr - ll <= (R)1.0e-13) return false;
+    if (w <= ll) return local_to_coord(uL, fL, w, alpha, sqg, gdd, cof, F);
+    if (w >= lr) return local_to_coord(uR, fR, w, alpha, sqg, gdd, cof, F);
+    R uh[NEQ], fh[NEQ];
+    R den = lr - ll;
+    for (int e = 0; e < NEQ; e++) {
+        uh[e] = (lr * uR[e] - ll * uL[e] + fL[e] - fR[e]) / den;
+        fh[e] = (lr * fL[e] - ll * fR[e] + lr * ll * (uR[e] - uL[e])) / den;
+    }
+    for (int e = 0; e < NEQ; e++)
+        if (!finite(uh[e]) || !finite(fh[e])) return false;
+    const R Eh = uh[EQ_PR] + uh[EQ_DE], FEh = fh[EQ_PR] + fh[EQ_DE];
+    const R A = FEh, B = -(Eh + fh[EQ_VX]), Cq = uh[EQ_VX];
+    R disc = B * B - (R)4 * A * Cq;
+    const R dscale = rmax(rmax(Lim<R>::tiny(), B * B), fabs((R)4 * A * Cq));
+    if (!finite(disc) || disc < (R)-1.0e-12 * dscale) return false;
+    disc = rmax((R)0, disc);
+    const R cscale = rmax(rmax(Lim<R>::tiny(), fabs(B)), fabs(Cq));
+    const bool quad = fabs(A) > (R)1.0e-13 * cscale;
+    R ls;
+    if (!quad) {
+        if (fabs(B) <= (R)1.0e-14 * rmax(Lim<R>::tiny(), fabs(Cq))) return false;
+        ls = -Cq / B;
+    } else {
+        den = -B - sqrt(disc);
+        if (fabs(den) > (R)1.0e-14 * rmax(rmax(Lim<R>::tiny(), fabs(B)), sqrt(disc))) {
+            ls = den / ((R)2 * A);
+        } else {
+            den = -B + sqrt(disc);
+            if (fabs(den) <= (R)1.0e-14 * rmax(rmax(Lim<R>::tiny(), fabs(B)), sqrt(disc)))
+                return false;
+            ls = ((R)2 * Cq) / den;
+        }
+    }
+    const R rtol = (R)1.0e-10 * rmax(rmax((R)1, fabs(ll)), fabs(lr));
+    bool valid = finite(ls) && ls >= ll - rtol && ls <= lr + rtol && fabs(ls) <= (R)1 + rtol;
+    if (!valid && quad) {
+        const R ra = (-B + sqrt(disc)) / ((R)2 * A);
+        if (finite(ra) && ra >= ll - rtol && ra <= lr + rtol && fabs(ra) <= (R)1 + rtol) {
+            ls = ra;
+            valid = true;
+        }
+    }
+    if (!valid) return false;
+    ls = rmin(lr, rmax(ll, ls));
+    const R AL = ll * (uL[EQ_PR] + uL[EQ_DE]) - uL[EQ_VX];
+    const R BL = uL[EQ_VX] * (ll - qLh[EQ_VX]) - qLh[EQ_PR];
+    const R AR = lr * (uR[EQ_PR] + uR[EQ_DE]) - uR[EQ_VX];
+    const R BR = uR[EQ_VX] * (lr - qRh[EQ_VX]) - qRh[EQ_PR];
+    den = (R)1 - ll * ls;
+    if (fabs(den) <= (R)1.0e-13) return false;
+    const R psL = (AL * ls - BL) / den;
+    den = (R)1 - lr * ls;
+    if (fabs(den) <= (R)1.0e-13) return false;
+    const R psR = (AR * ls - BR) / den;
+    const R sscale = rmax(rmax(rmax(rmax(Lim<R>::tiny(), fabs(uL[EQ_PR] + uL[EQ_DE])),
+                                    fabs(uR[EQ_PR] + uR[EQ_DE])),
+                               fabs(uL[EQ_VX])),
+                          fabs(uR[EQ_VX]));
+    const R ptol = (R)1.0e-8 * rmax(rmax(rmax(rmax(Lim<R>::tiny(), fabs(qLh[EQ_PR])),
+                                              fabs(qRh[EQ_PR])),
+                                         fabs(psL)),
+                                    fabs(psR)) +
+                   (R)1.0e2 * Lim<R>::eps() * sscale;
+    if (!finite(psL) || !finite(psR) || fabs(psL - psR) > ptol) return false;
+    const R ps = (R)0.5 * (psL + psR);
+    if (!finite(ps) || fabs(ll - ls) <= (R)1.0e-13 || fabs(lr - ls) <= (R)1.0e-13) return false;
+    const R pv[NEQ] = {0, ls, 1, 0, 0};
+    R usL[NEQ], usR[NEQ], fsL[NEQ], fsR[NEQ];
+    for (int e = 0; e < NEQ; e++) {
+        usL[e] = (ll * uL[e] - fL[e] + ps * pv[e]) / (ll - ls);
+        usR[e] = (lr * uR[e] - fR[e] + ps * pv[e]) / (lr - ls);
+    }
+    for (int e = 0; e < NEQ; e++) {
+        fsL[e] = fL[e] + ll * (usL[e] - uL[e]);
+        fsR[e] = fR[e] + lr * (usR[e] - uR[e]);
+    }
+    if (!admissible(usL) || !admissible(usR)) return false;
+    for (int e = 0; e < NEQ; e++)
+        if (!finite(fsL[e]) || !finite(fsR[e])) return false;
+    if (w <= ls) return local_to_coord(usL, fsL, w, alpha, sqg, gdd, cof, F);
+    return local_to_coord(usR, fsR, w, alpha, sqg, gdd, cof, F);
+}
+
+// resolve_riemann_problem (fluxes.f90:56-72).
+template <typename R, int DIR, bool SRHD>
+GRHD_HD void riemann(int rs, R g1, R gam, const R qL[NEQ], const R qR[NEQ], const FaceR<R> &f,
+                     R F[NEQ])
+{
+    if (rs == 2) {
+        if (!(DIR >= 1 && DIR <= 3)) {
+            hlle<R, DIR, SRHD>(g1, gam, qL, qR, f, F);
+            return;
+        }
+        if (hllc_try<R, DIR>(g1, gam, qL, qR, f, F)) return;
+    }
+    hlle<R, DIR, SRHD>(g1, gam, qL, qR, f, F);
+}
+
+// -------------------------------------------------------- reconstruction --
+template <typename R> GRHD_HD R minmod2(R a, R b)
+{   // reconstruction.f90:28-32
+    return (R)0.5 * (sgn1(a) + sgn1(b)) * rmin(fabs(a), fabs(b));
+}
+template <typename R> GRHD_HD R minmod3(R a, R b, R c)
+{   // reconstruction.f90:34-38
+    return sgn1(a) * rmax((R)0, rmin(rmin(fabs(a), sgn1(a) * b), sgn1(a) * c));
+}
+template <typename R> GRHD_HD R minmod4(R a, R b, R c, R d)
+{   // reconstruction.f90:40-45
+    return (R)0.125 * (sgn1(a) + sgn1(b)) * fabs(sgn1(a) + sgn1(c)) * fabs(sgn1(a) + sgn1(d)) *
+           rmin(rmin(rmin(fabs(a), fabs(b)), fabs(c)), fabs(d));
+}
+// weno3_interface with the "dynamic epsilon" (reconstruction.f90:141-181).
+template <typename R> GRHD_HD R weno3(R qm1, R q0, R qp1)
+{
+    const R d0 = (R)(2.0 / 3.0), d1 = (R)(1.0 / 3.0), eps_min = (R)1.0e-14;
+    const R p0 = (R)0.5 * q0 + (R)0.5 * qp1;
+    const R p1 = (R)-0.5 * qm1 + (R)1.5 * q0;
+    const R b0 = (q0 - qp1) * (q0 - qp1);
+    const R b1 = (qm1 - q0) * (qm1 - q0);
+    const R sens = fabs(b0 - b1) / (rmax(b0, b1) + eps_min);
+    const R epsh = eps_min + (qm1 * qm1 + q0 * q0 + qp1 * qp1) * exp((R)-10 * sens);
+    const R a0 = d0 / ((epsh + b0) * (epsh + b0));
+    const R a1 = d1 / ((epsh + b1) * (epsh + b1));
+    const R w0 = a0 / (a0 + a1), w1 = a1 / (a0 + a1);
+    return w0 * p0 + w1 * p1;
+}
+// weno5_interface, WENO5-Z (reconstruction.f90:187-215).
+template <typename R> GRHD_HD R weno5(R qm2, R qm1, R q0, R qp1, R qp2)
+{
+    const R eps = (R)1.0e-14, d0 = (R)0.3, d1 = (R)0.6, d2 = (R)0.1;
+    const R t0 = q0 - (R)2 * qp1 + qp2, u0 = (R)3 * q0 - (R)4 * qp1 + qp2;
+    const R t1 = qm1 - (R)2 * q0 + qp1, u1 = qm1 - qp1;
+    const R t2 = qm2 - (R)2 * qm1 + q0, u2 = qm2 - (R)4 * qm1 + (R)3 * q0;
+    const R c13 = (R)(13.0 / 12.0);
+    const R b0 = c13 * (t0 * t0) + (R)0.25 * (u0 * u0);
+    const R b1 = c13 * (t1 * t1) + (R)0.25 * (u1 * u1);
+    const R b2 = c13 * (t2 * t2) + (R)0.25 * (u2 * u2);
+    const R tau5 = fabs(b0 - b2);
+    const R a0 = d0 * ((R)1 + (tau5 / (eps + b0)));
+    const R a1 = d1 * ((R)1 + (tau5 / (eps + b1)));
+    const R a2 = d2 * ((R)1 + (tau5 / (eps + b2)));
+    const R w0 = a0 / (a0 + a1 + a2), w1 = a1 / (a0 + a1 + a2), w2 = a2 / (a0 + a1 + a2);
+    const R p0 = (R)(1.0 / 3.0) * q0 + (R)(5.0 / 6.0) * qp1 - (R)(1.0 / 6.0) * qp2;
+    const R p1 = -(R)(1.0 / 6.0) * qm1 + (R)(5.0 / 6.0) * q0 + (R)(1.0 / 3.0) * qp1;
+    const R p2 = (R)(1.0 / 3.0) * qm2 - (R)(7.0 / 6.0) * qm1 + (R)(11.0 / 6.0) * q0;
+    return w0 * p0 + w1 * p1 + w2 * p2;
+}
+// mp5_interface, Suresh-Huynh MP5 with alpha = 4 (reconstruction.f90:222-254).
+template <typename R> GRHD_HD R mp5(R qm2, R qm1, R q0, R qp1, R qp2)
+{
+    const R alpha = (R)4, eps = (R)1.0e-10;
+    const R vL = ((R)2 * qm2 - (R)13 * qm1 + (R)47 * q0 + (R)27 * qp1 - (R)3 * qp2) / (R)60;
+    const R vmp = q0 + minmod2(qp1 - q0, alpha * (q0 - qm1));
+    if ((vL - q0) * (vL - vmp) < eps) return vL;
+    const R djm1 = qm2 - (R)2 * qm1 + q0;
+    const R dj = qm1 - (R)2 * q0 + qp1;
+    const R djp1 = q0 - (R)2 * qp1 + qp2;
+    const R dm4p = minmod4((R)4 * dj - djp1, (R)4 * djp1 - dj, dj, djp1);
+    const R dm4m = minmod4((R)4 * dj - djm1, (R)4 * djm1 - dj, dj, djm1);
+    const R vul = q0 + alpha * (q0 - qm1);
+    const R vav = (R)0.5 * (q0 + qp1);
+    const R vmd = vav - (R)0.5 * dm4p;
+    const R vlc = q0 + (R)0.5 * (q0 - qm1) + (R)4 * dm4m / (R)3;
+    const R vmin = rmax(rmin(rmin(q0, qp1), vmd), rmin(rmin(q0, vul), vlc));
+    const R vmax = rmin(rmax(rmax(q0, qp1), vmd), rmax(rmax(q0, vul), vlc));
+    return vL + minmod2(vmin - vL, vmax - vL);
+}
+// tvd_slope (reconstruction.f90:259-278).
+template <typename R> GRHD_HD R tvd_slope(R a, R b, int lim)
+{
+    if (lim == 2) {
+        if (a * b <= (R)0) return 0;
+        return sgn1(a) * rmax(rmin((R)2 * fabs(a), fabs(b)), rmin(fabs(a), (R)2 * fabs(b)));
+    }
+    if (lim == 3) return minmod3((R)2 * b, (R)0.5 * (a + b), (R)2 * a);
+    return minmod2(a, b);
+}
+
+// One face of reconstruct_1d_core (reconstruction.f90:285-657).
+// s[v][o + 2] holds the primitive v at offset o = -2..3 from the face's left
+// cell.  i is the face index (0..nodes) along direction dir (1..3).
+template <typename R>
+GRHD_HD void recon_face(const Params &P, const R s[NEQ][6], int i, int nodes, int dir,
+                        R qL[NEQ], R qR[NEQ])
+{
+#define S_(v, o) s[v][(o) + 2]
+    const int rec = P.rec;
+    const bool mink = (P.mk <= MK_MINK_SPH_LOG);
+    const R rf = (R)P.rho_floor, pf = (R)P.p_floor;
+    const bool radial0 = (!mink && dir == 1 && i <= 0);
+    if (rec == 1) {
+#pragma unroll
+        for (int v = 0; v < NEQ; v++) {
+            qL[v] = S_(v, 0);
+            qR[v] = S_(v, 1);
+        }
+    } else if (P.sensor) {
+        R pmax = S_(EQ_PR, -2), pmin = S_(EQ_PR, -2), rmx = S_(EQ_DE, -2), rmn = S_(EQ_DE, -2);
+        for (int o = -1; o <= 2; o++) {
+            pmax = rmax(pmax, S_(EQ_PR, o));
+            pmin = rmin(pmin, S_(EQ_PR, o));
+            rmx = rmax(rmx, S_(EQ_DE, o));
+            rmn = rmin(rmn, S_(EQ_DE, o));
+        }
+        const R pj = (pmax > (R)1.0e3 * pf) ? (pmax - pmin) / rmax(pmin, pf) : (R)0;
+        const R rj = (rmx > (R)1.0e3 * rf) ? (rmx - rmn) / rmax(rmn, rf) : (R)0;
+        const bool god = radial0 || S_(EQ_DE, 0) <= (R)1.0e2 * rf || S_(EQ_DE, 1) <= (R)1.0e2 * rf ||
+                         pj >= (R)10 || rj >= (R)10;
+        const bool tvdmc = (pj >= (R)0.80 || rj >= (R)0.50);
+        const bool w3 = (pj >= (R)0.40 || rj >= (R)0.15);
+        const bool mp5w3 = w3 || (!mink && dir == 1 && i <= 3) ||
+                           (!mink && dir == 2 && (i <= 2 || i >= nodes - 2));
+#pragma unroll
+        for (int v = 0; v < NEQ; v++) {
+            if (god) {
+                qL[v] = S_(v, 0);
+                qR[v] = S_(v, 1);
+            } else if (rec == 2 || tvdmc) {
+                const int lim = (rec == 2) ? P.lim : 3;
+                qL[v] = S_(v, 0) + (R)0.5 * tvd_slope(S_(v, 0) - S_(v, -1), S_(v, 1) - S_(v, 0), lim);
+                qR[v] = S_(v, 1) - (R)0.5 * tvd_slope(S_(v, 1) - S_(v, 0), S_(v, 2) - S_(v, 1), lim);
+            } else if (rec == 3 || (rec == 4 && mp5w3) || (rec == 5 && w3)) {
+                qL[v] = weno3(S_(v, -1), S_(v, 0), S_(v, 1));
+                qR[v] = weno3(S_(v, 2), S_(v, 1), S_(v, 0));
+            } else if (rec == 4) {
+                qL[v] = mp5(S_(v, -2), S_(v, -1), S_(v, 0), S_(v, 1), S_(v, 2));
+                qR[v] = mp5(S_(v, 3), S_(v, 2), S_(v, 1), S_(v, 0), S_(v, -1));
+            } else {
+                qL[v] = weno5(S_(v, -2), S_(v, -1), S_(v, 0), S_(v, 1), S_(v, 2));
+                qR[v] = weno5(S_(v, 3), S_(v, 2), S_(v, 1), S_(v, 0), S_(v, -1));
+            }
+        }
+    } else {
+#pragma unroll
+        for (int v = 0; v < NEQ; v++) {
+            if (rec == 2) {
+                qL[v] = S_(v, 0) + (R)0.5 * tvd_slope(S_(v, 0) - S_(v, -1), S_(v, 1) - S_(v, 0), P.lim);
+                qR[v] = S_(v, 1) - (R)0.5 * tvd_slope(S_(v, 1) - S_(v, 0), S_(v, 2) - S_(v, 1), P.lim);
+            } else if (radial0) {
+                qL[v] = S_(v, 0);
+                qR[v] = S_(v, 1);
+            } else if (rec == 3) {
+                qL[v] = weno3(S_(v, -1), S_(v, 0), S_(v, 1));
+                qR[v] = weno3(S_(v, 2), S_(v, 1), S_(v, 0));
+            } else if (rec == 4) {
+                qL[v] = mp5(S_(v, -2), S_(v, -1), S_(v, 0), S_(v, 1), S_(v, 2));
+                qR[v] = mp5(S_(v, 3), S_(v, 2), S_(v, 1), S_(v, 0), S_(v, -1));
+            } else {
+                qL[v] = weno5(S_(v, -2), S_(v, -1), S_(v, 0), S_(v, 1), S_(v, 2));
+                qR[v] = weno5(S_(v, 3), S_(v, 2), S_(v, 1), S_(v, 0), S_(v, -1));
+            }
+        }
+    }
+    // positivity floors (reconstruction.f90:647-655)
+    qL[EQ_DE] = rmax(qL[EQ_DE], rf);
+    qR[EQ_DE] = rmax(qR[EQ_DE], rf);
+    qL[EQ_PR] = rmax(qL[EQ_PR], pf);
+    qR[EQ_PR] = rmax(qR[EQ_PR], pf);
+#undef S_
+}
+
+// -------------------------------------------------------------- con2prim --
+// eval_V_and_dV (equations.f90:212-220) and eval_residual (223-246).
+GRHD_HD void eval_V(double g1, double W, double D, double tau, double &V, double &dV)
+{
+    const double den = g1 * (W * W) - 1.0;
+    V = (g1 * (W * W) * (tau + D) - D * W) / den;
+    dV = (g1 * W * (D * W - 2.0 * (tau + D)) + D) / (den * den);
+}
+GRHD_HD void eval_res(double g1, double W, double D, double tau, double Q, double &f, double &df)
+{
+    double V, dV;
+    eval_V(g1, W, D, tau, V, dV);
+    f = (V * V) * (1.0 - 1.0 / (W * W)) - Q;
+    df = 2.0 * V * (dV * (W * W * W - W) + V) / (W * W * W);
+}
+
+// root_brent (equations.f90:343-424).
+GRHD_HD double root_brent(double g1, double rho_floor, double D, double tau, double Q)
+{
+    const double x1 = 1.0, xacc = 1.0e-8, EPS = 3.0e-8;
+    const double wg = (sqrt(Q) + D) / rmax(D, rho_floor);
+    const double x2 = rmax(1.0e4, wg * 2.0);
+    double a, b, cc, d = 0.0, e = 0.0, fa, fb, fc, df;
+    eval_res(g1, x1, D, tau, Q, fa, df);
+    eval_res(g1, x2, D, tau, Q, fb, df);
+    if (sgn1(fa) * sgn1(fb) > 0.0) return -1.0;
+    a = x1;
+    b = x2;
+    cc = b;
+    fc = fb;
+    for (int it = 1; it <= 100; it++) {
+        if (sgn1(fb) * sgn1(fc) > 0.0) {
+            cc = a;
+            fc = fa;
+            d = b - a;
+            e = d;
+        }
+        if (fabs(fc) < fabs(fb)) {
+            a = b;
+            b = cc;
+            cc = a;
+            fa = fb;
+            fb = fc;
+            fc = fa;
+        }
+        const double tol1 = 2.0 * EPS * fabs(b) + 0.5 * xacc;
+        const double xm = 0.5 * (cc - b);
+        if (fabs(xm) <= tol1 || fabs(fb) < 1.0e-14) return b;
+        if (fabs(e) >= tol1 && fabs(fa) > fabs(fb)) {
+            double o, q, r;
+            const double y1 = fb / fa;
+            if (fabs(a - cc) < 1.0e-14) {
+                o = 2.0 * xm * y1;
+                q = 1.0 - y1;
+            } else {
+                q = fa / fc;
+                r = fb / fc;
+                o = y1 * (2.0 * xm * q * (q - r) - (b - a) * (r - 1.0));
+                q = (q - 1.0) * (r - 1.0) * (y1 - 1.0);
+            }
+            if (o > 0.0) q = -q;
+            o = fabs(o);
+            if (2.0 * o < rmin(3.0 * xm * q - fabs(tol1 * q), fabs(e * q))) {
+                e = d;
+                d = o / q;
+            } else {
+                d = xm;
+                e = d;
+            }
+        } else {
+            d = xm;
+            e = d;
+        }
+        a = b;
+        fa = fb;
+        if (fabs(d) > tol1) b = b + d;
+        else b = b + copysign(tol1, xm);
+        eval_res(g1, b, D, tau, Q, fb, df);
+    }
+    return -1.0;
+}
+
+// root_safe_newton (equations.f90:249-340).  *brent is set when the
+// Brent fallback runs.
+GRHD_HD double root_newton(double g1, double rho_floor, double D, double tau, double Q, int &brent)
+{
+    const double x1 = 1.0, xacc = 1.0e-10;
+    const double wg = (sqrt(Q) + D) / rmax(D, rho_floor);
+    const double x2 = rmax(1.0e3, wg * 2.0);
+    double xl, xh, f, df, flo, fhi, dx, dxold, root, tmp;
+    eval_res(g1, x1, D, tau, Q, flo, df);
+    eval_res(g1, x2, D, tau, Q, fhi, df);
+    if (fabs(flo) < 1.0e-14) return x1;
+    if (fabs(fhi) < 1.0e-14) return x2;
+    if (sgn1(flo) * sgn1(fhi) > 0.0) return -1.0;
+    if (flo < 0.0) {
+        xl = x1;
+        xh = x2;
+    } else {
+        xh = x1;
+        xl = x2;
+    }
+    root = 0.5 * (x1 + x2);
+    dxold = fabs(x2 - x1);
+    dx = dxold;
+    eval_res(g1, root, D, tau, Q, f, df);
+    for (int it = 1; it <= 100; it++) {
+        if (((root - xh) * df - f) * ((root - xl) * df - f) > 0.0 || fabs(2.0 * f) > fabs(dxold * df)) {
+            dxold = dx;
+            dx = 0.5 * (xh - xl);
+            root = xl + dx;
+            if (fabs(xl - root) < 1.0e-14) return root;
+        } else {
+            dxold = dx;
+            dx = f / df;
+            tmp = root;
+            root = root - dx;
+            if (fabs(tmp - root) < 1.0e-14) return root;
+        }
+        if (fabs(dx) < xacc) return root;
+        eval_res(g1, root, D, tau, Q, f, df);
+        if (f < 0.0) xl = root;
+        else xh = root;
+    }
+    brent = 1;
+    return root_brent(g1, rho_floor, D, tau, Q);
+}
+
+// cons_to_prim (equations.f90:94-205).  c is repaired in place; returns true
+// when the cell was reset to the atmosphere.
+GRHD_HD bool c2p(double g1, double rho_floor, double p_floor, const double g[3][3],
+                 const double gi[3][3], double sqg, double c[NEQ], double q[NEQ], int &brent)
+{
+    const double vmax = 1.0 - 1.0e-5;
+    const double Df = rho_floor * sqg, tf = p_floor * (g1 - 1.0) * sqg;
+    bool atm = (c[EQ_DE] <= Df);
+    if (!atm) {
+        const double S1 = c[EQ_VX], S2 = c[EQ_VY], S3 = c[EQ_VZ];
+        const double Ssq = S1 * (gi[0][0] * S1 + gi[0][1] * S2 + gi[0][2] * S3) +
+                           S2 * (gi[1][0] * S1 + gi[1][1] * S2 + gi[1][2] * S3) +
+                           S3 * (gi[2][0] * S1 + gi[2][1] * S2 + gi[2][2] * S3);
+        const double E = c[EQ_PR] + c[EQ_DE];
+        const double Smax = rmax(0.0, E * E - c[EQ_DE] * c[EQ_DE]) * vmax;
+        if (Ssq > Smax && Ssq > 1.0e-20) {
+            const double sf = sqrt(Smax / Ssq);
+            c[EQ_VX] = c[EQ_VX] * sf;
+            c[EQ_VY] = c[EQ_VY] * sf;
+            c[EQ_VZ] = c[EQ_VZ] * sf;
+        }
+        double Q = 0.0;
+        for (int a = 0; a < 3; a++)
+            for (int b = 0; b < 3; b++) Q = Q + gi[a][b] * c[EQ_VX + a] * c[EQ_VX + b];
+        const double W = root_newton(g1, rho_floor, c[EQ_DE], c[EQ_PR], Q, brent);
+        if (W < 1.0) {
+            atm = true;
+        } else {
+            double V, dV;
+            eval_V(g1, W, c[EQ_DE], c[EQ_PR], V, dV);
+            if (V <= 0.0 || isnan(V)) {
+                atm = true;
+            } else {
+                q[EQ_DE] = c[EQ_DE] / (W * sqg);
+                q[EQ_PR] = (V - c[EQ_PR] - c[EQ_DE]) / sqg;
+                q[EQ_VX] = (gi[0][0] * c[EQ_VX] + gi[0][1] * c[EQ_VY] + gi[0][2] * c[EQ_VZ]) / V;
+                q[EQ_VY] = (gi[1][0] * c[EQ_VX] + gi[1][1] * c[EQ_VY] + gi[1][2] * c[EQ_VZ]) / V;
+                q[EQ_VZ] = (gi[2][0] * c[EQ_VX] + gi[2][1] * c[EQ_VY] + gi[2][2] * c[EQ_VZ]) / V;
+                double vf = 0.0;
+                for (int a = 0; a < 3; a++)
+                    for (int b = 0; b < 3; b++) vf = vf + g[a][b] * q[EQ_VX + a] * q[EQ_VX + b];
+                atm = (q[EQ_DE] < rho_floor || q[EQ_PR] < p_floor || isnan(q[EQ_DE]) ||
+                       isnan(q[EQ_PR]) || isnan(q[EQ_VX]) || isnan(q[EQ_VY]) || isnan(q[EQ_VZ]) ||
+                       isnan(vf) || vf >= vmax);
+            }
+        }
+    }
+    if (atm) {
+        q[EQ_DE] = rho_floor;
+        q[EQ_PR] = p_floor;
+        q[EQ_VX] = q[EQ_VY] = q[EQ_VZ] = 0.0;
+        c[EQ_DE] = Df;
+        c[EQ_PR] = tf;
+        c[EQ_VX] = c[EQ_VY] = c[EQ_VZ] = 0.0;
+    }
+    return atm;
+}
+
+// ---------------------------------------------------------------- sources --
+// calc_sources (equations.f90:509-552) with T^{mu nu} from
+// calculate_stress_energy_tensor (metrics.f90:1251-1310).  The d_phi
+// derivatives vanish for every metric here, so S_3 = 0.
+GRHD_HD void sources(double g1, const CentreMetric &m, const double q[NEQ], double src[NEQ])
+{
+    double v[3] = {q[EQ_VX], q[EQ_VY], q[EQ_VZ]}, u4[4], T[4][4];
+    const double vsq = vsq_clamp<double>(m.g, v);
+    const double h = 1.0 + g1 * q[EQ_PR] / q[EQ_DE];
+    const double W = 1.0 / sqrt(1.0 - vsq);
+    u4[0] = W / m.alpha;
+    for (int i = 0; i < 3; i++) u4[i + 1] = W * (v[i] - m.beta[i] / m.alpha);
+    for (int mu = 0; mu < 4; mu++)
+        for (int nu = mu; nu < 4; nu++) {
+            T[mu][nu] = (q[EQ_DE] * h * u4[mu] * u4[nu]) + (q[EQ_PR] * m.gm[mu][nu]);
+            T[nu][mu] = T[mu][nu];
+        }
+    for (int e = 0; e < NEQ; e++) src[e] = 0.0;
+    for (int d = 0; d < 2; d++) {
+        double s = 0.0;
+        for (int mu = 0; mu < 4; mu++)
+            for (int nu = 0; nu < 4; nu++) s = s + 0.5 * T[mu][nu] * m.dg[mu][nu][d];
+        src[EQ_VX + d] = s;
+    }
+    double st = 0.0;
+    for (int mu = 0; mu < 4; mu++) {
+        st = st + T[mu][0] * m.dlna[mu];
+        for (int nu = 0; nu < 4; nu++) st = st - T[mu][nu] * m.ch0[mu][nu];
+    }
+    src[EQ_PR] = st;
+    for (int e = 0; e < NEQ; e++) src[e] = src[e] * m.alpha * m.sqg;
+    src[EQ_PR] = src[EQ_PR] * m.alpha;
+}
+
+}  // namespace grhd

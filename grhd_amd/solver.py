@@ -1,0 +1,156 @@
+"""Host-side mirror of the reference update path on top of the C ABI.
+
+:class:`Solver` owns one device context.  Its methods are named after the
+reference routines they replace; arrays use the reference layout (see
+include/grhd_c.h): primitives ``(nz+2ng+1, ny+2ng+1, nx+2ng+1, 5)``, conserved
+``(nz, ny, nx, 5)``.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from .params import RunConfig
+
+_dp = ctypes.POINTER(ctypes.c_double)
+
+
+def _ptr(a: np.ndarray):
+    if a.dtype != np.float64 or not a.flags.c_contiguous:
+        raise ValueError("arrays must be C-contiguous float64")
+    return a.ctypes.data_as(_dp)
+
+
+def fishbone_unit_density_k(cfg: RunConfig) -> float:
+    """Polytropic K of the torus with rho = 1 at the pressure maximum."""
+    c = _lib.Config.from_run(cfg)
+    k = ctypes.c_double()
+    _lib.check(_lib.lib().grhd_fishbone_unit_density_k(ctypes.byref(c), ctypes.byref(k)))
+    return k.value
+
+
+class Solver:
+    def __init__(self, cfg: RunConfig):
+        self.cfg = cfg
+        self._L = _lib.lib()
+        self._c = _lib.Config.from_run(cfg)
+        h = ctypes.c_void_p()
+        _lib.check(self._L.grhd_create(ctypes.byref(self._c), ctypes.byref(h)))
+        self.h = h
+        dims = (ctypes.c_int32 * 4)()
+        self._chk(self._L.grhd_dims(self.h, dims))
+        self.nx, self.ny, self.nz, self.ng = list(dims)
+        self.t = 0.0
+
+    # ------------------------------------------------------------------ basics
+    def _chk(self, rc):
+        _lib.check(rc, self.h)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self._L.grhd_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def pshape(self):
+        ng = self.ng
+        return (self.nz + 2 * ng + 1, self.ny + 2 * ng + 1, self.nx + 2 * ng + 1, 5)
+
+    @property
+    def ushape(self):
+        return (self.nz, self.ny, self.nx, 5)
+
+    @property
+    def n_cells(self) -> int:
+        return self.nx * self.ny * self.nz
+
+    # -------------------------------------------------------------- state I/O
+    def initial_conditions(self):
+        """set_initial_conditions + prim_to_cons (grhd2.f90:121-134)."""
+        self._chk(self._L.grhd_initial_conditions(self.h))
+
+    def perturb(self, kind: int, amplitude: float, seed: int):
+        """Seeded multiplicative noise: kind 1 pressure, 2 isentropic density."""
+        self._chk(self._L.grhd_perturb(self.h, int(kind), float(amplitude), int(seed)))
+
+    def set_primitives(self, p: np.ndarray):
+        self._chk(self._L.grhd_set_primitives(self.h, _ptr(np.ascontiguousarray(p, np.float64))))
+
+    def put_primitives(self, p: np.ndarray):
+        self._chk(self._L.grhd_put_primitives(self.h, _ptr(np.ascontiguousarray(p, np.float64))))
+
+    def set_conserved(self, u: np.ndarray):
+        self._chk(self._L.grhd_set_conserved(self.h, _ptr(np.ascontiguousarray(u, np.float64))))
+
+    def put_conserved(self, u: np.ndarray):
+        self._chk(self._L.grhd_put_conserved(self.h, _ptr(np.ascontiguousarray(u, np.float64))))
+
+    def get_primitives(self) -> np.ndarray:
+        p = np.zeros(self.pshape)
+        self._chk(self._L.grhd_get_primitives(self.h, _ptr(p)))
+        return p
+
+    def get_conserved(self) -> np.ndarray:
+        u = np.zeros(self.ushape)
+        self._chk(self._L.grhd_get_conserved(self.h, _ptr(u)))
+        return u
+
+    # --------------------------------------------------------------- update
+    def compute_dt(self) -> float:
+        """update_adaptive_dt (evolution.f90:198-275)."""
+        d = ctypes.c_double()
+        self._chk(self._L.grhd_compute_dt(self.h, ctypes.byref(d)))
+        return d.value
+
+    def step(self, dt: float, nsteps: int = 1):
+        self._chk(self._L.grhd_step(self.h, float(dt), int(nsteps)))
+
+    def advance(self, t_final: float | None = None, max_steps: int = 2 ** 31 - 1):
+        """The reference time loop (grhd2.f90:179-265) from self.t."""
+        t_final = self.cfg.final_time if t_final is None else t_final
+        t = ctypes.c_double(self.t)
+        n = ctypes.c_int32()
+        last = ctypes.c_double()
+        self._chk(self._L.grhd_advance(self.h, float(t_final), int(max_steps), ctypes.byref(t),
+                                       ctypes.byref(n), ctypes.byref(last)))
+        self.t = t.value
+        return n.value, last.value
+
+    def boundaries(self):
+        self._chk(self._L.grhd_boundaries(self.h))
+
+    def rhs(self) -> np.ndarray:
+        r = np.zeros(self.ushape)
+        self._chk(self._L.grhd_rhs(self.h, _ptr(r)))
+        return r
+
+    def con2prim(self):
+        self._chk(self._L.grhd_con2prim(self.h))
+
+    def counters(self) -> dict:
+        c = (ctypes.c_int64 * 4)()
+        self._chk(self._L.grhd_counters(self.h, c))
+        return {"atmosphere_resets": c[0], "brent_fallbacks": c[1], "steps": c[2]}
+
+    def synchronize(self):
+        self._chk(self._L.grhd_synchronize(self.h))
+
+    def last_timing(self):
+        ms = ctypes.c_double()
+        n = ctypes.c_int64()
+        self._chk(self._L.grhd_last_timing(self.h, ctypes.byref(ms), ctypes.byref(n)))
+        return ms.value, n.value

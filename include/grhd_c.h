@@ -1,0 +1,192 @@
+/*
+ * grhd_c.h -- C ABI of the MI355X GRHD2 finite-volume update (libgrhd_hip.so).
+ *
+ * The reference has no FFI: its update path is a set of Fortran module
+ * procedures sharing module state.  Each entry point below replaces one of
+ * those procedures (or one block of the reference main program) and is what a
+ * Fortran driver binds through iso_c_binding (grhd_amd/fortran/grhd_c_binding.f90)
+ * and what the Python tests bind through ctypes.
+ *
+ * Conventions
+ *  - Every function returns an int status: GRHD_OK (0) or a GRHD_ERR_* code.
+ *    No exception crosses the ABI.  grhd_last_error() describes the last
+ *    failure of a context (or of grhd_create when ctx is NULL).
+ *  - A context owns all device memory and one HIP stream.  Host buffers are
+ *    borrowed for the duration of a call only.
+ *  - Host arrays use the reference's own Fortran layout:
+ *      primitives p(neq, -ng:nx+ng, -ng:ny+ng, -ng:nz+ng)   (neq = 5 fastest)
+ *      conserved  u(neq, nx, ny, nz)
+ *    i.e. C arrays [nz+2ng+1][ny+2ng+1][nx+2ng+1][5] and [nz][ny][nx][5].
+ *    Variables: 0 rho / D, 1 p / tau, 2..4 v^i / S_i (variables.f90:24-30).
+ *  - A context is used by one host thread at a time; distinct contexts are
+ *    independent.
+ */
+#ifndef GRHD_C_H
+#define GRHD_C_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define GRHD_OK 0
+#define GRHD_ERR_INVALID_CONFIG 1   /* what parameters.f90:359-588 rejects */
+#define GRHD_ERR_INVALID_METRIC 2   /* evolution.f90:211-272 error stops */
+#define GRHD_ERR_OUT_OF_MEMORY 3
+#define GRHD_ERR_HIP 4
+#define GRHD_ERR_INVALID_ARGUMENT 5
+#define GRHD_ERR_UNSUPPORTED 6
+
+/* identifiers: same values as the reference (variables.f90:224-237) */
+#define GRHD_REC_GODUNOV 1
+#define GRHD_REC_TVD 2
+#define GRHD_REC_WENO3 3
+#define GRHD_REC_MP5 4
+#define GRHD_REC_WENO5 5
+#define GRHD_RS_HLLE 1
+#define GRHD_RS_HLLC 2
+#define GRHD_LIM_MINMOD 1
+#define GRHD_LIM_SUPERBEE 2
+#define GRHD_LIM_MC 3
+#define GRHD_METRIC_MINKOWSKI 0
+#define GRHD_METRIC_EDDINGTON_FINKELSTEIN 1
+#define GRHD_METRIC_KERR_SCHILD 2
+#define GRHD_GEOM_CARTESIAN 0
+#define GRHD_GEOM_CYLINDRICAL 1
+#define GRHD_GEOM_SPHERICAL 2
+#define GRHD_GEOM_SPHEROIDAL 3
+/* arithmetic of the face computation (reconstruction + Riemann solver);
+ * con2prim, the RK3 accumulation, sources and metric determinants are FP64
+ * in both modes */
+#define GRHD_PRECISION_FP64 0
+#define GRHD_PRECISION_MIXED 1
+
+/* Problem description: the fields of the .par surface the update path reads
+ * (parameters.f90:146-353) after the problem preset has been applied.
+ * problem: 1 sod, 2 strong_shock, 3 shu_osher, 4 kelvin_helmholtz,
+ * 5 axisymmetric_jet, 6 convergence, 7 michel, 8 dust, 9 offaxis_blast,
+ * 10 fishbone_equatorial, 11 fishbone_sagittal, 12 fishbone_3d
+ * (parameters.f90:893-921).  Angles y_min..z_max are in radians. */
+typedef struct grhd_config {
+    int32_t problem;
+    int32_t metric;
+    int32_t geometry;
+    int32_t log_r;
+    double adiabatic_index;
+    double bh_mass;
+    double spin;
+    int32_t nx, ny, nz;
+    int32_t reconstruction;
+    double r_min, r_max, y_min, y_max, z_min, z_max;
+    int32_t riemann_solver;
+    int32_t tvd_limiter;
+    int32_t shock_sensor;
+    int32_t precision;
+    double cfl;
+    double final_time;
+    /* initial-condition parameters (variables.f90:166-219) */
+    double sod_interface, sod_density_left, sod_density_right;
+    double sod_pressure_left, sod_pressure_right;
+    double strong_interface, strong_density_left, strong_density_right;
+    double strong_pressure_left, strong_pressure_right;
+    double shu_interface, shu_density_left, shu_pressure_left, shu_density_right;
+    double shu_density_amplitude, shu_wave_number, shu_pressure_right;
+    double khi_half_width, khi_density_inner, khi_density_outer;
+    double khi_velocity_inner, khi_velocity_outer, khi_pressure;
+    double khi_perturbation_amplitude, khi_perturbation_wave_number, khi_perturbation_width;
+    double jet_ambient_density, jet_ambient_pressure, jet_ambient_velocity;
+    double jet_nozzle_radius, jet_nozzle_length, jet_density, jet_pressure, jet_velocity;
+    double convergence_density, convergence_amplitude, convergence_velocity;
+    double convergence_pressure, convergence_wave_number;
+    double michel_critical_radius, michel_critical_density;
+    double dust_accretion_constant;
+    double offaxis_radial_center, offaxis_phi_center, offaxis_width;
+    double offaxis_background_density, offaxis_background_pressure;
+    double offaxis_density_amplitude, offaxis_pressure_amplitude;
+    double fm_inner_radius, fm_pressure_max_radius, fm_polytropic_constant;
+    /* 1 accepts fishbone_3d with theta away from the poles (a build option,
+     * the reference requires theta = [0, pi]); the trans-polar fill is kept. */
+    int32_t allow_open_theta;
+    int32_t reserved[7];
+} grhd_config;
+
+typedef struct grhd_ctx grhd_ctx;
+
+/* Allocates the device state for one problem.  Replaces allocate_and_grid +
+ * precalculate_metric_cache + choose_numerical_architecture + the floors
+ * (initialization.f90:23-391, 452-457): no metric is stored, it is evaluated
+ * in the kernels.  Validation follows parameters.f90:359-588. */
+int grhd_create(const grhd_config *cfg, grhd_ctx **out);
+int grhd_destroy(grhd_ctx *ctx);
+/* Message for the last failure of ctx; ctx == NULL gives the last grhd_create failure. */
+const char *grhd_last_error(const grhd_ctx *ctx);
+/* dims[0..3] = nx, ny, nz, nghost */
+int grhd_dims(const grhd_ctx *ctx, int32_t *dims);
+
+/* set_initial_conditions (conditions.f90:26-214) followed by prim_to_cons over
+ * the interior (grhd2.f90:126-134) and the Michel injector cache
+ * (conditions.f90:229-246). */
+int grhd_initial_conditions(grhd_ctx *ctx);
+/* Multiplicative white noise on the cells denser than 5 % of the maximum
+ * density, followed by U = P2C(p): type 1 pressure noise
+ * (inject_pressure_noise, conditions.f90:603-636), type 2 isentropic density
+ * noise (inject_density_noise, conditions.f90:730-768).  The deviates come
+ * from a counter-based generator keyed by (seed, cell index); the reference's
+ * compiler-runtime random_number stream is not portable. */
+int grhd_perturb(grhd_ctx *ctx, int32_t type, double amplitude, uint64_t seed);
+/* Polytropic constant K of the Fishbone-Moncrief torus that puts rho = 1 at
+ * the pressure maximum: K = (h_c - 1)(Gamma - 1)/Gamma with
+ * h_c = exp(W_in - W(r_c, pi/2)) (conditions.f90:470-546). */
+int grhd_fishbone_unit_density_k(const grhd_config *cfg, double *K);
+/* Uploads a full ghosted primitive array and rebuilds U = P2C(p) on the
+ * device (grhd2.f90:206-218 after a perturbation). */
+int grhd_set_primitives(grhd_ctx *ctx, const double *p);
+/* Uploads a full ghosted primitive array without touching U. */
+int grhd_put_primitives(grhd_ctx *ctx, const double *p);
+/* Uploads interior conserved variables, then cons_to_prim and boundaries
+ * rebuild p (restart path, grhd2.f90:103-116). */
+int grhd_set_conserved(grhd_ctx *ctx, const double *u);
+/* Uploads interior conserved variables only. */
+int grhd_put_conserved(grhd_ctx *ctx, const double *u);
+/* Downloads the ghosted primitives.  Ghost cells that no kernel reads
+ * (corners and the ghost planes of axes with n == 1) are copies of the
+ * neighbouring plane. */
+int grhd_get_primitives(grhd_ctx *ctx, double *p);
+int grhd_get_conserved(grhd_ctx *ctx, double *u);
+
+/* update_adaptive_dt (evolution.f90:198-275): dt = CFL / sum_d max(alpha
+ * sqrt(gamma^dd) + |beta^d|)/dx^d.  Returns GRHD_ERR_INVALID_METRIC where the
+ * reference error-stops. */
+int grhd_compute_dt(grhd_ctx *ctx, double *dt);
+/* nsteps full SSP-RK3 steps with a fixed dt (grhd2.f90:232-265: boundaries,
+ * three stages of calc_rhs, RK combination, cons_to_prim, boundaries). */
+int grhd_step(grhd_ctx *ctx, double dt, int32_t nsteps);
+/* The reference time loop (grhd2.f90:179-232) up to t_final or max_steps:
+ * dt from grhd_compute_dt, clamped at the end.  *t is read and updated. */
+int grhd_advance(grhd_ctx *ctx, double t_final, int32_t max_steps, double *t,
+                 int32_t *steps_done, double *last_dt);
+
+/* Unit entry points for parity checks. */
+/* set_boundary_conditions on the device primitives (conditions.f90:773-1066). */
+int grhd_boundaries(grhd_ctx *ctx);
+/* calc_rhs (evolution.f90:30-191) of the current primitives (boundaries are
+ * applied first); rhs has the interior conserved layout. */
+int grhd_rhs(grhd_ctx *ctx, double *rhs);
+/* cons_to_prim over all interior cells (grhd2.f90:252-260); repairs U in place. */
+int grhd_con2prim(grhd_ctx *ctx);
+/* Counters since creation: [0] cells reset to atmosphere by cons_to_prim,
+ * [1] cells whose root solve fell back to Brent, [2] steps taken. */
+int grhd_counters(grhd_ctx *ctx, int64_t *counters);
+
+/* Stream and synchronisation: all work of a context is queued on its stream. */
+int grhd_set_stream(grhd_ctx *ctx, void *hip_stream);
+int grhd_synchronize(grhd_ctx *ctx);
+/* Time of the last grhd_step / grhd_advance call measured with HIP events on
+ * the context stream, in milliseconds, and the number of fused-stage kernel
+ * launches it made. */
+int grhd_last_timing(grhd_ctx *ctx, double *ms, int64_t *launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,51 @@
+"""Comparison helpers shared by the parity tests."""
+import numpy as np
+
+
+def read_set_mask(nx, ny, nz, ng):
+    """Cells of the ghosted primitive array that the update reads: the interior
+    plus, on each active axis, the ghost layers of rows whose transverse indices
+    are interior (corners and inactive-axis ghosts are never read)."""
+    shape = (nz + 2 * ng + 1, ny + 2 * ng + 1, nx + 2 * ng + 1)
+    m = np.zeros(shape, bool)
+    I = slice(ng + 1, ng + 1 + nx)
+    J = slice(ng + 1, ng + 1 + ny)
+    K = slice(ng + 1, ng + 1 + nz)
+    m[K, J, :] = True  # x rows incl. ghosts (x always has ghost layers)
+    if ny > 1:
+        m[K, :, I] = True
+    if nz > 1:
+        m[:, J, I] = True
+    return m
+
+
+def rel_errors(a, b):
+    """Per-variable relative L1 and Linf of a against b (last axis = variable):
+    L1 = sum|a-b| / sum|b|,  Linf = max|a-b| / max|b|."""
+    a = np.asarray(a, float).reshape(-1, a.shape[-1])
+    b = np.asarray(b, float).reshape(-1, b.shape[-1])
+    d = np.abs(a - b)
+    s1 = np.abs(b).sum(axis=0)
+    sm = np.abs(b).max(axis=0)
+    l1 = np.where(s1 > 0, d.sum(axis=0) / np.where(s1 > 0, s1, 1), d.sum(axis=0))
+    li = np.where(sm > 0, d.max(axis=0) / np.where(sm > 0, sm, 1), d.max(axis=0))
+    return l1, li
+
+
+def seeded_noise(p, nx, ny, nz, ng, kind, amp, seed):
+    """Multiplicative noise on the interior (kind 1 pressure, 2 density with
+    p scaled by f^Gamma is left to the caller); numpy generator, for feeding the
+    same perturbed state to two checkers."""
+    rng = np.random.default_rng(seed)
+    q = p.copy()
+    sl = (slice(ng + 1, ng + 1 + nz), slice(ng + 1, ng + 1 + ny), slice(ng + 1, ng + 1 + nx))
+    rho = q[sl + (0,)]
+    mask = rho > 0.05 * rho.max()
+    f = 1.0 + amp * (2.0 * rng.random(rho.shape) - 1.0)
+    f = np.where(mask, f, 1.0)
+    if kind == 1:
+        q[sl + (1,)] *= f
+    else:
+        q[sl + (0,)] *= f
+        q[sl + (1,)] *= f ** 1.3333333333333333
+    return q

@@ -1,0 +1,148 @@
+"""HIP path against the oracle (and, where it travels with the tree, the
+reference build) on the same inputs.
+
+Tolerances (FP64 path): per-variable relative L1 and Linf <= 1e-12 after one
+RK3 step; initial data, boundaries and dt agree to 1e-14 relative.  The
+device code keeps the reference's rounding sequence (no FMA contraction); the
+residual differences come from the device's transcendental functions (exp,
+sin, cos, pow) in the analytic metric and initial data.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from grhd_amd.params import config_from_par
+from grhd_amd.solver import Solver
+from tests.cases import CASES, NOISY
+from tests.helpers import read_set_mask, rel_errors, seeded_noise
+
+pytestmark = pytest.mark.gpu
+
+TOL_STEP = 1e-12
+TOL_POINT = 1e-14
+
+
+def _setup(name):
+    cfg = config_from_par(CASES[name])
+    S = Solver(cfg)
+    O = oracle.Oracle(cfg)
+    S.initial_conditions()
+    p = S.get_primitives()
+    if name in NOISY:
+        kind, amp, seed = NOISY[name]
+        p = seeded_noise(p, S.nx, S.ny, S.nz, S.ng, kind, amp, seed)
+        S.set_primitives(p)
+    if cfg.problem == 7:
+        O.cache_injector(p)
+    return cfg, S, O, p
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_initial_data_matches_reference(name):
+    if not oracle.reference_available():
+        pytest.skip("reference build not present")
+    cfg = config_from_par(CASES[name])
+    with Solver(cfg) as S:
+        S.initial_conditions()
+        pg = S.get_primitives()
+    R = oracle.Reference(cfg)
+    try:
+        pr = R.get_prim()
+    finally:
+        R.close()
+    m = read_set_mask(S.nx, S.ny, S.nz, S.ng)
+    # the read set includes the outermost ghost layer (-ng), which keeps its
+    # initial value for the whole run and is read by some sensor stencils
+    l1, li = rel_errors(pg[m], pr[m])
+    assert li.max() <= 1e-13, f"initial data Linf {li}"
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_one_rk3_step_matches_oracle(name):
+    cfg, S, O, p = _setup(name)
+    try:
+        u_o = O.prim_to_cons(p)
+        l1, li = rel_errors(S.get_conserved(), u_o)
+        assert li.max() <= TOL_POINT, f"prim_to_cons {li}"
+        dt_g, dt_o = S.compute_dt(), O.dt()
+        assert abs(dt_g - dt_o) <= TOL_POINT * dt_o
+        # boundaries on the read set
+        p_o = p.copy()
+        O.boundaries(p_o)
+        S.boundaries()
+        m = read_set_mask(S.nx, S.ny, S.nz, S.ng)
+        l1, li = rel_errors(S.get_primitives()[m], p_o[m])
+        assert li.max() <= TOL_POINT, f"boundaries {li}"
+        # calc_rhs
+        l1, li = rel_errors(S.rhs(), O.rhs(p_o))
+        assert l1.max() <= TOL_STEP and li.max() <= TOL_STEP, f"rhs L1 {l1} Linf {li}"
+        # one full SSP-RK3 step with the oracle's dt
+        S.step(dt_o, 1)
+        O.step(dt_o, u_o, p_o)
+        l1, li = rel_errors(S.get_conserved(), u_o)
+        assert l1.max() <= TOL_STEP and li.max() <= TOL_STEP, f"step U L1 {l1} Linf {li}"
+        l1, li = rel_errors(S.get_primitives()[m], p_o[m])
+        assert l1.max() <= TOL_STEP and li.max() <= TOL_STEP, f"step P L1 {l1} Linf {li}"
+    finally:
+        S.close()
+        O.close()
+
+
+@pytest.mark.parametrize("name", ["sod_c1", "fm3d_ks_mp5_hllc_sensor", "michel_ef_log",
+                                  "khi_weno3_sensor"])
+def test_ten_steps_stay_close(name):
+    cfg, S, O, p = _setup(name)
+    try:
+        u_o = O.prim_to_cons(p)
+        p_o = p.copy()
+        t_final = cfg.final_time if cfg.final_time > 0 else 1.0
+        n_g, _ = S.advance(t_final, 10)
+        t_o, _, n_o = O.run(u_o, p_o, 10, t=0.0, t_final=t_final)
+        assert n_g == n_o and abs(S.t - t_o) <= 1e-14 * max(1.0, t_o)
+        l1, li = rel_errors(S.get_conserved(), u_o)
+        assert l1.max() <= 1e-11 and li.max() <= 1e-9, f"L1 {l1} Linf {li}"
+    finally:
+        S.close()
+        O.close()
+
+
+def test_sod_c1_full_run_matches_reference():
+    """C1 to t_final = 0.4 (400 steps incl. the reference's final clamp)."""
+    cfg = config_from_par(CASES["sod_c1"])
+    with Solver(cfg) as S:
+        S.initial_conditions()
+        n, last = S.advance(cfg.final_time)
+        ug = S.get_conserved()
+    O = oracle.Oracle(cfg)
+    p = np.zeros(O.pshape)
+    with Solver(cfg) as S2:
+        S2.initial_conditions()
+        p = S2.get_primitives()
+    u = O.prim_to_cons(p)
+    t, dt, n_o = O.run(u, p, 10 ** 6)
+    assert n == n_o == 400
+    l1, li = rel_errors(ug, u)
+    assert l1.max() <= 1e-10, f"L1 {l1} Linf {li}"
+
+
+def test_mixed_precision_error_is_stated():
+    """Mixed path (FP32 reconstruction + Riemann): L1 density error vs the FP64
+    oracle after 10 steps of the 3D torus case stays below 1e-5."""
+    name = "fm3d_ks_mp5_hllc_sensor"
+    cfg = config_from_par(CASES[name]).replace(precision=1)
+    S = Solver(cfg)
+    O = oracle.Oracle(cfg)
+    S.initial_conditions()
+    p = S.get_primitives()
+    kind, amp, seed = NOISY[name]
+    p = seeded_noise(p, S.nx, S.ny, S.nz, S.ng, kind, amp, seed)
+    S.set_primitives(p)
+    u_o = O.prim_to_cons(p)
+    p_o = p.copy()
+    S.advance(1.0, 10)
+    O.run(u_o, p_o, 10, t=0.0, t_final=1.0)
+    m = read_set_mask(S.nx, S.ny, S.nz, S.ng)
+    l1, li = rel_errors(S.get_primitives()[m], p_o[m])
+    S.close()
+    O.close()
+    assert l1[0] <= 1e-5, f"mixed density L1 {l1[0]}"
