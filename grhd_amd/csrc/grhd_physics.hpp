@@ -39,6 +39,7 @@ struct Params {
     int mk;
     int nx, ny, nz, ng;
     int rec, rs, lim, sensor;
+    int kc;     // phi run length per thread in the column kernels
     int polar;  // spherical / spheroidal geometry
     int cyl;    // cylindrical geometry
     double gam, g1, mass, spin;

@@ -1,11 +1,18 @@
 """HIP path against the oracle (and, where it travels with the tree, the
 reference build) on the same inputs.
 
-Tolerances (FP64 path): per-variable relative L1 and Linf <= 1e-12 after one
-RK3 step; initial data, boundaries and dt agree to 1e-14 relative.  The
-device code keeps the reference's rounding sequence (no FMA contraction); the
-residual differences come from the device's transcendental functions (exp,
-sin, cos, pow) in the analytic metric and initial data.
+Tolerances (FP64 path):
+* conserved variables after one RK3 step: per-variable relative L1 and Linf
+  <= 1e-12;
+* initial data, boundaries and dt: 1e-14 relative;
+* primitives after a step, and anything after several steps: the con2prim
+  root solve stops at |dW| < 1e-10 (root_safe_newton, equations.f90:254,323),
+  so an input difference of one rounding can end the iteration at a different
+  point inside that tolerance; primitives are compared at L1 <= 1e-11,
+  Linf <= 1e-9, and ten-step states at L1 <= 1e-10, Linf <= 1e-8.
+The device code keeps the reference's rounding sequence (no FMA contraction);
+the residual input differences come from the device's transcendental
+functions (exp, sin, cos, pow) in the analytic metric and initial data.
 """
 import numpy as np
 import pytest
@@ -20,6 +27,8 @@ pytestmark = pytest.mark.gpu
 
 TOL_STEP = 1e-12
 TOL_POINT = 1e-14
+TOL_C2P_L1 = 1e-11
+TOL_C2P_LINF = 1e-9
 
 
 def _setup(name):
@@ -82,7 +91,7 @@ def test_one_rk3_step_matches_oracle(name):
         l1, li = rel_errors(S.get_conserved(), u_o)
         assert l1.max() <= TOL_STEP and li.max() <= TOL_STEP, f"step U L1 {l1} Linf {li}"
         l1, li = rel_errors(S.get_primitives()[m], p_o[m])
-        assert l1.max() <= TOL_STEP and li.max() <= TOL_STEP, f"step P L1 {l1} Linf {li}"
+        assert l1.max() <= TOL_C2P_L1 and li.max() <= TOL_C2P_LINF, f"step P L1 {l1} Linf {li}"
     finally:
         S.close()
         O.close()
@@ -100,7 +109,7 @@ def test_ten_steps_stay_close(name):
         t_o, _, n_o = O.run(u_o, p_o, 10, t=0.0, t_final=t_final)
         assert n_g == n_o and abs(S.t - t_o) <= 1e-14 * max(1.0, t_o)
         l1, li = rel_errors(S.get_conserved(), u_o)
-        assert l1.max() <= 1e-11 and li.max() <= 1e-9, f"L1 {l1} Linf {li}"
+        assert l1.max() <= 1e-10 and li.max() <= 1e-8, f"L1 {l1} Linf {li}"
     finally:
         S.close()
         O.close()
