@@ -7,6 +7,14 @@ namespace grhd {
 template <int MK>
 void launch_boundaries_mk(const Params &P, hipStream_t s, double *p, const double *injector);
 
+// The stage kernels (face fluxes + cell update) with FP64 or FP32 faces.
+void launch_stage_f64(const Params &P, hipStream_t s, double *p, void *const F[3],
+                      const int64_t fstr[3], const double *u0, double *u, int stage, double dt,
+                      unsigned long long *counters);
+void launch_stage_f32(const Params &P, hipStream_t s, double *p, void *const F[3],
+                      const int64_t fstr[3], const double *u0, double *u, int stage, double dt,
+                      unsigned long long *counters);
+
 // One RK stage: face fluxes, cell update (stage 1..3; 0 = rhs only into u),
 // then the ghost fill when with_boundaries.
 void launch_stage(const Params &P, hipStream_t s, double *p, void *const F[3], const int64_t fstr[3],

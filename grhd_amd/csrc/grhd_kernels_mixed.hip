@@ -1,0 +1,16 @@
+// grhd_kernels_mixed.hip -- the RK stage with FP32 reconstruction and Riemann
+// solver (face fluxes stored as FP32); the cell update, sources, con2prim and
+// the RK accumulation stay FP64.  Compiled with FMA contraction: this path is
+// judged by its error against the FP64 path, not by bitwise agreement.
+#include "grhd_stage.hpp"
+
+namespace grhd {
+
+void launch_stage_f32(const Params &P, hipStream_t s, double *p, void *const F[3],
+                      const int64_t fstr[3], const double *u0, double *u, int stage, double dt,
+                      unsigned long long *counters)
+{
+    GRHD_STAGE_DISPATCH(float)
+}
+
+}  // namespace grhd

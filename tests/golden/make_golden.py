@@ -1,0 +1,128 @@
+"""Write the golden vectors under tests/golden/ from the reference build.
+
+Runs the unmodified reference (oracle/_ref/libgrhd_ref.so, built from the
+reference sources by oracle/Makefile) on the parity cases and stores inputs
+and outputs as compressed .npz fixtures:
+
+  <case>.npz : p0 (ghosted primitives fed in), u1 (conserved after one RK3
+               step), p1, u10 (after ten steps), dt, t10
+  sod_c1_final.npz : C1 to t = 0.4 (u, p row, number of steps)
+  michel_c2_256.npz: C2 at nr = 256 to t = 500 with seeded 1e-3 density
+               noise: p0, final u, the reference's L1/L2/Linf density error
+               against the analytic solution (output.f90:1189-1213)
+
+Usage: python tests/golden/make_golden.py [case ...]   (OMP_NUM_THREADS=1)
+"""
+import os
+import sys
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parents[2]
+sys.path.insert(0, str(ROOT))
+
+import oracle  # noqa: E402
+from grhd_amd.params import config_from_par  # noqa: E402
+from tests.cases import CASES, NOISY  # noqa: E402
+from tests.helpers import seeded_noise  # noqa: E402
+
+OUT = Path(__file__).resolve().parent
+STEP_CASES = ["sod_c1", "fm3d_ks_mp5_hllc_sensor", "michel_ef_log", "khi_weno3_sensor",
+              "jet_cylindrical", "fm_sagittal_ef_log", "shu_tvd_minmod_sensor"]
+
+MICHEL_C2 = """problem = michel
+logarithmic_r = true
+nx = {nr}
+nz = 1
+r_min = 1.8
+r_max = 20
+michel_critical_radius = 8
+reconstruction = weno5
+riemann_solver = hlle
+final_time = 500
+"""
+
+
+def michel_norms(p, cfg, ng):
+    """calc_convergence_norms for MichelA (output.f90:1189-1213, 1223-1224)."""
+    R = oracle.Reference(cfg)
+    try:
+        exact = R.get_prim()
+    finally:
+        R.close()
+    nx = cfg.nx
+    i0 = max(1, int(0.10 * nx))
+    i1 = min(nx, max(i0, int(0.95 * nx)))
+    j = max(1, cfg.ny // 2)
+    rows = slice(ng + i0, ng + i1 + 1)
+    d = np.abs(p[ng + 1, ng + j, rows, 0] - exact[ng + 1, ng + j, rows, 0])
+    n = i1 - i0 + 1
+    return np.array([d.sum() / n, np.sqrt((d ** 2).sum() / n), d.max()])
+
+
+def step_case(name):
+    cfg = config_from_par(CASES[name])
+    R = oracle.Reference(cfg)
+    try:
+        p0 = R.get_prim()
+        if name in NOISY:
+            kind, amp, seed = NOISY[name]
+            p0 = seeded_noise(p0, R.nx, R.ny, R.nz, R.ng, kind, amp, seed)
+            R.set_prim(p0)
+        t_final = cfg.final_time if cfg.final_time > 0 else 1.0
+        R.set_time(0.0, t_final)
+        dt = R.dt()
+        R.step(1)
+        u1, p1 = R.get_cons(), R.get_prim()
+        t10, _, _ = R.step(9)
+        u10 = R.get_cons()
+    finally:
+        R.close()
+    np.savez_compressed(OUT / f"{name}.npz", p0=p0, u1=u1, p1=p1, u10=u10, dt=dt, t10=t10)
+
+
+def sod_final():
+    cfg = config_from_par(CASES["sod_c1"])
+    R = oracle.Reference(cfg)
+    try:
+        p0 = R.get_prim()
+        R.set_time(0.0, cfg.final_time)
+        t, dt, n = R.step(10 ** 6)
+        u, p = R.get_cons(), R.get_prim()
+    finally:
+        R.close()
+    g = R.ng
+    np.savez_compressed(OUT / "sod_c1_final.npz", p0=p0, u=u, p_row=p[g + 1, g + 1], t=t,
+                        n_steps=n, last_dt=dt)
+
+
+def michel_c2(nr=256):
+    cfg = config_from_par(MICHEL_C2.format(nr=nr))
+    R = oracle.Reference(cfg)
+    try:
+        p0 = seeded_noise(R.get_prim(), R.nx, R.ny, R.nz, R.ng, 2, 1.0e-3, 3435)
+        R.set_prim(p0)
+        R.set_time(0.0, cfg.final_time)
+        t, dt, n = R.step(10 ** 8)
+        u, p = R.get_cons(), R.get_prim()
+        ng = R.ng
+    finally:
+        R.close()
+    norms = michel_norms(p, cfg, ng)
+    np.savez_compressed(OUT / f"michel_c2_{nr}.npz", p0=p0, u=u, p_row=p[ng + 1, ng + 1], t=t,
+                        n_steps=n, norms=norms)
+    print(f"michel nr={nr}: {n} steps, L1/L2/Linf = {norms}")
+
+
+if __name__ == "__main__":
+    os.environ.setdefault("OMP_NUM_THREADS", "1")
+    which = sys.argv[1:] or STEP_CASES + ["sod_final", "michel_c2"]
+    for w in which:
+        if w == "sod_final":
+            sod_final()
+        elif w == "michel_c2":
+            michel_c2(256)
+        else:
+            step_case(w)
+        print("wrote", w, flush=True)
