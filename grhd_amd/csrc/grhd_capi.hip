@@ -161,8 +161,6 @@ Params make_params(const grhd_config &c)
     P.rs = c.riemann_solver;
     P.lim = c.tvd_limiter;
     P.sensor = c.shock_sensor ? 1 : 0;
-    P.kc = 8;
-    if (const char *e = std::getenv("GRHD_PHI_RUN")) P.kc = std::max(1, std::atoi(e));
     P.polar = (c.geometry == GRHD_GEOM_SPHERICAL || c.geometry == GRHD_GEOM_SPHEROIDAL);
     P.cyl = (c.geometry == GRHD_GEOM_CYLINDRICAL);
     P.gam = c.adiabatic_index;

@@ -39,7 +39,6 @@ struct Params {
     int mk;
     int nx, ny, nz, ng;
     int rec, rs, lim, sensor;
-    int kc;     // phi run length per thread in the column kernels
     int polar;  // spherical / spheroidal geometry
     int cyl;    // cylindrical geometry
     double gam, g1, mass, spin;
@@ -1038,8 +1037,13 @@ GRHD_HD double root_brent(double g1, double rho_floor, double D, double tau, dou
 }
 
 // root_safe_newton (equations.f90:249-340).  *brent is set when the
-// Brent fallback runs.
-GRHD_HD double root_newton(double g1, double rho_floor, double D, double tau, double Q, int &brent)
+// Brent fallback runs.  w_start > 0 starts the safeguarded iteration there
+// instead of at the bracket midpoint (warm start from the previous Lorentz
+// factor, used by the mixed-precision path only; the bracket, the
+// Newton/bisection safeguard, the tolerances and the Brent fallback are
+// unchanged).
+GRHD_HD double root_newton(double g1, double rho_floor, double D, double tau, double Q, int &brent,
+                           double w_start = 0.0)
 {
     const double x1 = 1.0, xacc = 1.0e-10;
     const double wg = (sqrt(Q) + D) / rmax(D, rho_floor);
@@ -1057,7 +1061,7 @@ GRHD_HD double root_newton(double g1, double rho_floor, double D, double tau, do
         xh = x1;
         xl = x2;
     }
-    root = 0.5 * (x1 + x2);
+    root = (w_start > x1 && w_start < x2) ? w_start : 0.5 * (x1 + x2);
     dxold = fabs(x2 - x1);
     dx = dxold;
     eval_res(g1, root, D, tau, Q, f, df);
@@ -1086,7 +1090,8 @@ GRHD_HD double root_newton(double g1, double rho_floor, double D, double tau, do
 // cons_to_prim (equations.f90:94-205).  c is repaired in place; returns true
 // when the cell was reset to the atmosphere.
 GRHD_HD bool c2p(double g1, double rho_floor, double p_floor, const double g[3][3],
-                 const double gi[3][3], double sqg, double c[NEQ], double q[NEQ], int &brent)
+                 const double gi[3][3], double sqg, double c[NEQ], double q[NEQ], int &brent,
+                 double w_start = 0.0)
 {
     const double vmax = 1.0 - 1.0e-5;
     const double Df = rho_floor * sqg, tf = p_floor * (g1 - 1.0) * sqg;
@@ -1107,7 +1112,7 @@ GRHD_HD bool c2p(double g1, double rho_floor, double p_floor, const double g[3][
         double Q = 0.0;
         for (int a = 0; a < 3; a++)
             for (int b = 0; b < 3; b++) Q = Q + gi[a][b] * c[EQ_VX + a] * c[EQ_VX + b];
-        const double W = root_newton(g1, rho_floor, c[EQ_DE], c[EQ_PR], Q, brent);
+        const double W = root_newton(g1, rho_floor, c[EQ_DE], c[EQ_PR], Q, brent, w_start);
         if (W < 1.0) {
             atm = true;
         } else {

@@ -78,12 +78,13 @@ __global__ __launch_bounds__(256) void k_flux(Params P, const double *__restrict
 }
 
 // --------------------------------------------------------- cell update --
-// stage 1..3: rhs (evolution.f90:85-88,137-139,180-182), RK combination
-// (grhd2.f90:243-249), cons_to_prim (grhd2.f90:252-260).  stage 0 writes the
-// rhs itself to u (parity entry point).  Each thread owns a run of P.kc cells
-// along phi and evaluates the centre metric (Christoffels, dg, gamma^ij)
-// once for the run.
-template <int MK, typename R>
+// stage 1..3: rhs (evolution.f90:85-88,137-139,180-182) and the RK
+// combination (grhd2.f90:243-249) into u; cons_to_prim follows in
+// k_c2p_stage.  stage 0 writes the rhs itself to u (parity entry point).
+// The metric depends on (r, theta) only; RUN > 1 lets a thread reuse its
+// centre metric over RUN consecutive phi cells (fewer metric evaluations, but
+// twice the registers: measured slower, so the launcher uses RUN = 1).
+template <int MK, typename R, int RUN>
 __global__ __launch_bounds__(256) void k_update(Params P, double *__restrict__ p,
                                                 const R *__restrict__ Fx, const R *__restrict__ Fy,
                                                 const R *__restrict__ Fz, int64_t fsx, int64_t fsy,
@@ -91,7 +92,7 @@ __global__ __launch_bounds__(256) void k_update(Params P, double *__restrict__ p
                                                 double *__restrict__ u, int stage, double dt,
                                                 unsigned long long *__restrict__ counters)
 {
-    const int nchunk = (P.nz + P.kc - 1) / P.kc;
+    const int nchunk = (P.nz + RUN - 1) / RUN;
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (int64_t)P.nx * P.ny * nchunk) return;
     const int64_t ncell = (int64_t)P.nx * P.ny * P.nz;
@@ -101,8 +102,7 @@ __global__ __launch_bounds__(256) void k_update(Params P, double *__restrict__ p
     const int i = i0 + 1, j = j0 + 1;
     CentreMetric m;
     metric_centre<MK>(P, xc(P, i), yc(P, j), m);
-    const int k0b = chunk * P.kc, k0e = min(k0b + P.kc, P.nz);
-    unsigned n_atm = 0, n_brent = 0;
+    const int k0b = chunk * RUN, k0e = min(k0b + RUN, P.nz);
     for (int k0 = k0b; k0 < k0e; k0++) {
         const int k = k0 + 1;
         const int64_t tc = ((int64_t)k0 * P.ny + j0) * P.nx + i0;
@@ -147,17 +147,48 @@ __global__ __launch_bounds__(256) void k_update(Params P, double *__restrict__ p
             else if (stage == 2) c[v] = 0.75 * a0 + 0.25 * (u[v * ncell + tc] + rhs[v] * dt);
             else c[v] = (a0 + 2.0 * (u[v * ncell + tc] + rhs[v] * dt)) / 3.0;
         }
-        int brent = 0;
-        n_atm += c2p(P.g1, P.rho_floor, P.p_floor, m.g, m.ginv, m.sqg, c, q, brent) ? 1u : 0u;
-        n_brent += brent ? 1u : 0u;
 #pragma unroll
-        for (int v = 0; v < NEQ; v++) {
-            u[v * ncell + tc] = c[v];
-            p[v * P.vstride + pc] = q[v];
-        }
+        for (int v = 0; v < NEQ; v++) u[v * ncell + tc] = c[v];
     }
-    if (n_atm) atomicAdd(&counters[0], (unsigned long long)n_atm);
-    if (n_brent) atomicAdd(&counters[1], (unsigned long long)n_brent);
+}
+
+// cons_to_prim after the RK combination (grhd2.f90:252-260), one cell per
+// thread: a serial, divide-heavy root solve per cell, kept apart from the
+// sources so that it runs at a higher occupancy.  WARM starts the root solve
+// from the cell's previous Lorentz factor (mixed-precision path).
+template <int MK, bool WARM>
+__global__ __launch_bounds__(256) void k_c2p_stage(Params P, double *__restrict__ p,
+                                                   double *__restrict__ u,
+                                                   unsigned long long *__restrict__ counters)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t ncell = (int64_t)P.nx * P.ny * P.nz;
+    if (t >= ncell) return;
+    const int i = (int)(t % P.nx) + 1;
+    const int64_t rem = t / P.nx;
+    const int j = (int)(rem % P.ny) + 1, k = (int)(rem / P.ny) + 1;
+    double g[3][3], gi[3][3], sqg, alpha, beta[3], q[NEQ], c[NEQ];
+    metric_c2p<MK>(P, xc(P, i), yc(P, j), g, gi, sqg, alpha, beta);
+    const int64_t pc = pindex(P, i, j, k);
+    double w_start = 0.0;
+    if (WARM) {
+        double v[3] = {p[2 * P.vstride + pc], p[3 * P.vstride + pc], p[4 * P.vstride + pc]};
+        double vsq = 0.0;
+        for (int a = 0; a < 3; a++)
+            for (int b = 0; b < 3; b++) vsq = vsq + g[a][b] * v[a] * v[b];
+        w_start = (vsq < 1.0) ? 1.0 / sqrt(1.0 - vsq) : 0.0;
+    }
+#pragma unroll
+    for (int v = 0; v < NEQ; v++) c[v] = u[v * ncell + t];
+    int brent = 0;
+    const bool atm = c2p(P.g1, P.rho_floor, P.p_floor, g, gi, sqg, c, q, brent, w_start);
+    if (atm) atomicAdd(&counters[0], 1ull);
+    if (brent) atomicAdd(&counters[1], 1ull);
+#pragma unroll
+    for (int v = 0; v < NEQ; v++) {
+        u[v * ncell + t] = c[v];
+        p[v * P.vstride + pc] = q[v];
+    }
 }
 
 template <int MK, typename R>
@@ -168,13 +199,19 @@ void launch_stage_mk(const Params &P, hipStream_t s, double *p, void *const F[3]
     const int64_t nfx = (int64_t)(P.nx + 1) * P.ny * P.nz;
     const int64_t nfy = (int64_t)P.nx * (P.ny + 1) * P.nz;
     const int64_t nfz = (int64_t)P.nx * P.ny * (P.nz + 1);
-    const int64_t ncol = (int64_t)P.nx * P.ny * ((P.nz + P.kc - 1) / P.kc);
+    constexpr int RUN = 1;  // one cell per thread: 4 waves/SIMD beat metric reuse along phi
+    const int64_t ncol = (int64_t)P.nx * P.ny * ((P.nz + RUN - 1) / RUN);
     R *Fx = (R *)F[0], *Fy = (R *)F[1], *Fz = (R *)F[2];
     k_flux<MK, 1, R><<<blocks_for(nfx, bs), bs, 0, s>>>(P, p, Fx, fstr[0]);
     if (P.ny > 1) k_flux<MK, 2, R><<<blocks_for(nfy, bs), bs, 0, s>>>(P, p, Fy, fstr[1]);
     if (P.nz > 1) k_flux<MK, 3, R><<<blocks_for(nfz, bs), bs, 0, s>>>(P, p, Fz, fstr[2]);
-    k_update<MK, R><<<blocks_for(ncol, bs), bs, 0, s>>>(P, p, Fx, Fy, Fz, fstr[0], fstr[1], fstr[2], u0,
+    k_update<MK, R, RUN><<<blocks_for(ncol, bs), bs, 0, s>>>(P, p, Fx, Fy, Fz, fstr[0], fstr[1], fstr[2], u0,
                                                         u, stage, dt, counters);
+    if (stage > 0) {
+        const int64_t ncell = (int64_t)P.nx * P.ny * P.nz;
+        constexpr bool warm = sizeof(R) == sizeof(float);
+        k_c2p_stage<MK, warm><<<blocks_for(ncell, bs), bs, 0, s>>>(P, p, u, counters);
+    }
 }
 
 // Instantiates the stage launcher of one face arithmetic for every metric kind.

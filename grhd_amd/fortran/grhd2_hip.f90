@@ -1,0 +1,165 @@
+! =============================================================================
+! grhd2_hip -- driver of the GRHD2 update on one MI355X.
+!
+!   grhd2_hip [--check] file.par
+!
+! The reference main program (grhd2.f90:37-393) with its update replaced by
+! the HIP library: the .par file is read with the reference's semantics
+! (grhd_par), the library builds the problem (grhd_create validates it,
+! grhd_initial_conditions sets the initial data and U = P2C(p)), and the time
+! loop -- CFL dt, final-time clamp, SSP-RK3 stages with con2prim and boundaries
+! -- runs on the device through grhd_advance.  The seeded perturbation is
+! injected when t reaches perturbation_time (grhd2.f90:186-227) and the mass
+! monitor is printed every mass_monitor_stride steps (grhd2.f90:282-301).  At the
+! end the conserved state up(5,nx,ny,nz) is written in the reference's order to
+! <output_folder>/<output_prefix>_final_u.bin (raw float64).
+! =============================================================================
+program grhd2_hip
+  use grhd_c
+  use grhd_par
+  implicit none
+
+  type(grhd_config) :: cfg
+  type(run_control) :: rc
+  type(c_ptr) :: ctx
+  integer(c_int) :: st
+  integer(c_int32_t) :: dims(4), done, chunk, n_adv
+  integer(c_int64_t) :: counters(4)
+  character(len=1024) :: arg1, parfile
+  logical :: check_only, perturbed
+  real(c_double) :: t, last_dt, dt, dx, dy, dz, mass
+  real(c_double), allocatable :: u(:,:,:,:)
+  integer :: nargs, nx, ny, nz, unit_out
+  integer(c_int64_t) :: nsteps
+  real(c_double) :: wall1
+  integer(8) :: c0, c1, crate
+
+  nargs = command_argument_count()
+  check_only = .false.
+  if (nargs == 1) then
+    call get_command_argument(1, parfile)
+  else if (nargs == 2) then
+    call get_command_argument(1, arg1)
+    call get_command_argument(2, parfile)
+    if (trim(arg1) /= '--check') then
+      write(*,*) 'usage: grhd2_hip [--check] file.par'
+      error stop 1
+    end if
+    check_only = .true.
+  else
+    write(*,*) 'usage: grhd2_hip [--check] file.par'
+    error stop 1
+  end if
+
+  call read_par(trim(parfile), cfg, rc)
+  if (rc%restart) then
+    write(*,*) 'ERROR: restart from a checkpoint is not supported by this driver'
+    error stop 1
+  end if
+
+  st = grhd_create(cfg, ctx)
+  if (st /= GRHD_OK) then
+    write(*,'(A,1X,A)') 'ERROR in '//trim(parfile)//':', trim(grhd_error_message(c_null_ptr))
+    error stop 1
+  end if
+  call require(grhd_dims(ctx, dims), 'grhd_dims')
+  nx = dims(1); ny = dims(2); nz = dims(3)
+  write(*,'(A,A)') 'problem: ', trim(problem_label(cfg%problem))
+  write(*,'(A,3(I0,1X),A,I0)') 'grid nx ny nz: ', nx, ny, nz, ' ghosts: ', dims(4)
+  write(*,'(A,ES12.5,A,ES12.5)') 'final_time: ', cfg%final_time, '  cfl: ', cfg%cfl
+  if (check_only) then
+    write(*,'(A)') 'parameter file valid; simulation not started.'
+    call require(grhd_destroy(ctx), 'grhd_destroy')
+    stop
+  end if
+
+  call require(grhd_initial_conditions(ctx), 'grhd_initial_conditions')
+  allocate(u(5, nx, ny, nz))
+  ! logical spacings (initialization.f90:88-100)
+  if (cfg%log_r /= 0) then
+    dx = (log(cfg%r_max) - log(cfg%r_min)) / dble(max(1, nx))
+  else
+    dx = (cfg%r_max - cfg%r_min) / dble(max(1, nx))
+  end if
+  dy = (cfg%y_max - cfg%y_min) / dble(max(1, ny))
+  dz = (cfg%z_max - cfg%z_min) / dble(max(1, nz))
+  call require(grhd_get_conserved(ctx, u), 'grhd_get_conserved')
+  mass = sum(u(1,:,:,:)) * dx * dy * dz
+  write(*,'(A,ES22.14)') 'Initial mass in domain: ', mass
+  call require(grhd_compute_dt(ctx, dt), 'grhd_compute_dt')
+  write(*,'(A,ES22.14)') 'dt (CFL, metric light cone): ', dt
+
+  t = 0.0d0
+  nsteps = 0
+  perturbed = (rc%perturbation == 0)
+  chunk = int(max(1, rc%mass_monitor_stride), c_int32_t)
+  if (rc%mass_monitor_stride <= 0) chunk = huge(chunk)
+  call system_clock(c0, crate)
+  do while (t < cfg%final_time)
+    if (.not. perturbed .and. t >= rc%perturbation_time) then
+      call require(grhd_perturb(ctx, int(rc%perturbation, c_int32_t), rc%perturbation_amplitude, &
+                                int(rc%perturbation_seed, c_int64_t)), 'grhd_perturb')
+      perturbed = .true.
+      write(*,'(A,ES14.6)') '>>> perturbation injected at t = ', t
+    end if
+    n_adv = chunk
+    if (.not. perturbed) n_adv = min(n_adv, steps_until(rc%perturbation_time))
+    call require(grhd_advance(ctx, cfg%final_time, n_adv, t, done, last_dt), 'grhd_advance')
+    nsteps = nsteps + done
+    if (done == 0) exit
+    if (rc%mass_monitor_stride > 0 .and. mod(nsteps, int(rc%mass_monitor_stride, 8)) == 0) then
+      call require(grhd_get_conserved(ctx, u), 'grhd_get_conserved')
+      mass = sum(u(1,:,:,:)) * dx * dy * dz
+      write(*,'(A,I0,A,ES14.6,A,ES22.14)') 'step ', nsteps, '  t = ', t, '  total mass: ', mass
+    end if
+  end do
+  call require(grhd_synchronize(ctx), 'grhd_synchronize')
+  call system_clock(c1)
+  wall1 = dble(c1 - c0) / dble(crate)
+
+  call require(grhd_get_conserved(ctx, u), 'grhd_get_conserved')
+  call require(grhd_counters(ctx, counters), 'grhd_counters')
+  write(*,'(A)') '========================================================='
+  write(*,'(A,F14.6)') ' finished at t = ', t
+  write(*,'(A,I0)') ' time steps: ', nsteps
+  write(*,'(A,F12.3,A)') ' wall time: ', wall1, ' s'
+  if (wall1 > 0.0d0) write(*,'(A,ES12.4)') ' cell-updates/s: ', &
+      dble(nx) * dble(ny) * dble(nz) * dble(nsteps) / wall1
+  write(*,'(A,I0)') ' cells reset to atmosphere by con2prim: ', counters(1)
+  write(*,'(A)') '========================================================='
+
+  call execute_command_line('mkdir -p '//trim(rc%output_folder))
+  open(newunit=unit_out, file=trim(rc%output_folder)//'/'//trim(rc%output_prefix)//'_final_u.bin', &
+       access='stream', form='unformatted', status='replace')
+  write(unit_out) u
+  close(unit_out)
+  call require(grhd_destroy(ctx), 'grhd_destroy')
+
+contains
+
+  ! Steps of the reference loop (dt, final clamp, t += dt) from t until
+  ! t >= t_event, counted with the same accumulation grhd_advance performs.
+  integer(c_int32_t) function steps_until(t_event)
+    real(c_double), intent(in) :: t_event
+    real(c_double) :: tt, d
+    tt = t
+    steps_until = 0
+    do while (tt < t_event .and. tt < cfg%final_time .and. steps_until < chunk)
+      d = dt
+      if (tt + d > cfg%final_time) d = cfg%final_time - tt
+      tt = tt + d
+      steps_until = steps_until + 1
+    end do
+  end function steps_until
+
+  subroutine require(status, what)
+    integer(c_int), intent(in) :: status
+    character(len=*), intent(in) :: what
+    if (status /= GRHD_OK) then
+      write(*,'(A,A,A,I0,A,A)') 'ERROR: ', what, ' failed (', status, '): ', &
+        trim(grhd_error_message(ctx))
+      error stop 1
+    end if
+  end subroutine require
+
+end program grhd2_hip
