@@ -153,6 +153,19 @@ def cpu_baseline(seconds=15.0, nx=64, ny=32, nz=32):
     }
 
 
+def copy_calibration(nbytes=1 << 30, reps=5):
+    """A plain device-to-device copy of nbytes, reps times: its FETCH_SIZE and
+    WRITE_SIZE per dispatch give the counters' scale against known traffic
+    (nbytes read and nbytes written per dispatch)."""
+    import torch
+    x = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(reps):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes": nbytes, "reps": reps}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -164,7 +177,12 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--cpu-grid", default="64,32,32")
     ap.add_argument("--cpu-child", default=None)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain device copy of known size (for HBM counter scaling)")
     a = ap.parse_args()
+    if a.copy_calibration:
+        copy_calibration()
+        return
     cnx, cny, cnz = (int(v) for v in a.cpu_grid.split(","))
     if a.cpu_child:
         cpu_child(a.cpu_child, cnx, cny, cnz, a.cpu_seconds)

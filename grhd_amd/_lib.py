@@ -86,6 +86,7 @@ _lib = None
 
 _SIGS = {
     "grhd_create": ([ctypes.POINTER(Config), ctypes.POINTER(ctypes.c_void_p)], ctypes.c_int),
+    "grhd_validate": ([ctypes.POINTER(Config)], ctypes.c_int),
     "grhd_destroy": ([ctypes.c_void_p], ctypes.c_int),
     "grhd_last_error": ([ctypes.c_void_p], ctypes.c_char_p),
     "grhd_dims": ([ctypes.c_void_p, ctypes.POINTER(_I)], ctypes.c_int),

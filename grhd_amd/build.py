@@ -66,5 +66,32 @@ def build(verbose: bool = True, jobs: int = 4) -> Path:
     return LIB
 
 
+FORTRAN = PKG / "fortran"
+DRIVER = PKG / "bin" / "grhd2_hip"
+FC = os.environ.get("FC", "/opt/rocm/bin/amdflang")
+
+
+def build_driver(verbose: bool = True) -> Path:
+    """Fortran driver (grhd_amd/bin/grhd2_hip) linked against libgrhd_hip.so."""
+    out = DRIVER.parent
+    obj = out / "_obj"
+    obj.mkdir(parents=True, exist_ok=True)
+    objs = []
+    for src in ["grhd_c.f90", "grhd_par.f90", "grhd2_hip.f90"]:
+        o = obj / (Path(src).stem + ".o")
+        cmd = [FC, "-O2", f"-J{obj}", "-c", str(FORTRAN / src), "-o", str(o)]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.check_call(cmd)
+        objs.append(o)
+    cmd = [FC, "-O2", "-o", str(DRIVER), *map(str, objs), f"-L{PKG}", "-lgrhd_hip",
+           "-Wl,-rpath,$ORIGIN/.."]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.check_call(cmd)
+    return DRIVER
+
+
 if __name__ == "__main__":
     build(verbose="-q" not in sys.argv)
+    build_driver(verbose="-q" not in sys.argv)

@@ -354,6 +354,18 @@ const char *grhd_last_error(const grhd_ctx *ctx)
     return g_create_err.c_str();
 }
 
+int grhd_validate(const grhd_config *cfg)
+{
+    if (!cfg) return fail(nullptr, GRHD_ERR_INVALID_ARGUMENT, "null argument");
+    std::string msg;
+    if (validate(*cfg, msg)) return fail(nullptr, GRHD_ERR_INVALID_CONFIG, msg);
+    const Params P = make_params(*cfg);
+    if ((P.problem == 5 && (P.nx < P.ng || (P.nz > 1 && P.nz < P.ng))) ||
+        (P.problem == 11 && P.ny < P.ng) || (P.problem == 12 && P.ny < P.ng))
+        return fail(nullptr, GRHD_ERR_INVALID_CONFIG, "grid smaller than the ghost depth");
+    return GRHD_OK;
+}
+
 int grhd_create(const grhd_config *cfg, grhd_ctx **out)
 {
     if (!cfg || !out) return fail(nullptr, GRHD_ERR_INVALID_ARGUMENT, "null argument");

@@ -57,6 +57,19 @@ program grhd2_hip
     error stop 1
   end if
 
+  st = grhd_validate(cfg)
+  if (st /= GRHD_OK) then
+    write(*,'(A,1X,A)') 'ERROR in '//trim(parfile)//':', trim(grhd_error_message(c_null_ptr))
+    error stop 1
+  end if
+  write(*,'(A,A)') 'problem: ', trim(problem_label(cfg%problem))
+  write(*,'(A,3(I0,1X))') 'grid nx ny nz: ', cfg%nx, cfg%ny, cfg%nz
+  write(*,'(A,ES12.5,A,ES12.5)') 'final_time: ', cfg%final_time, '  cfl: ', cfg%cfl
+  if (check_only) then
+    write(*,'(A)') 'parameter file valid; simulation not started.'
+    stop
+  end if
+
   st = grhd_create(cfg, ctx)
   if (st /= GRHD_OK) then
     write(*,'(A,1X,A)') 'ERROR in '//trim(parfile)//':', trim(grhd_error_message(c_null_ptr))
@@ -64,14 +77,6 @@ program grhd2_hip
   end if
   call require(grhd_dims(ctx, dims), 'grhd_dims')
   nx = dims(1); ny = dims(2); nz = dims(3)
-  write(*,'(A,A)') 'problem: ', trim(problem_label(cfg%problem))
-  write(*,'(A,3(I0,1X),A,I0)') 'grid nx ny nz: ', nx, ny, nz, ' ghosts: ', dims(4)
-  write(*,'(A,ES12.5,A,ES12.5)') 'final_time: ', cfg%final_time, '  cfl: ', cfg%cfl
-  if (check_only) then
-    write(*,'(A)') 'parameter file valid; simulation not started.'
-    call require(grhd_destroy(ctx), 'grhd_destroy')
-    stop
-  end if
 
   call require(grhd_initial_conditions(ctx), 'grhd_initial_conditions')
   allocate(u(5, nx, ny, nz))

@@ -11,7 +11,7 @@ module grhd_c
   implicit none
   private
   public :: grhd_config, c_ptr, c_null_ptr, c_int, c_double, c_int32_t, c_int64_t
-  public :: grhd_create, grhd_destroy, grhd_last_error, grhd_dims, grhd_initial_conditions
+  public :: grhd_create, grhd_validate, grhd_destroy, grhd_last_error, grhd_dims, grhd_initial_conditions
   public :: grhd_set_primitives, grhd_get_primitives, grhd_set_conserved, grhd_get_conserved
   public :: grhd_compute_dt, grhd_step, grhd_advance, grhd_perturb, grhd_counters
   public :: grhd_last_timing, grhd_synchronize, grhd_error_message
@@ -56,6 +56,10 @@ module grhd_c
       type(grhd_config), intent(in) :: cfg
       type(c_ptr), intent(out) :: ctx
     end function grhd_create
+    integer(c_int) function grhd_validate(cfg) bind(C, name='grhd_validate')
+      import :: c_int, grhd_config
+      type(grhd_config), intent(in) :: cfg
+    end function grhd_validate
     integer(c_int) function grhd_destroy(ctx) bind(C, name='grhd_destroy')
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx

@@ -117,6 +117,9 @@ typedef struct grhd_ctx grhd_ctx;
  * (initialization.f90:23-391, 452-457): no metric is stored, it is evaluated
  * in the kernels.  Validation follows parameters.f90:359-588. */
 int grhd_create(const grhd_config *cfg, grhd_ctx **out);
+/* The validation of grhd_create alone, without touching a device (the
+ * reference's --check, grhd2.f90:42-55); the message is grhd_last_error(NULL). */
+int grhd_validate(const grhd_config *cfg);
 int grhd_destroy(grhd_ctx *ctx);
 /* Message for the last failure of ctx; ctx == NULL gives the last grhd_create failure. */
 const char *grhd_last_error(const grhd_ctx *ctx);

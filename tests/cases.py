@@ -177,3 +177,19 @@ riemann_solver = hllc
 # smooth equilibria also exercise the non-trivial branches
 NOISY = {"fm3d_ks_mp5_hllc_sensor": (2, 1.0e-2, 3435), "fm_equatorial_ks_log": (1, 1.0e-2, 3435),
          "michel_ef_log": (2, 1.0e-3, 3435)}
+
+
+# Michel accretion C2 (the reference's convergence case for the EF log-r
+# metric): nr cells in r in [1.8, 20], critical radius 8, WENO5 + HLLE, to
+# t = 500 with 1e-3 seeded density noise.
+MICHEL_C2 = """problem = michel
+logarithmic_r = true
+nx = {nr}
+nz = 1
+r_min = 1.8
+r_max = 20
+michel_critical_radius = 8
+reconstruction = weno5
+riemann_solver = hlle
+final_time = 500
+"""

@@ -24,41 +24,23 @@ sys.path.insert(0, str(ROOT))
 
 import oracle  # noqa: E402
 from grhd_amd.params import config_from_par  # noqa: E402
-from tests.cases import CASES, NOISY  # noqa: E402
+from tests.cases import CASES, MICHEL_C2, NOISY  # noqa: E402
+from tests.helpers import michel_norms as helpers_michel_norms  # noqa: E402
 from tests.helpers import seeded_noise  # noqa: E402
 
 OUT = Path(__file__).resolve().parent
 STEP_CASES = ["sod_c1", "fm3d_ks_mp5_hllc_sensor", "michel_ef_log", "khi_weno3_sensor",
               "jet_cylindrical", "fm_sagittal_ef_log", "shu_tvd_minmod_sensor"]
 
-MICHEL_C2 = """problem = michel
-logarithmic_r = true
-nx = {nr}
-nz = 1
-r_min = 1.8
-r_max = 20
-michel_critical_radius = 8
-reconstruction = weno5
-riemann_solver = hlle
-final_time = 500
-"""
-
 
 def michel_norms(p, cfg, ng):
-    """calc_convergence_norms for MichelA (output.f90:1189-1213, 1223-1224)."""
+    """Norms of p against the reference's analytic initial data."""
     R = oracle.Reference(cfg)
     try:
         exact = R.get_prim()
     finally:
         R.close()
-    nx = cfg.nx
-    i0 = max(1, int(0.10 * nx))
-    i1 = min(nx, max(i0, int(0.95 * nx)))
-    j = max(1, cfg.ny // 2)
-    rows = slice(ng + i0, ng + i1 + 1)
-    d = np.abs(p[ng + 1, ng + j, rows, 0] - exact[ng + 1, ng + j, rows, 0])
-    n = i1 - i0 + 1
-    return np.array([d.sum() / n, np.sqrt((d ** 2).sum() / n), d.max()])
+    return helpers_michel_norms(p, exact, cfg.nx, cfg.ny, ng)
 
 
 def step_case(name):

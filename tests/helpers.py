@@ -49,3 +49,18 @@ def seeded_noise(p, nx, ny, nz, ng, kind, amp, seed):
         q[sl + (0,)] *= f
         q[sl + (1,)] *= f ** 1.3333333333333333
     return q
+
+
+def michel_norms(p, exact, nx, ny, ng):
+    """Density error norms of the Michel problem against the analytic
+    solution, as calc_convergence_norms computes them (output.f90:1189-1213,
+    1223-1224): cells i0 = max(1, int(0.10 nx)) .. i1 = min(nx, max(i0,
+    int(0.95 nx))) of row j = max(1, ny // 2), k = 1; returns (L1, L2, Linf)
+    with L1 and L2 divided by the cell count."""
+    i0 = max(1, int(0.10 * nx))
+    i1 = min(nx, max(i0, int(0.95 * nx)))
+    j = max(1, ny // 2)
+    rows = slice(ng + i0, ng + i1 + 1)
+    d = np.abs(p[ng + 1, ng + j, rows, 0] - exact[ng + 1, ng + j, rows, 0])
+    n = i1 - i0 + 1
+    return np.array([d.sum() / n, np.sqrt((d ** 2).sum() / n), d.max()])
