@@ -4,7 +4,7 @@
  * The reference has no FFI: its update path is a set of Fortran module
  * procedures sharing module state.  Each entry point below replaces one of
  * those procedures (or one block of the reference main program) and is what a
- * Fortran driver binds through iso_c_binding (grhd_amd/fortran/grhd_c_binding.f90)
+ * Fortran driver binds through iso_c_binding (grhd_amd/fortran/grhd_c.f90)
  * and what the Python tests bind through ctypes.
  *
  * Conventions

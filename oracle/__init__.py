@@ -266,7 +266,14 @@ class Reference:
         with open(path, "w") as fh:
             fh.write(text)
         b = path.encode()
-        rc = self.lib.ref_init(b, len(b))
+        # the reference creates its preset output folders relative to the
+        # working directory during set-up: keep them inside the temporary one
+        cwd = os.getcwd()
+        os.chdir(self._tmp.name)
+        try:
+            rc = self.lib.ref_init(b, len(b))
+        finally:
+            os.chdir(cwd)
         if rc != 0:
             raise RuntimeError("reference init failed")
         dims = (ctypes.c_int * 8)()
