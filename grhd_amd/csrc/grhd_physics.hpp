@@ -78,8 +78,13 @@ template <> struct Lim<float> {
     static GRHD_HD float eps() { return FLT_EPSILON; }
 };
 
-template <typename R> GRHD_HD R rmax(R a, R b) { return a > b ? a : b; }
-template <typename R> GRHD_HD R rmin(R a, R b) { return a < b ? a : b; }
+// MAX/MIN of the reference.  The hardware min/max (one instruction instead of
+// a compare and two selects for FP64) returns the same value for every pair of
+// non-NaN operands; only the sign of a zero result may differ when the operands
+// are +0 and -0, which no later operation of the path can turn into a different
+// non-zero value.
+template <typename R> GRHD_HD R rmax(R a, R b) { return fmax(a, b); }
+template <typename R> GRHD_HD R rmin(R a, R b) { return fmin(a, b); }
 template <typename R> GRHD_HD R sgn1(R a) { return copysign((R)1, a); }
 GRHD_HD bool finite(double a) { return isfinite(a); }
 GRHD_HD bool finite(float a) { return isfinite(a); }
@@ -1091,7 +1096,7 @@ GRHD_HD double root_newton(double g1, double rho_floor, double D, double tau, do
 // when the cell was reset to the atmosphere.
 GRHD_HD bool c2p(double g1, double rho_floor, double p_floor, const double g[3][3],
                  const double gi[3][3], double sqg, double c[NEQ], double q[NEQ], int &brent,
-                 double w_start = 0.0)
+                 double w_start = 0.0, bool *repaired = nullptr)
 {
     const double vmax = 1.0 - 1.0e-5;
     const double Df = rho_floor * sqg, tf = p_floor * (g1 - 1.0) * sqg;
@@ -1105,6 +1110,7 @@ GRHD_HD bool c2p(double g1, double rho_floor, double p_floor, const double g[3][
         const double Smax = rmax(0.0, E * E - c[EQ_DE] * c[EQ_DE]) * vmax;
         if (Ssq > Smax && Ssq > 1.0e-20) {
             const double sf = sqrt(Smax / Ssq);
+            if (repaired) *repaired = true;
             c[EQ_VX] = c[EQ_VX] * sf;
             c[EQ_VY] = c[EQ_VY] * sf;
             c[EQ_VZ] = c[EQ_VZ] * sf;
@@ -1136,6 +1142,7 @@ GRHD_HD bool c2p(double g1, double rho_floor, double p_floor, const double g[3][
         }
     }
     if (atm) {
+        if (repaired) *repaired = true;
         q[EQ_DE] = rho_floor;
         q[EQ_PR] = p_floor;
         q[EQ_VX] = q[EQ_VY] = q[EQ_VZ] = 0.0;

@@ -11,6 +11,17 @@ static inline unsigned blocks_for(int64_t n, int bs) { return (unsigned)((n + bs
 
 template <int MK> constexpr bool kSRHD = (MK == MK_MINK_CART);
 
+// Workgroups are dispatched round-robin over the 8 XCDs (block b runs on XCD
+// b % 8), and each XCD has its own L2.  Renumbering so that every XCD walks a
+// contiguous range of logical blocks keeps neighbouring rows -- which share
+// stencil data -- in one L2.  A bijection of [0, nb) for any nb.
+constexpr unsigned kXCD = 8;
+__device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned nb)
+{
+    const unsigned per = nb / kXCD, rem = nb % kXCD, x = b % kXCD, q = b / kXCD;
+    return x * per + (x < rem ? x : rem) + q;
+}
+
 // -------------------------------------------------------------- fluxes --
 // Face arrays have shape [5][nfz][nfy][nfx] with nf_d = n_d + (d == DIR).
 // Lanes of a wave are consecutive in i, so every load and store is coalesced.
@@ -41,7 +52,7 @@ __global__ __launch_bounds__(256) void k_flux(Params P, const double *__restrict
     // one face per thread: the face kernels are limited by their register
     // footprint (Riemann solver state), so they keep the shortest live ranges
     const int nfx = P.nx + (DIR == 1), nfy = P.ny + (DIR == 2), nfz = P.nz + (DIR == 3);
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t t = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (t >= (int64_t)nfx * nfy * nfz) return;
     const int a = (int)(t % nfx);
     const int64_t rem = t / nfx;
@@ -93,7 +104,7 @@ __global__ __launch_bounds__(256) void k_update(Params P, double *__restrict__ p
                                                 unsigned long long *__restrict__ counters)
 {
     const int nchunk = (P.nz + RUN - 1) / RUN;
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t t = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (t >= (int64_t)P.nx * P.ny * nchunk) return;
     const int64_t ncell = (int64_t)P.nx * P.ny * P.nz;
     const int i0 = (int)(t % P.nx);
@@ -161,7 +172,7 @@ __global__ __launch_bounds__(256) void k_c2p_stage(Params P, double *__restrict_
                                                    double *__restrict__ u,
                                                    unsigned long long *__restrict__ counters)
 {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t t = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     const int64_t ncell = (int64_t)P.nx * P.ny * P.nz;
     if (t >= ncell) return;
     const int i = (int)(t % P.nx) + 1;
@@ -181,13 +192,16 @@ __global__ __launch_bounds__(256) void k_c2p_stage(Params P, double *__restrict_
 #pragma unroll
     for (int v = 0; v < NEQ; v++) c[v] = u[v * ncell + t];
     int brent = 0;
-    const bool atm = c2p(P.g1, P.rho_floor, P.p_floor, g, gi, sqg, c, q, brent, w_start);
+    bool repaired = false;
+    const bool atm = c2p(P.g1, P.rho_floor, P.p_floor, g, gi, sqg, c, q, brent, w_start, &repaired);
     if (atm) atomicAdd(&counters[0], 1ull);
     if (brent) atomicAdd(&counters[1], 1ull);
 #pragma unroll
-    for (int v = 0; v < NEQ; v++) {
-        u[v * ncell + t] = c[v];
-        p[v * P.vstride + pc] = q[v];
+    for (int v = 0; v < NEQ; v++) p[v * P.vstride + pc] = q[v];
+    // U changes only where con2prim rescales the momentum or resets the cell
+    if (repaired) {
+#pragma unroll
+        for (int v = 0; v < NEQ; v++) u[v * ncell + t] = c[v];
     }
 }
 
