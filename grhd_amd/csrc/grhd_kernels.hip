@@ -26,7 +26,7 @@ __global__ __launch_bounds__(256) void k_p2c(Params P, const double *__restrict_
     metric_c2p<MK>(P, xc(P, i), yc(P, j), g, gi, sqg, alpha, beta);
     const int64_t pc = pindex(P, i, j, k);
     for (int v = 0; v < NEQ; v++) q[v] = p[v * P.vstride + pc];
-    p2c<double>(P.g1, q, g, sqg, U);
+    p2c<double>(P.g1, q, Mat3<double>{g}, sqg, U);
     for (int v = 0; v < NEQ; v++) u[v * ncell + t] = U[v];
 }
 

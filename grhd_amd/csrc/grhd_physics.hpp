@@ -389,13 +389,20 @@ GRHD_HD void metric_c2p(const Params &P, double xp, double yp, double g[3][3], d
 // ------------------------------------------------------- equations (face) --
 // Prelude shared by prim_to_cons, calc_fluxes and the wave speeds:
 // v^2 = gamma_ij v^i v^j with the causal clamp to v_max (equations.f90:49-63).
-template <typename R>
-GRHD_HD R vsq_clamp(const R g[3][3], R v[3])
+// G is any metric accessor with gam(a, b): Mat3 over a 3x3 array, or the
+// face geometry FaceGeo.
+template <typename R> struct Mat3 {
+    const R (*m)[3];
+    GRHD_HD R gam(int a, int b) const { return m[a][b]; }
+};
+
+template <typename R, typename G>
+GRHD_HD R vsq_clamp(const G &g, R v[3])
 {
     const R vmax = (R)(1.0 - 1.0e-5);
     R vsq = 0;
     for (int a = 0; a < 3; a++)
-        for (int b = 0; b < 3; b++) vsq = vsq + g[a][b] * v[a] * v[b];
+        for (int b = 0; b < 3; b++) vsq = vsq + g.gam(a, b) * v[a] * v[b];
     if (vsq >= vmax) {
         v[0] = v[0] * sqrt(vmax / vsq);
         v[1] = v[1] * sqrt(vmax / vsq);
@@ -405,22 +412,22 @@ GRHD_HD R vsq_clamp(const R g[3][3], R v[3])
     return vsq;
 }
 
-template <typename R>
-GRHD_HD void lower(const R g[3][3], const R v[3], R vc[3])
+template <typename R, typename G>
+GRHD_HD void lower(const G &g, const R v[3], R vc[3])
 {
     for (int a = 0; a < 3; a++) {
         vc[a] = 0;
-        for (int b = 0; b < 3; b++) vc[a] = vc[a] + g[a][b] * v[b];
+        for (int b = 0; b < 3; b++) vc[a] = vc[a] + g.gam(a, b) * v[b];
     }
 }
 
 // prim_to_cons (equations.f90:30-88) / prim_to_cons_face (fluxes.f90:76-122).
-template <typename R>
-GRHD_HD void p2c(R g1, const R q[NEQ], const R g[3][3], R sqg, R U[NEQ])
+template <typename R, typename G>
+GRHD_HD void p2c(R g1, const R q[NEQ], const G &g, R sqg, R U[NEQ])
 {
     R v[3] = {q[EQ_VX], q[EQ_VY], q[EQ_VZ]}, vc[3];
-    const R vsq = vsq_clamp(g, v);
-    lower(g, v, vc);
+    const R vsq = vsq_clamp<R>(g, v);
+    lower<R>(g, v, vc);
     const R h = (R)1 + g1 * q[EQ_PR] / q[EQ_DE];
     const R W = (R)1 / sqrt((R)1 - vsq);
     U[EQ_DE] = q[EQ_DE] * W;
@@ -431,38 +438,67 @@ GRHD_HD void p2c(R g1, const R q[NEQ], const R g[3][3], R sqg, R U[NEQ])
     for (int e = 0; e < NEQ; e++) U[e] = U[e] * sqg;
 }
 
-// Face metric in the face arithmetic.
-template <typename R> struct FaceR {
-    R alpha, beta[3], g[3][3], sqg;
+// ------------------------------------------------------- face geometry --
+// Everything the Riemann solver reads about a face that depends on its
+// position only: the face metric (alpha, beta^i, gamma_ij, sqrt(gamma)) in the
+// face arithmetic, the HLLC orthonormal frame (coframe, gamma^dd, the frame's
+// validity and the face speed w = beta^d / (alpha sqrt(gamma^dd))), gamma^dd
+// by Cramer's rule for the GRHD wave speeds, and whether the face flux is
+// identically zero (cylindrical axis, polar faces).  The metric does not depend
+// on phi, so one evaluation serves every phi plane; it lives in LDS, one
+// column of NFG values per lane.
+enum : int {
+    FG_ALPHA = 0, FG_BETA = 1, FG_G = 4, FG_SQG = 10, FG_COF = 11, FG_GDD = 20, FG_W = 21,
+    FG_GII = 22, FG_FLAGS = 23, NFG = 24
+};
+enum : int { FG_FRAME_OK = 1, FG_ZERO = 2 };
+// gamma_ij is symmetric: six stored components 00 01 02 11 12 22.
+GRHD_HD constexpr int sym_index(int a, int b)
+{
+    return (a <= b) ? (a == 0 ? b : (a == 1 ? 2 + b : 5)) : (b == 0 ? a : (b == 1 ? 2 + a : 5));
+}
+
+template <typename R> struct FaceGeo {
+    const R *col;  // LDS column base: field f of this lane at col[f * 64]
+    GRHD_HD R get(int f) const { return col[f * 64]; }
+    GRHD_HD R alpha() const { return get(FG_ALPHA); }
+    GRHD_HD R beta(int d) const { return get(FG_BETA + d); }
+    GRHD_HD R gam(int a, int b) const { return get(FG_G + sym_index(a, b)); }
+    GRHD_HD R sqg() const { return get(FG_SQG); }
+    GRHD_HD R cof(int a, int i) const { return get(FG_COF + 3 * a + i); }
+    GRHD_HD R gdd() const { return get(FG_GDD); }
+    GRHD_HD R wface() const { return get(FG_W); }
+    GRHD_HD R gii() const { return get(FG_GII); }
+    GRHD_HD int flags() const { return (int)get(FG_FLAGS); }
 };
 
 // calc_fluxes (equations.f90:434-502), DIR = 1..3.
-template <typename R, int DIR>
-GRHD_HD void phys_flux(R g1, const R q[NEQ], const FaceR<R> &f, R F[NEQ])
+template <typename R, int DIR, typename FG>
+GRHD_HD void phys_flux(R g1, const R q[NEQ], const FG &f, R F[NEQ])
 {
     R v[3] = {q[EQ_VX], q[EQ_VY], q[EQ_VZ]}, vc[3];
-    const R vsq = vsq_clamp(f.g, v);
-    lower(f.g, v, vc);
-    const R vn = v[DIR - 1], bn = f.beta[DIR - 1];
+    const R vsq = vsq_clamp<R>(f, v);
+    lower<R>(f, v, vc);
+    const R vn = v[DIR - 1], bn = f.beta(DIR - 1);
     const R h = (R)1 + g1 * q[EQ_PR] / q[EQ_DE];
     const R W = (R)1 / sqrt((R)1 - vsq);
-    const R veff = f.alpha * vn - bn;
+    const R veff = f.alpha() * vn - bn;
     F[EQ_DE] = q[EQ_DE] * W * veff;
     F[EQ_PR] = q[EQ_DE] * h * (W * W) * veff + q[EQ_PR] * bn - F[EQ_DE];
-    F[EQ_VX] = q[EQ_DE] * h * (W * W) * vc[0] * veff + (DIR == 1 ? f.alpha * q[EQ_PR] : (R)0);
-    F[EQ_VY] = q[EQ_DE] * h * (W * W) * vc[1] * veff + (DIR == 2 ? f.alpha * q[EQ_PR] : (R)0);
-    F[EQ_VZ] = q[EQ_DE] * h * (W * W) * vc[2] * veff + (DIR == 3 ? f.alpha * q[EQ_PR] : (R)0);
-    for (int e = 0; e < NEQ; e++) F[e] = F[e] * f.sqg;
+    F[EQ_VX] = q[EQ_DE] * h * (W * W) * vc[0] * veff + (DIR == 1 ? f.alpha() * q[EQ_PR] : (R)0);
+    F[EQ_VY] = q[EQ_DE] * h * (W * W) * vc[1] * veff + (DIR == 2 ? f.alpha() * q[EQ_PR] : (R)0);
+    F[EQ_VZ] = q[EQ_DE] * h * (W * W) * vc[2] * veff + (DIR == 3 ? f.alpha() * q[EQ_PR] : (R)0);
+    for (int e = 0; e < NEQ; e++) F[e] = F[e] * f.sqg();
 }
 
 // Characteristic speeds: SRHD for Minkowski+Cartesian (fluxes.f90:669-720),
 // GRHD otherwise (fluxes.f90:724-788; the n_steps==0 diagnostic stop is not
 // reproduced).
-template <typename R, int DIR, bool SRHD>
-GRHD_HD void wave_speeds(R g1, R gam, const R q[NEQ], const FaceR<R> &f, R &wmin, R &wmax)
+template <typename R, int DIR, bool SRHD, typename FG>
+GRHD_HD void wave_speeds(R g1, R gam, const R q[NEQ], const FG &f, R &wmin, R &wmax)
 {
     R v[3] = {q[EQ_VX], q[EQ_VY], q[EQ_VZ]};
-    const R vsq = vsq_clamp(f.g, v);
+    const R vsq = vsq_clamp<R>(f, v);
     const R enth = (R)1 + g1 * q[EQ_PR] / q[EQ_DE];
     const R cs2 = gam * q[EQ_PR] / (q[EQ_DE] * enth);
     const R cs = sqrt(cs2);
@@ -475,19 +511,12 @@ GRHD_HD void wave_speeds(R g1, R gam, const R q[NEQ], const FaceR<R> &f, R &wmin
         e2 = t1 * (vn * ((R)1 - cs2) + t2);
         e3 = t1 * (vn * ((R)1 - cs2) - t2);
     } else {
-        const R(*g)[3] = f.g;
-        const R det = g[0][0] * (g[1][1] * g[2][2] - g[1][2] * g[2][1]) -
-                      g[0][1] * (g[1][0] * g[2][2] - g[1][2] * g[2][0]) +
-                      g[0][2] * (g[1][0] * g[2][1] - g[1][1] * g[2][0]);
-        R gii;
-        if (DIR == 1) gii = (g[1][1] * g[2][2] - g[1][2] * g[2][1]) / det;
-        else if (DIR == 2) gii = (g[0][0] * g[2][2] - g[0][2] * g[2][0]) / det;
-        else gii = (g[0][0] * g[1][1] - g[0][1] * g[1][0]) / det;
-        const R bn = f.beta[DIR - 1];
-        const R t1 = f.alpha / ((R)1 - vsq * cs2);
+        const R gii = f.gii();
+        const R bn = f.beta(DIR - 1);
+        const R t1 = f.alpha() / ((R)1 - vsq * cs2);
         const R t2 = cs * sqrt(rmax((R)0, ((R)1 - vsq) * (gii * ((R)1 - vsq * cs2) -
                                                         (vn * vn) * ((R)1 - cs2))));
-        e1 = f.alpha * vn - bn;
+        e1 = f.alpha() * vn - bn;
         e2 = t1 * (vn * ((R)1 - cs2) + t2) - bn;
         e3 = t1 * (vn * ((R)1 - cs2) - t2) - bn;
     }
@@ -496,14 +525,14 @@ GRHD_HD void wave_speeds(R g1, R gam, const R q[NEQ], const FaceR<R> &f, R &wmin
 }
 
 // calc_hlle_fluxes (fluxes.f90:125-156).
-template <typename R, int DIR, bool SRHD>
-GRHD_HD void hlle(R g1, R gam, const R qL[NEQ], const R qR[NEQ], const FaceR<R> &f, R F[NEQ])
+template <typename R, int DIR, bool SRHD, typename FG>
+GRHD_HD void hlle(R g1, R gam, const R qL[NEQ], const R qR[NEQ], const FG &f, R F[NEQ])
 {
     R fL[NEQ], fR[NEQ], uL[NEQ], uR[NEQ], lLmin, lLmax, lRmin, lRmax;
     phys_flux<R, DIR>(g1, qL, f, fL);
     phys_flux<R, DIR>(g1, qR, f, fR);
-    p2c<R>(g1, qL, f.g, f.sqg, uL);
-    p2c<R>(g1, qR, f.g, f.sqg, uR);
+    p2c<R>(g1, qL, f, f.sqg(), uL);
+    p2c<R>(g1, qR, f, f.sqg(), uR);
     wave_speeds<R, DIR, SRHD>(g1, gam, qL, f, lLmin, lLmax);
     wave_speeds<R, DIR, SRHD>(g1, gam, qR, f, lRmin, lRmax);
     const R ap = rmax(rmax((R)0, lLmax), lRmax);
@@ -648,9 +677,9 @@ GRHD_HD bool admissible(const R U[NEQ])
 }
 
 // local_flux_to_coordinate (fluxes.f90:596-624).
-template <typename R>
+template <typename R, typename FG>
 GRHD_HD bool local_to_coord(const R Ul[NEQ], const R Fl[NEQ], R w, R alpha, R sqg, R gdd,
-                            const R cof[3][3], R F[NEQ])
+                            const FG &fg, R F[NEQ])
 {
     R mf[NEQ];
     for (int e = 0; e < NEQ; e++) {
@@ -664,7 +693,7 @@ GRHD_HD bool local_to_coord(const R Ul[NEQ], const R Fl[NEQ], R w, R alpha, R sq
     F[EQ_DE] = fac * mf[EQ_DE];
     F[EQ_PR] = fac * mf[EQ_PR];
     for (int i = 0; i < 3; i++)
-        for (int a = 0; a < 3; a++) F[EQ_VX + i] = F[EQ_VX + i] + fac * cof[a][i] * mf[EQ_VX + a];
+        for (int a = 0; a < 3; a++) F[EQ_VX + i] = F[EQ_VX + i] + fac * fg.cof(a, i) * mf[EQ_VX + a];
     for (int e = 0; e < NEQ; e++)
         if (!finite(F[e])) return false;
     return true;
@@ -672,14 +701,15 @@ GRHD_HD bool local_to_coord(const R Ul[NEQ], const R Fl[NEQ], R w, R alpha, R sq
 
 // calc_hllc_fluxes (fluxes.f90:174-397).  Returns false where the reference
 // falls back to HLLE; the caller then evaluates HLLE.
-template <typename R, int DIR>
-GRHD_HD bool hllc_try(R g1, R gam, const R qL[NEQ], const R qR[NEQ], const FaceR<R> &f, R F[NEQ])
+template <typename R, int DIR, typename FG>
+GRHD_HD bool hllc_try(R g1, R gam, const R qL[NEQ], const R qR[NEQ], const FG &f, R F[NEQ])
 {
-    R cof[3][3], gdd, qLh[NEQ], qRh[NEQ], uL[NEQ], uR[NEQ], fL[NEQ], fR[NEQ];
+    R qLh[NEQ], qRh[NEQ], uL[NEQ], uR[NEQ], fL[NEQ], fR[NEQ];
     R lLmin, lLmax, lRmin, lRmax;
-    const R alpha = f.alpha, sqg = f.sqg;
-    if (alpha <= (R)0 || sqg <= (R)0 || !finite(alpha) || !finite(sqg)) return false;
-    if (!face_frame<R, DIR>(f.g, cof, gdd)) return false;
+    // lapse, sqrt(gamma) and the face frame were checked with the geometry
+    // (fluxes.f90:195-205, 402-473)
+    if (!(f.flags() & FG_FRAME_OK)) return false;
+    const R alpha = f.alpha(), sqg = f.sqg(), gdd = f.gdd();
     for (int e = 0; e < NEQ; e++) {
         qLh[e] = qL[e];
         qRh[e] = qR[e];
@@ -688,17 +718,17 @@ GRHD_HD bool hllc_try(R g1, R gam, const R qL[NEQ], const R qR[NEQ], const FaceR
         qLh[EQ_VX + a] = 0;
         qRh[EQ_VX + a] = 0;
         for (int i = 0; i < 3; i++) {
-            qLh[EQ_VX + a] = qLh[EQ_VX + a] + cof[a][i] * qL[EQ_VX + i];
-            qRh[EQ_VX + a] = qRh[EQ_VX + a] + cof[a][i] * qR[EQ_VX + i];
+            qLh[EQ_VX + a] = qLh[EQ_VX + a] + f.cof(a, i) * qL[EQ_VX + i];
+            qRh[EQ_VX + a] = qRh[EQ_VX + a] + f.cof(a, i) * qR[EQ_VX + i];
         }
     }
     if (!srhd_local(g1, gam, qLh, uL, fL, lLmin, lLmax)) return false;
     if (!srhd_local(g1, gam, qRh, uR, fR, lRmin, lRmax)) return false;
     const R ll = rmin(lLmin, lRmin), lr = rmax(lLmax, lRmax);
-    const R w = f.beta[DIR - 1] / (alpha * sqrt(gdd));
+    const R w = f.wface();
     if (!finite(w) || lr - ll <= (R)1.0e-13) return false;
-    if (w <= ll) return local_to_coord(uL, fL, w, alpha, sqg, gdd, cof, F);
-    if (w >= lr) return local_to_coord(uR, fR, w, alpha, sqg, gdd, cof, F);
+    if (w <= ll) return local_to_coord(uL, fL, w, alpha, sqg, gdd, f, F);
+    if (w >= lr) return local_to_coord(uR, fR, w, alpha, sqg, gdd, f, F);
     R uh[NEQ], fh[NEQ];
     R den = lr - ll;
     for (int e = 0; e < NEQ; e++) {
@@ -776,13 +806,13 @@ GRHD_HD bool hllc_try(R g1, R gam, const R qL[NEQ], const R qR[NEQ], const FaceR
     if (!admissible(usL) || !admissible(usR)) return false;
     for (int e = 0; e < NEQ; e++)
         if (!finite(fsL[e]) || !finite(fsR[e])) return false;
-    if (w <= ls) return local_to_coord(usL, fsL, w, alpha, sqg, gdd, cof, F);
-    return local_to_coord(usR, fsR, w, alpha, sqg, gdd, cof, F);
+    if (w <= ls) return local_to_coord(usL, fsL, w, alpha, sqg, gdd, f, F);
+    return local_to_coord(usR, fsR, w, alpha, sqg, gdd, f, F);
 }
 
 // resolve_riemann_problem (fluxes.f90:56-72).
-template <typename R, int DIR, bool SRHD>
-GRHD_HD void riemann(int rs, R g1, R gam, const R qL[NEQ], const R qR[NEQ], const FaceR<R> &f,
+template <typename R, int DIR, bool SRHD, typename FG>
+GRHD_HD void riemann(int rs, R g1, R gam, const R qL[NEQ], const R qR[NEQ], const FG &f,
                      R F[NEQ])
 {
     if (rs == 2) {
@@ -1160,7 +1190,7 @@ GRHD_HD bool c2p(double g1, double rho_floor, double p_floor, const double g[3][
 GRHD_HD void sources(double g1, const CentreMetric &m, const double q[NEQ], double src[NEQ])
 {
     double v[3] = {q[EQ_VX], q[EQ_VY], q[EQ_VZ]}, u4[4], T[4][4];
-    const double vsq = vsq_clamp<double>(m.g, v);
+    const double vsq = vsq_clamp<double>(Mat3<double>{m.g}, v);
     const double h = 1.0 + g1 * q[EQ_PR] / q[EQ_DE];
     const double W = 1.0 / sqrt(1.0 - vsq);
     u4[0] = W / m.alpha;

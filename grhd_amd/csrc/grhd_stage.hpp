@@ -24,10 +24,61 @@ __device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned nb)
 
 // -------------------------------------------------------------- fluxes --
 // Face arrays have shape [5][nfz][nfy][nfx] with nf_d = n_d + (d == DIR).
-// Lanes of a wave are consecutive in i, so every load and store is coalesced.
+
+// The position-only part of face (i, j) into the LDS column col (field f at
+// col[f * 64]): face metric in the face arithmetic (as calc_rhs passes it to
+// the Riemann solver, evolution.f90:74-77, 110-114, 158-162), gamma^dd for
+// the GRHD wave speeds (fluxes.f90:755-770), the HLLC frame and face speed
+// (fluxes.f90:195-205, 402-473), and the zero-flux faces on the cylindrical
+// axis and at the poles (evolution.f90:69-71, 122-124).
+template <int MK, int DIR, typename R>
+__device__ __forceinline__ void face_geometry(const Params &P, int i, int j, R *col)
+{
+    const double eps64 = 64.0 * DBL_EPSILON;
+    bool zero = false;
+    if (DIR == 1 && P.cyl) zero = fabs(xf(P, i)) <= eps64;
+    if (DIR == 2 && P.polar) zero = fabs(sin(yf(P, j))) <= eps64;
+    int flags = zero ? FG_ZERO : 0;
+    if (!zero) {
+        FaceMetric m;
+        if (DIR == 1) metric_face<MK>(P, xf(P, i), yc(P, j), m);
+        else if (DIR == 2) metric_face<MK>(P, xc(P, i), yf(P, j), m);
+        else metric_face<MK>(P, xc(P, i), yc(P, j), m);  // z faces: centre metric
+        R g[3][3], beta[3];
+        const R alpha = (R)m.alpha, sqg = (R)sqrt(m.det);
+        for (int x = 0; x < 3; x++) {
+            beta[x] = (R)m.beta[x];
+            for (int y = 0; y < 3; y++) g[x][y] = (R)m.g[x][y];
+        }
+        const R det = g[0][0] * (g[1][1] * g[2][2] - g[1][2] * g[2][1]) -
+                      g[0][1] * (g[1][0] * g[2][2] - g[1][2] * g[2][0]) +
+                      g[0][2] * (g[1][0] * g[2][1] - g[1][1] * g[2][0]);
+        R gii;
+        if (DIR == 1) gii = (g[1][1] * g[2][2] - g[1][2] * g[2][1]) / det;
+        else if (DIR == 2) gii = (g[0][0] * g[2][2] - g[0][2] * g[2][0]) / det;
+        else gii = (g[0][0] * g[1][1] - g[0][1] * g[1][0]) / det;
+        R cof[3][3], gdd = 0;
+        const bool ok = !(alpha <= (R)0 || sqg <= (R)0 || !finite(alpha) || !finite(sqg)) &&
+                        face_frame<R, DIR>(g, cof, gdd);
+        const R w = ok ? beta[DIR - 1] / (alpha * sqrt(gdd)) : (R)0;
+        if (ok) flags |= FG_FRAME_OK;
+        col[FG_ALPHA * 64] = alpha;
+        for (int x = 0; x < 3; x++) col[(FG_BETA + x) * 64] = beta[x];
+        for (int x = 0; x < 3; x++)
+            for (int y = x; y < 3; y++) col[(FG_G + sym_index(x, y)) * 64] = g[x][y];
+        col[FG_SQG * 64] = sqg;
+        for (int x = 0; x < 3; x++)
+            for (int y = 0; y < 3; y++) col[(FG_COF + 3 * x + y) * 64] = ok ? cof[x][y] : (R)0;
+        col[FG_GDD * 64] = gdd;
+        col[FG_W * 64] = w;
+        col[FG_GII * 64] = gii;
+    }
+    col[FG_FLAGS * 64] = (R)flags;
+}
+
 template <int MK, int DIR, typename R>
 __device__ __forceinline__ void flux_face(const Params &P, const double *__restrict__ p,
-                                          const FaceR<R> &f, int i, int j, int k, int fi, int nodes,
+                                          const FaceGeo<R> &f, int i, int j, int k, int fi, int nodes,
                                           R out[NEQ])
 {
     R s[NEQ][6];
@@ -45,47 +96,81 @@ __device__ __forceinline__ void flux_face(const Params &P, const double *__restr
     riemann<R, DIR, kSRHD<MK>>(P.rs, (R)P.g1, (R)P.gam, qL, qR, f, out);
 }
 
-template <int MK, int DIR, typename R>
-__global__ __launch_bounds__(256) void k_flux(Params P, const double *__restrict__ p,
-                                              R *__restrict__ F, int64_t fstride)
+// Tiling of the face kernels: a tile is 64 consecutive faces of one (x, y)
+// plane (flattened, so rows need not be multiples of 64) times kt planes along
+// z.  The face geometry depends on (x, y) only, so a group of gw = kt waves
+// shares one LDS copy of it: the group's first wave evaluates it and each wave
+// of the group takes one plane.  A 256-thread block holds 4 / gw groups; grids
+// with fewer than 4 planes use groups of one wave.
+struct FluxTiling {
+    int kt, gw;
+};
+static inline FluxTiling flux_tiling(int nfz)
 {
-    // one face per thread: the face kernels are limited by their register
-    // footprint (Riemann solver state), so they keep the shortest live ranges
+    if (nfz >= 4) return {4, 4};
+    return {1, 1};
+}
+
+// Four waves per SIMD: the compiler keeps the face kernels at 128 VGPRs with a
+// few doubles of scratch spill, measured faster than 151 VGPRs at three waves.
+template <int MK, int DIR, typename R>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_flux(Params P, const double *__restrict__ p,
+                                              R *__restrict__ F, int64_t fstride, int kt, int gw)
+{
+    extern __shared__ double lds_geo[];
     const int nfx = P.nx + (DIR == 1), nfy = P.ny + (DIR == 2), nfz = P.nz + (DIR == 3);
-    const int64_t t = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    if (t >= (int64_t)nfx * nfy * nfz) return;
-    const int a = (int)(t % nfx);
-    const int64_t rem = t / nfx;
-    const int b = (int)(rem % nfy), c = (int)(rem / nfy);
+    const int64_t nab = (int64_t)nfx * nfy;
+    const int64_t nchunk = (nab + 63) / 64;
+    const int groups = 4 / gw;
+    const int64_t ncg = (nchunk + groups - 1) / groups;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int grp = wave / gw, wig = wave % gw;
+    const unsigned blk = xcd_block(blockIdx.x, gridDim.x);
+    const int64_t chunk = (int64_t)(blk % ncg) * groups + grp;
+    const int ctile = (int)(blk / ncg);
+    const int64_t ab = chunk * 64 + lane;
+    const bool active = ab < nab;
+    const int a = active ? (int)(ab % nfx) : 0, b = active ? (int)(ab / nfx) : 0;
     // reference indices: faces 0..n along DIR, cells 1..n across
-    const int i = a + (DIR != 1), j = b + (DIR != 2), k = c + (DIR != 3);
-    const int fi = (DIR == 1) ? i : (DIR == 2) ? j : k;
+    const int i = a + (DIR != 1), j = b + (DIR != 2);
+    R *col = reinterpret_cast<R *>(lds_geo) + grp * NFG * 64 + lane;
+    if (wig == 0 && active) face_geometry<MK, DIR, R>(P, i, j, col);
+    __syncthreads();
+    if (!active) return;
+    const FaceGeo<R> fg{col};
     const int nodes = (DIR == 1) ? P.nx : (DIR == 2) ? P.ny : P.nz;
-    const double eps64 = 64.0 * DBL_EPSILON;
-    bool zero = false;
-    if (DIR == 1 && P.cyl) zero = fabs(xf(P, i)) <= eps64;           // evolution.f90:69-71
-    if (DIR == 2 && P.polar) zero = fabs(sin(yf(P, j))) <= eps64;    // evolution.f90:122-124
-    R out[NEQ];
-    if (zero) {
+    const bool zero = (fg.flags() & FG_ZERO) != 0;
+    // one plane per wave: a loop over planes would keep the geometry loads
+    // hoisted in registers (more VGPRs, fewer waves)
+    const int c = ctile * kt + wig;
+    if (c >= nfz) return;
+    {
+        const int k = c + (DIR != 3);
+        const int fi = (DIR == 1) ? i : (DIR == 2) ? j : k;
+        R out[NEQ];
+        if (zero) {
 #pragma unroll
-        for (int v = 0; v < NEQ; v++) out[v] = 0;
-    } else {
-        FaceMetric m;
-        if (DIR == 1) metric_face<MK>(P, xf(P, i), yc(P, j), m);
-        else if (DIR == 2) metric_face<MK>(P, xc(P, i), yf(P, j), m);
-        else metric_face<MK>(P, xc(P, i), yc(P, j), m);  // z faces: centre metric
-        FaceR<R> f;
-        f.alpha = (R)m.alpha;
-        f.sqg = (R)sqrt(m.det);
-        for (int x = 0; x < 3; x++) {
-            f.beta[x] = (R)m.beta[x];
-            for (int y = 0; y < 3; y++) f.g[x][y] = (R)m.g[x][y];
+            for (int v = 0; v < NEQ; v++) out[v] = 0;
+        } else {
+            flux_face<MK, DIR, R>(P, p, fg, i, j, k, fi, nodes, out);
         }
-        flux_face<MK, DIR, R>(P, p, f, i, j, k, fi, nodes, out);
-    }
-    const int64_t fo = ((int64_t)c * nfy + b) * nfx + a;
+        const int64_t fo = (int64_t)c * nab + ab;
 #pragma unroll
-    for (int v = 0; v < NEQ; v++) F[v * fstride + fo] = out[v];
+        for (int v = 0; v < NEQ; v++) F[v * fstride + fo] = out[v];
+    }
+}
+
+template <int MK, int DIR, typename R>
+void launch_flux(const Params &P, hipStream_t s, const double *p, R *F, int64_t fstr)
+{
+    const int nfx = P.nx + (DIR == 1), nfy = P.ny + (DIR == 2), nfz = P.nz + (DIR == 3);
+    const FluxTiling t = flux_tiling(nfz);
+    const int groups = 4 / t.gw;
+    const int64_t nchunk = ((int64_t)nfx * nfy + 63) / 64;
+    const int64_t ncg = (nchunk + groups - 1) / groups;
+    const int64_t nct = (nfz + t.kt - 1) / t.kt;
+    const size_t lds = (size_t)groups * NFG * 64 * sizeof(R);
+    k_flux<MK, DIR, R><<<(unsigned)(ncg * nct), 256, lds, s>>>(P, p, F, fstr, t.kt, t.gw);
 }
 
 // --------------------------------------------------------- cell update --
@@ -210,15 +295,12 @@ void launch_stage_mk(const Params &P, hipStream_t s, double *p, void *const F[3]
                      const double *u0, double *u, int stage, double dt, unsigned long long *counters)
 {
     const int bs = 256;
-    const int64_t nfx = (int64_t)(P.nx + 1) * P.ny * P.nz;
-    const int64_t nfy = (int64_t)P.nx * (P.ny + 1) * P.nz;
-    const int64_t nfz = (int64_t)P.nx * P.ny * (P.nz + 1);
     constexpr int RUN = 1;  // one cell per thread: 4 waves/SIMD beat metric reuse along phi
     const int64_t ncol = (int64_t)P.nx * P.ny * ((P.nz + RUN - 1) / RUN);
     R *Fx = (R *)F[0], *Fy = (R *)F[1], *Fz = (R *)F[2];
-    k_flux<MK, 1, R><<<blocks_for(nfx, bs), bs, 0, s>>>(P, p, Fx, fstr[0]);
-    if (P.ny > 1) k_flux<MK, 2, R><<<blocks_for(nfy, bs), bs, 0, s>>>(P, p, Fy, fstr[1]);
-    if (P.nz > 1) k_flux<MK, 3, R><<<blocks_for(nfz, bs), bs, 0, s>>>(P, p, Fz, fstr[2]);
+    launch_flux<MK, 1, R>(P, s, p, Fx, fstr[0]);
+    if (P.ny > 1) launch_flux<MK, 2, R>(P, s, p, Fy, fstr[1]);
+    if (P.nz > 1) launch_flux<MK, 3, R>(P, s, p, Fz, fstr[2]);
     k_update<MK, R, RUN><<<blocks_for(ncol, bs), bs, 0, s>>>(P, p, Fx, Fy, Fz, fstr[0], fstr[1], fstr[2], u0,
                                                         u, stage, dt, counters);
     if (stage > 0) {
