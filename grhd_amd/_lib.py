@@ -111,6 +111,20 @@ _SIGS = {
     "grhd_synchronize": ([ctypes.c_void_p], ctypes.c_int),
     "grhd_last_timing": ([ctypes.c_void_p, ctypes.POINTER(_D), ctypes.POINTER(ctypes.c_int64)],
                          ctypes.c_int),
+    "grhd_footprint": ([ctypes.POINTER(Config), ctypes.POINTER(ctypes.c_uint64)], ctypes.c_int),
+    "grhd_batch_create": ([_I, ctypes.c_uint64, ctypes.POINTER(ctypes.c_void_p)], ctypes.c_int),
+    "grhd_batch_destroy": ([ctypes.c_void_p], ctypes.c_int),
+    "grhd_batch_last_error": ([ctypes.c_void_p], ctypes.c_char_p),
+    "grhd_batch_add": ([ctypes.c_void_p, ctypes.POINTER(Config), ctypes.POINTER(_I)], ctypes.c_int),
+    "grhd_batch_problem": ([ctypes.c_void_p, _I, ctypes.POINTER(ctypes.c_void_p)], ctypes.c_int),
+    "grhd_batch_remove": ([ctypes.c_void_p, _I], ctypes.c_int),
+    "grhd_batch_step": ([ctypes.c_void_p, _I], ctypes.c_int),
+    "grhd_batch_status": ([ctypes.c_void_p, _I, ctypes.POINTER(_I), ctypes.POINTER(_D),
+                           ctypes.POINTER(ctypes.c_int64)], ctypes.c_int),
+    "grhd_batch_set_variant": ([ctypes.c_void_p, _I], ctypes.c_int),
+    "grhd_batch_info": ([ctypes.c_void_p, ctypes.POINTER(_I), ctypes.POINTER(_I),
+                         ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(_D),
+                         ctypes.POINTER(ctypes.c_int64)], ctypes.c_int),
 }
 
 

@@ -14,11 +14,12 @@ PKG = Path(__file__).resolve().parent
 ROOT = PKG.parent
 CSRC = PKG / "csrc"
 LIB = PKG / "libgrhd_hip.so"
-SOURCES = ["grhd_kernels.hip", "grhd_kernels_mixed.hip", "grhd_initial.hip", "grhd_capi.hip"]
+SOURCES = ["grhd_kernels.hip", "grhd_kernels_mixed.hip", "grhd_batch_f64.hip", "grhd_batch_mixed.hip",
+           "grhd_initial.hip", "grhd_capi.hip", "grhd_batch.hip"]
 # The mixed-precision stage is judged by its error against the FP64 path, so
 # it may contract multiply-adds; every other unit keeps the reference's
 # rounding sequence.
-EXTRA = {"grhd_kernels_mixed.hip": ["-ffp-contract=fast"]}
+EXTRA = {"grhd_kernels_mixed.hip": ["-ffp-contract=fast"], "grhd_batch_mixed.hip": ["-ffp-contract=fast"]}
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("GRHD_OFFLOAD_ARCH", "gfx950")
@@ -38,7 +39,7 @@ FLAGS = [
 ]
 
 
-def build(verbose: bool = True, jobs: int = 4) -> Path:
+def build(verbose: bool = True, jobs: int = 7) -> Path:
     objs = []
     procs = []
     for src in SOURCES:

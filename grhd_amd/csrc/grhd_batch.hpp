@@ -1,0 +1,53 @@
+// grhd_batch.hpp -- descriptors shared by the batch host layer (grhd_batch.hip)
+// and the batched stage kernels (grhd_batch_kernels.hpp).
+#pragma once
+#include "grhd_physics.hpp"
+
+namespace grhd {
+
+// One problem of a batch, as the batched kernels read it (device copy).
+struct RunDesc {
+    Params P;
+    double *p;                // ghosted primitives
+    double *u[2];             // the two conserved buffers
+    void *F[3];               // face fluxes (FP64 or FP32)
+    int64_t fstr[3];
+    const double *injector;
+    unsigned long long *counters;
+    int ucur;                 // u[ucur] holds U^n at the first step of the launch chunk
+    int pad_;
+};
+
+// Phases of one RK stage and their logical-block lists.
+enum : int { PH_FX = 0, PH_FY = 1, PH_FZ = 2, PH_CELL = 3, PH_BOUND = 4, NPH = 5 };
+
+struct BatchTables {
+    const RunDesc *runs;      // indexed by slot
+    const int *slot;          // [n] slots of the group's problems
+    const int64_t *begin;     // [NPH][n + 1] first logical block of each problem
+    int n;
+    int gw;                   // waves sharing one face-geometry column (1, 2 or 4)
+    int lds_groups;           // face-geometry groups per block the LDS holds
+    int pad_;
+    int64_t total[NPH];       // logical blocks per phase
+};
+
+struct BatchLaunch {
+    BatchTables tables;
+    const double *dts;        // dt of every slot at this step (0: skip the problem)
+    int sstep;                // step index within the launch chunk
+    int stage;                // RK stage 1..3
+    int mk;
+    bool persistent;
+    unsigned grid;            // persistent grid (co-resident blocks)
+    unsigned *bar;
+    unsigned *err;
+    uint64_t limit_ticks;
+};
+
+void batch_stage_double(const BatchLaunch &B, hipStream_t s);
+void batch_stage_float(const BatchLaunch &B, hipStream_t s);
+int batch_occupancy_double(int mk, size_t lds);
+int batch_occupancy_float(int mk, size_t lds);
+
+}  // namespace grhd

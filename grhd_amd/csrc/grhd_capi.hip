@@ -13,34 +13,12 @@
 #include <vector>
 
 #include "../../include/grhd_c.h"
+#include "grhd_ctx.hpp"
 #include "grhd_initial.hpp"
 #include "grhd_kernels.hpp"
 
 using namespace grhd;
 
-struct grhd_ctx {
-    grhd_config cfg;
-    Params P;
-    hipStream_t stream = nullptr;
-    bool own_stream = false;
-    bool mixed = false;
-    double *p = nullptr;      // ghosted primitives, SoA [5][NZs][NYs][NXs]
-    double *u[2] = {nullptr, nullptr};  // conserved, SoA [5][nz][ny][nx]
-    int ucur = 0;             // u[ucur] holds U^n
-    void *F[3] = {nullptr, nullptr, nullptr};
-    int64_t fstr[3] = {0, 0, 0};
-    double *injector = nullptr;
-    double *partial = nullptr;
-    int partial_n = 0;
-    unsigned long long *counters = nullptr;
-    int64_t steps = 0;
-    double dt_cfl = 0.0;
-    bool dt_ready = false;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    double last_ms = 0.0;
-    int64_t last_launches = 0;
-    std::string err;
-};
 
 namespace {
 
@@ -345,7 +323,35 @@ int run_steps(grhd_ctx *c, const double *dts, int n)
 
 }  // namespace
 
+namespace grhd {
+uint64_t ctx_footprint(const grhd_config &cfg)
+{
+    grhd_config c = cfg;
+    if (c.metric == GRHD_METRIC_KERR_SCHILD) c.geometry = GRHD_GEOM_SPHEROIDAL;
+    const Params P = make_params(c);
+    const uint64_t ncell = (uint64_t)P.nx * P.ny * P.nz;
+    const uint64_t rsz = (cfg.precision == GRHD_PRECISION_MIXED) ? sizeof(float) : sizeof(double);
+    const uint64_t nf[3] = {(uint64_t)(P.nx + 1) * P.ny * P.nz,
+                            P.ny > 1 ? (uint64_t)P.nx * (P.ny + 1) * P.nz : 1,
+                            P.nz > 1 ? (uint64_t)P.nx * P.ny * (P.nz + 1) : 1};
+    uint64_t b = (uint64_t)P.vstride * NEQ * sizeof(double) + 2 * ncell * NEQ * sizeof(double);
+    for (int d = 0; d < 3; d++) b += nf[d] * (d == 0 || nf[d] > 1 ? NEQ : 1) * rsz;
+    b += (uint64_t)NEQ * P.ng * P.ny * P.nz * sizeof(double) + 8;
+    b += (((uint64_t)P.nx * P.ny + 255) / 256) * 4 * sizeof(double) + 4 * sizeof(unsigned long long);
+    return b;
+}
+}  // namespace grhd
+
 extern "C" {
+
+int grhd_footprint(const grhd_config *cfg, uint64_t *bytes)
+{
+    if (!cfg || !bytes) return fail(nullptr, GRHD_ERR_INVALID_ARGUMENT, "null argument");
+    std::string msg;
+    if (validate(*cfg, msg)) return fail(nullptr, GRHD_ERR_INVALID_CONFIG, msg);
+    *bytes = ctx_footprint(*cfg);
+    return GRHD_OK;
+}
 
 const char *grhd_last_error(const grhd_ctx *ctx)
 {

@@ -1,0 +1,38 @@
+// grhd_ctx.hpp -- the state of one problem on the device (opaque grhd_ctx of
+// the C ABI), shared by the single-problem entry points and the batch layer.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "../../include/grhd_c.h"
+#include "grhd_physics.hpp"
+
+struct grhd_ctx {
+    grhd_config cfg;
+    grhd::Params P;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    bool mixed = false;
+    double *p = nullptr;      // ghosted primitives, SoA [5][NZs][NYs][NXs]
+    double *u[2] = {nullptr, nullptr};  // conserved, SoA [5][nz][ny][nx]
+    int ucur = 0;             // u[ucur] holds U^n
+    void *F[3] = {nullptr, nullptr, nullptr};
+    int64_t fstr[3] = {0, 0, 0};
+    double *injector = nullptr;
+    double *partial = nullptr;
+    int partial_n = 0;
+    unsigned long long *counters = nullptr;
+    int64_t steps = 0;
+    double dt_cfl = 0.0;
+    bool dt_ready = false;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double last_ms = 0.0;
+    int64_t last_launches = 0;
+    std::string err;
+};
+
+namespace grhd {
+// Device bytes grhd_create allocates for cfg (admission by HBM footprint).
+uint64_t ctx_footprint(const grhd_config &cfg);
+}  // namespace grhd

@@ -105,19 +105,19 @@ __device__ __forceinline__ void flux_face(const Params &P, const double *__restr
 struct FluxTiling {
     int kt, gw;
 };
-static inline FluxTiling flux_tiling(int nfz)
+__host__ __device__ inline FluxTiling flux_tiling(int nfz, int gw_pref = 4)
 {
-    if (nfz >= 4) return {4, 4};
+    if (nfz >= gw_pref) return {gw_pref, gw_pref};
     return {1, 1};
 }
 
-// Four waves per SIMD: the compiler keeps the face kernels at 128 VGPRs with a
-// few doubles of scratch spill, measured faster than 151 VGPRs at three waves.
+// Faces of logical block blk of one problem (0 <= blk < flux_blocks()).
+// lds holds (4 / gw) * NFG * 64 values of R.
 template <int MK, int DIR, typename R>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_flux(Params P, const double *__restrict__ p,
-                                              R *__restrict__ F, int64_t fstride, int kt, int gw)
+__device__ __forceinline__ void flux_body(const Params &P, const double *__restrict__ p,
+                                          R *__restrict__ F, int64_t fstride, int64_t blk, int kt,
+                                          int gw, R *lds)
 {
-    extern __shared__ double lds_geo[];
     const int nfx = P.nx + (DIR == 1), nfy = P.ny + (DIR == 2), nfz = P.nz + (DIR == 3);
     const int64_t nab = (int64_t)nfx * nfy;
     const int64_t nchunk = (nab + 63) / 64;
@@ -125,30 +125,26 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
     const int64_t ncg = (nchunk + groups - 1) / groups;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int grp = wave / gw, wig = wave % gw;
-    const unsigned blk = xcd_block(blockIdx.x, gridDim.x);
-    const int64_t chunk = (int64_t)(blk % ncg) * groups + grp;
+    const int64_t chunk = (blk % ncg) * groups + grp;
     const int ctile = (int)(blk / ncg);
     const int64_t ab = chunk * 64 + lane;
     const bool active = ab < nab;
     const int a = active ? (int)(ab % nfx) : 0, b = active ? (int)(ab / nfx) : 0;
     // reference indices: faces 0..n along DIR, cells 1..n across
     const int i = a + (DIR != 1), j = b + (DIR != 2);
-    R *col = reinterpret_cast<R *>(lds_geo) + grp * NFG * 64 + lane;
+    R *col = lds + grp * NFG * 64 + lane;
     if (wig == 0 && active) face_geometry<MK, DIR, R>(P, i, j, col);
     __syncthreads();
-    if (!active) return;
-    const FaceGeo<R> fg{col};
-    const int nodes = (DIR == 1) ? P.nx : (DIR == 2) ? P.ny : P.nz;
-    const bool zero = (fg.flags() & FG_ZERO) != 0;
     // one plane per wave: a loop over planes would keep the geometry loads
     // hoisted in registers (more VGPRs, fewer waves)
     const int c = ctile * kt + wig;
-    if (c >= nfz) return;
-    {
+    if (active && c < nfz) {
+        const FaceGeo<R> fg{col};
+        const int nodes = (DIR == 1) ? P.nx : (DIR == 2) ? P.ny : P.nz;
         const int k = c + (DIR != 3);
         const int fi = (DIR == 1) ? i : (DIR == 2) ? j : k;
         R out[NEQ];
-        if (zero) {
+        if (fg.flags() & FG_ZERO) {
 #pragma unroll
             for (int v = 0; v < NEQ; v++) out[v] = 0;
         } else {
@@ -158,108 +154,124 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
 #pragma unroll
         for (int v = 0; v < NEQ; v++) F[v * fstride + fo] = out[v];
     }
+    __syncthreads();  // the LDS columns are reused by the block's next tile
+}
+
+template <int DIR>
+inline int64_t flux_blocks(const Params &P, FluxTiling t)
+{
+    const int nfx = P.nx + (DIR == 1), nfy = P.ny + (DIR == 2), nfz = P.nz + (DIR == 3);
+    const int groups = 4 / t.gw;
+    const int64_t nchunk = ((int64_t)nfx * nfy + 63) / 64;
+    return ((nchunk + groups - 1) / groups) * ((nfz + t.kt - 1) / t.kt);
+}
+
+// Four waves per SIMD: the compiler keeps the face kernels at 128 VGPRs with a
+// few doubles of scratch spill, measured faster than 151 VGPRs at three waves.
+template <int MK, int DIR, typename R>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_flux(
+    Params P, const double *__restrict__ p, R *__restrict__ F, int64_t fstride, int kt, int gw)
+{
+    extern __shared__ double lds_geo[];
+    flux_body<MK, DIR, R>(P, p, F, fstride, xcd_block(blockIdx.x, gridDim.x), kt, gw,
+                          reinterpret_cast<R *>(lds_geo));
 }
 
 template <int MK, int DIR, typename R>
 void launch_flux(const Params &P, hipStream_t s, const double *p, R *F, int64_t fstr)
 {
-    const int nfx = P.nx + (DIR == 1), nfy = P.ny + (DIR == 2), nfz = P.nz + (DIR == 3);
+    const int nfz = P.nz + (DIR == 3);
     const FluxTiling t = flux_tiling(nfz);
-    const int groups = 4 / t.gw;
-    const int64_t nchunk = ((int64_t)nfx * nfy + 63) / 64;
-    const int64_t ncg = (nchunk + groups - 1) / groups;
-    const int64_t nct = (nfz + t.kt - 1) / t.kt;
-    const size_t lds = (size_t)groups * NFG * 64 * sizeof(R);
-    k_flux<MK, DIR, R><<<(unsigned)(ncg * nct), 256, lds, s>>>(P, p, F, fstr, t.kt, t.gw);
+    const size_t lds = (size_t)(4 / t.gw) * NFG * 64 * sizeof(R);
+    k_flux<MK, DIR, R><<<(unsigned)flux_blocks<DIR>(P, t), 256, lds, s>>>(P, p, F, fstr, t.kt, t.gw);
 }
 
 // --------------------------------------------------------- cell update --
-// stage 1..3: rhs (evolution.f90:85-88,137-139,180-182) and the RK
-// combination (grhd2.f90:243-249) into u; cons_to_prim follows in
-// k_c2p_stage.  stage 0 writes the rhs itself to u (parity entry point).
-// The metric depends on (r, theta) only; RUN > 1 lets a thread reuse its
-// centre metric over RUN consecutive phi cells (fewer metric evaluations, but
-// twice the registers: measured slower, so the launcher uses RUN = 1).
-template <int MK, typename R, int RUN>
-__global__ __launch_bounds__(256) void k_update(Params P, double *__restrict__ p,
-                                                const R *__restrict__ Fx, const R *__restrict__ Fy,
-                                                const R *__restrict__ Fz, int64_t fsx, int64_t fsy,
-                                                int64_t fsz, const double *__restrict__ u0,
-                                                double *__restrict__ u, int stage, double dt,
-                                                unsigned long long *__restrict__ counters)
+// Cell t (0 <= t < nx ny nz, x fastest) of one problem.  stage 1..3: rhs
+// (evolution.f90:85-88, 137-139, 180-182) and the RK combination
+// (grhd2.f90:243-249) into u; cons_to_prim follows in c2p_body.  stage 0
+// writes the rhs itself to u (parity entry point).
+template <int MK, typename R>
+__device__ __forceinline__ void update_body(const Params &P, const double *__restrict__ p,
+                                            const R *__restrict__ Fx, const R *__restrict__ Fy,
+                                            const R *__restrict__ Fz, int64_t fsx, int64_t fsy,
+                                            int64_t fsz, const double *__restrict__ u0,
+                                            double *__restrict__ u, int stage, double dt, int64_t t)
 {
-    const int nchunk = (P.nz + RUN - 1) / RUN;
-    const int64_t t = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    if (t >= (int64_t)P.nx * P.ny * nchunk) return;
     const int64_t ncell = (int64_t)P.nx * P.ny * P.nz;
     const int i0 = (int)(t % P.nx);
     const int64_t rem = t / P.nx;
-    const int j0 = (int)(rem % P.ny), chunk = (int)(rem / P.ny);
-    const int i = i0 + 1, j = j0 + 1;
-    CentreMetric m;
-    metric_centre<MK>(P, xc(P, i), yc(P, j), m);
-    const int k0b = chunk * RUN, k0e = min(k0b + RUN, P.nz);
-    for (int k0 = k0b; k0 < k0e; k0++) {
-        const int k = k0 + 1;
-        const int64_t tc = ((int64_t)k0 * P.ny + j0) * P.nx + i0;
-        const int64_t pc = pindex(P, i, j, k);
-        double q[NEQ], rhs[NEQ], src[NEQ];
+    const int j0 = (int)(rem % P.ny), k0 = (int)(rem / P.ny);
+    const int i = i0 + 1, j = j0 + 1, k = k0 + 1;
+    const int64_t pc = pindex(P, i, j, k);
+    double q[NEQ], rhs[NEQ], src[NEQ];
 #pragma unroll
-        for (int v = 0; v < NEQ; v++) q[v] = p[v * P.vstride + pc];
-        if (MK != MK_MINK_CART) {
-            sources(P.g1, m, q, src);
-        } else {
+    for (int v = 0; v < NEQ; v++) q[v] = p[v * P.vstride + pc];
+    if (MK != MK_MINK_CART) {
+        CentreMetric m;
+        metric_centre<MK>(P, xc(P, i), yc(P, j), m);
+        sources(P.g1, m, q, src);
+    } else {
 #pragma unroll
-            for (int v = 0; v < NEQ; v++) src[v] = 0.0;
-        }
-        {
-            const int64_t fa = ((int64_t)k0 * P.ny + j0) * (P.nx + 1) + i0;
-#pragma unroll
-            for (int v = 0; v < NEQ; v++)
-                rhs[v] = -((double)Fx[v * fsx + fa + 1] - (double)Fx[v * fsx + fa]) / P.dx + src[v];
-        }
-        if (P.ny > 1) {
-            const int64_t fa = ((int64_t)k0 * (P.ny + 1) + j0) * P.nx + i0;
-#pragma unroll
-            for (int v = 0; v < NEQ; v++)
-                rhs[v] = rhs[v] - ((double)Fy[v * fsy + fa + P.nx] - (double)Fy[v * fsy + fa]) / P.dy;
-        }
-        if (P.nz > 1) {
-            const int64_t plane = (int64_t)P.nx * P.ny;
-#pragma unroll
-            for (int v = 0; v < NEQ; v++)
-                rhs[v] = rhs[v] - ((double)Fz[v * fsz + tc + plane] - (double)Fz[v * fsz + tc]) / P.dz;
-        }
-        if (stage == 0) {
-#pragma unroll
-            for (int v = 0; v < NEQ; v++) u[v * ncell + tc] = rhs[v];
-            continue;
-        }
-        double c[NEQ];
-#pragma unroll
-        for (int v = 0; v < NEQ; v++) {
-            const double a0 = u0[v * ncell + tc];
-            if (stage == 1) c[v] = a0 + rhs[v] * dt;
-            else if (stage == 2) c[v] = 0.75 * a0 + 0.25 * (u[v * ncell + tc] + rhs[v] * dt);
-            else c[v] = (a0 + 2.0 * (u[v * ncell + tc] + rhs[v] * dt)) / 3.0;
-        }
-#pragma unroll
-        for (int v = 0; v < NEQ; v++) u[v * ncell + tc] = c[v];
+        for (int v = 0; v < NEQ; v++) src[v] = 0.0;
     }
+    {
+        const int64_t fa = ((int64_t)k0 * P.ny + j0) * (P.nx + 1) + i0;
+#pragma unroll
+        for (int v = 0; v < NEQ; v++)
+            rhs[v] = -((double)Fx[v * fsx + fa + 1] - (double)Fx[v * fsx + fa]) / P.dx + src[v];
+    }
+    if (P.ny > 1) {
+        const int64_t fa = ((int64_t)k0 * (P.ny + 1) + j0) * P.nx + i0;
+#pragma unroll
+        for (int v = 0; v < NEQ; v++)
+            rhs[v] = rhs[v] - ((double)Fy[v * fsy + fa + P.nx] - (double)Fy[v * fsy + fa]) / P.dy;
+    }
+    if (P.nz > 1) {
+        const int64_t plane = (int64_t)P.nx * P.ny;
+#pragma unroll
+        for (int v = 0; v < NEQ; v++)
+            rhs[v] = rhs[v] - ((double)Fz[v * fsz + t + plane] - (double)Fz[v * fsz + t]) / P.dz;
+    }
+    if (stage == 0) {
+#pragma unroll
+        for (int v = 0; v < NEQ; v++) u[v * ncell + t] = rhs[v];
+        return;
+    }
+    double c[NEQ];
+#pragma unroll
+    for (int v = 0; v < NEQ; v++) {
+        const double a0 = u0[v * ncell + t];
+        if (stage == 1) c[v] = a0 + rhs[v] * dt;
+        else if (stage == 2) c[v] = 0.75 * a0 + 0.25 * (u[v * ncell + t] + rhs[v] * dt);
+        else c[v] = (a0 + 2.0 * (u[v * ncell + t] + rhs[v] * dt)) / 3.0;
+    }
+#pragma unroll
+    for (int v = 0; v < NEQ; v++) u[v * ncell + t] = c[v];
 }
 
-// cons_to_prim after the RK combination (grhd2.f90:252-260), one cell per
-// thread: a serial, divide-heavy root solve per cell, kept apart from the
-// sources so that it runs at a higher occupancy.  WARM starts the root solve
-// from the cell's previous Lorentz factor (mixed-precision path).
-template <int MK, bool WARM>
-__global__ __launch_bounds__(256) void k_c2p_stage(Params P, double *__restrict__ p,
-                                                   double *__restrict__ u,
-                                                   unsigned long long *__restrict__ counters)
+template <int MK, typename R>
+__global__ __launch_bounds__(256) void k_update(Params P, const double *__restrict__ p,
+                                                const R *__restrict__ Fx, const R *__restrict__ Fy,
+                                                const R *__restrict__ Fz, int64_t fsx, int64_t fsy,
+                                                int64_t fsz, const double *__restrict__ u0,
+                                                double *__restrict__ u, int stage, double dt)
 {
     const int64_t t = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)P.nx * P.ny * P.nz) return;
+    update_body<MK, R>(P, p, Fx, Fy, Fz, fsx, fsy, fsz, u0, u, stage, dt, t);
+}
+
+// cons_to_prim after the RK combination (grhd2.f90:252-260) of cell t: a
+// serial, divide-heavy root solve per cell, kept apart from the sources so
+// that it runs at a higher occupancy.  WARM starts the root solve from the
+// cell's previous Lorentz factor (mixed-precision path).
+template <int MK, bool WARM>
+__device__ __forceinline__ void c2p_body(const Params &P, double *__restrict__ p,
+                                         double *__restrict__ u,
+                                         unsigned long long *__restrict__ counters, int64_t t)
+{
     const int64_t ncell = (int64_t)P.nx * P.ny * P.nz;
-    if (t >= ncell) return;
     const int i = (int)(t % P.nx) + 1;
     const int64_t rem = t / P.nx;
     const int j = (int)(rem % P.ny) + 1, k = (int)(rem / P.ny) + 1;
@@ -290,21 +302,29 @@ __global__ __launch_bounds__(256) void k_c2p_stage(Params P, double *__restrict_
     }
 }
 
+template <int MK, bool WARM>
+__global__ __launch_bounds__(256) void k_c2p_stage(Params P, double *__restrict__ p,
+                                                   double *__restrict__ u,
+                                                   unsigned long long *__restrict__ counters)
+{
+    const int64_t t = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)P.nx * P.ny * P.nz) return;
+    c2p_body<MK, WARM>(P, p, u, counters, t);
+}
+
 template <int MK, typename R>
 void launch_stage_mk(const Params &P, hipStream_t s, double *p, void *const F[3], const int64_t fstr[3],
                      const double *u0, double *u, int stage, double dt, unsigned long long *counters)
 {
     const int bs = 256;
-    constexpr int RUN = 1;  // one cell per thread: 4 waves/SIMD beat metric reuse along phi
-    const int64_t ncol = (int64_t)P.nx * P.ny * ((P.nz + RUN - 1) / RUN);
+    const int64_t ncell = (int64_t)P.nx * P.ny * P.nz;
     R *Fx = (R *)F[0], *Fy = (R *)F[1], *Fz = (R *)F[2];
     launch_flux<MK, 1, R>(P, s, p, Fx, fstr[0]);
     if (P.ny > 1) launch_flux<MK, 2, R>(P, s, p, Fy, fstr[1]);
     if (P.nz > 1) launch_flux<MK, 3, R>(P, s, p, Fz, fstr[2]);
-    k_update<MK, R, RUN><<<blocks_for(ncol, bs), bs, 0, s>>>(P, p, Fx, Fy, Fz, fstr[0], fstr[1], fstr[2], u0,
-                                                        u, stage, dt, counters);
+    k_update<MK, R><<<blocks_for(ncell, bs), bs, 0, s>>>(P, p, Fx, Fy, Fz, fstr[0], fstr[1], fstr[2], u0,
+                                                     u, stage, dt);
     if (stage > 0) {
-        const int64_t ncell = (int64_t)P.nx * P.ny * P.nz;
         constexpr bool warm = sizeof(R) == sizeof(float);
         k_c2p_stage<MK, warm><<<blocks_for(ncell, bs), bs, 0, s>>>(P, p, u, counters);
     }

@@ -32,12 +32,18 @@ def fishbone_unit_density_k(cfg: RunConfig) -> float:
 
 
 class Solver:
-    def __init__(self, cfg: RunConfig):
+    def __init__(self, cfg: RunConfig, handle=None):
+        """A new device context for cfg, or (handle given) a view of a context
+        owned elsewhere -- a batch problem -- that close() leaves alive."""
         self.cfg = cfg
         self._L = _lib.lib()
         self._c = _lib.Config.from_run(cfg)
-        h = ctypes.c_void_p()
-        _lib.check(self._L.grhd_create(ctypes.byref(self._c), ctypes.byref(h)))
+        self._owned = handle is None
+        if handle is None:
+            h = ctypes.c_void_p()
+            _lib.check(self._L.grhd_create(ctypes.byref(self._c), ctypes.byref(h)))
+        else:
+            h = handle
         self.h = h
         dims = (ctypes.c_int32 * 4)()
         self._chk(self._L.grhd_dims(self.h, dims))
@@ -50,7 +56,8 @@ class Solver:
 
     def close(self):
         if getattr(self, "h", None):
-            self._L.grhd_destroy(self.h)
+            if self._owned:
+                self._L.grhd_destroy(self.h)
             self.h = None
 
     def __enter__(self):

@@ -36,6 +36,7 @@ extern "C" {
 #define GRHD_ERR_HIP 4
 #define GRHD_ERR_INVALID_ARGUMENT 5
 #define GRHD_ERR_UNSUPPORTED 6
+#define GRHD_ERR_CAPACITY 7         /* batch: no free slot or HBM budget */
 
 /* identifiers: same values as the reference (variables.f90:224-237) */
 #define GRHD_REC_GODUNOV 1
@@ -188,6 +189,50 @@ int grhd_synchronize(grhd_ctx *ctx);
  * the context stream, in milliseconds, and the number of fused-stage kernel
  * launches it made. */
 int grhd_last_timing(grhd_ctx *ctx, double *ms, int64_t *launches);
+
+/* Device bytes grhd_create allocates for cfg: the ghosted primitives, two
+ * conserved buffers and the face fluxes (the reference allocates p, u, up and
+ * rhs, initialization.f90:211-215, plus its metric cache).  Used to admit
+ * problems into a batch by HBM footprint. */
+int grhd_footprint(const grhd_config *cfg, uint64_t *bytes);
+
+/* ------------------------------------------------------------------ batch --
+ * Many independent problems resident on one device and advanced together.
+ * The reference runs one problem per process (grhd2.f90); these entry points
+ * have no counterpart there and serve the serving loop
+ * (grhd_amd/serving.py): problems of different kinds and grid sizes are
+ * admitted while others run, each keeps its own state and time, and every RK3
+ * stage of all of them goes into shared launches.  A problem's results are
+ * bit-identical to running it alone through grhd_advance. */
+typedef struct grhd_batch grhd_batch;
+/* capacity: slots; hbm_budget: bytes of problem state admitted at most
+ * (0: 90 % of the free device memory). */
+int grhd_batch_create(int32_t capacity, uint64_t hbm_budget, grhd_batch **out);
+int grhd_batch_destroy(grhd_batch *b);
+const char *grhd_batch_last_error(const grhd_batch *b);
+/* Admits a problem: validation, device state, initial conditions (as
+ * grhd_create + grhd_initial_conditions), its CFL dt.  GRHD_ERR_CAPACITY when
+ * no slot is free or its footprint exceeds the remaining budget. */
+int grhd_batch_add(grhd_batch *b, const grhd_config *cfg, int32_t *slot);
+/* The problem's context, for grhd_set_primitives / grhd_perturb /
+ * grhd_get_conserved / grhd_get_primitives between steps (not grhd_step). */
+int grhd_batch_problem(grhd_batch *b, int32_t slot, grhd_ctx **ctx);
+/* Frees a problem's slot and device state. */
+int grhd_batch_remove(grhd_batch *b, int32_t slot);
+/* Advances every unfinished problem by nsteps steps of the reference time
+ * loop (grhd2.f90:179-265: dt, final-time clamp, three stages), each with
+ * its own dt and time; a problem stops at its final_time. */
+int grhd_batch_step(grhd_batch *b, int32_t nsteps);
+/* done: final_time reached; t: its time; steps: steps taken in the batch. */
+int grhd_batch_status(grhd_batch *b, int32_t slot, int32_t *done, double *t, int64_t *steps);
+/* Launch form of the batched stage: 0 autotuned (default), 1-3 one kernel per
+ * phase with 4/2/1 waves per face-geometry column, 4-5 one persistent kernel
+ * per stage with 4/1.  All forms give identical results. */
+int grhd_batch_set_variant(grhd_batch *b, int32_t variant);
+/* variant in use, unfinished problems, admitted bytes, device time (ms) and
+ * kernel launches of the last grhd_batch_step. */
+int grhd_batch_info(grhd_batch *b, int32_t *variant, int32_t *active, uint64_t *hbm_used,
+                    double *last_ms, int64_t *launches);
 
 #ifdef __cplusplus
 }
