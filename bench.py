@@ -10,8 +10,11 @@ State resident in HBM before timing.  Reports FP64 (headline) and mixed
 FP32-face / FP64-state throughput, the HBM roofline fraction of the stage
 kernels from the algorithmic 560 B per cell-update, and the reference CPU path
 (OpenMP Fortran built from its sources) on a bounded sample on the host cores.
+Then a served ensemble: a seeded Poisson stream of independent problems of
+mixed kinds and sizes, continuously batched on the device, with p50/p99
+time-to-solution and aggregate cell-updates/s ("ensemble").
 
-python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline] [--serve-requests N]
 """
 from __future__ import annotations
 
@@ -153,6 +156,25 @@ def cpu_baseline(seconds=15.0, nx=64, ny=32, nz=32):
     }
 
 
+def run_ensemble(n, rate, time_scale, chunk, seed=3435):
+    """Served request mix: Poisson arrivals of independent problems (shock
+    tubes nx 200-6400, Michel nr 256/512 with spin 0 or in [0, 0.99), KH
+    128^2/256^2), continuously batched on the device; p50/p99 time-to-solution
+    and aggregate cell-updates/s."""
+    from grhd_amd.batch import Batch, footprint
+    from grhd_amd.serving import Server, request_mix
+    reqs = request_mix(n, rate=rate, seed=seed, time_scale=time_scale)
+    with Batch(capacity=64) as B:
+        rep = Server(B, footprint, hbm_budget=64 << 30, chunk=chunk).run(reqs)
+        info = B.info()
+    s = rep.summary()
+    s.update({"arrival_rate_per_s": rate, "final_time_scale": time_scale, "chunk_steps": chunk,
+              "launch_form": info["variant"], "seed": seed,
+              "mix": "sod/strong_shock/shu_osher nx 200-6400, michel nr 256/512 a=0 or U[0,0.99), "
+                     "kelvin_helmholtz 128^2/256^2 v=+-0.5 Gamma=4/3, each to its preset final_time"})
+    return s
+
+
 def copy_calibration(nbytes=1 << 30, reps=5):
     """A plain device-to-device copy of nbytes, reps times: its FETCH_SIZE and
     WRITE_SIZE per dispatch give the counters' scale against known traffic
@@ -177,6 +199,11 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--cpu-grid", default="64,32,32")
     ap.add_argument("--cpu-child", default=None)
+    ap.add_argument("--serve-requests", type=int, default=32,
+                    help="requests of the served ensemble (0: skip it)")
+    ap.add_argument("--serve-rate", type=float, default=16.0)
+    ap.add_argument("--serve-time-scale", type=float, default=1.0)
+    ap.add_argument("--serve-chunk", type=int, default=32)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain device copy of known size (for HBM counter scaling)")
     a = ap.parse_args()
@@ -238,6 +265,8 @@ def main():
             "speedup_vs_fp64": (res["mixed"]["cell_updates_per_s"] / res["fp64"]["cell_updates_per_s"]
                                 if "fp64" in res else None),
         }
+    if a.serve_requests > 0:
+        line["ensemble"] = run_ensemble(a.serve_requests, a.serve_rate, a.serve_time_scale, a.serve_chunk)
     line["cpu_baseline"] = None if a.no_cpu_baseline else cpu_baseline(a.cpu_seconds, cnx, cny, cnz)
     line["counters"] = head["counters"]
     print(json.dumps(line), flush=True)

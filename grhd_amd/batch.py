@@ -15,8 +15,9 @@ from .params import RunConfig
 from .solver import Solver
 
 GRHD_ERR_CAPACITY = 7
-VARIANTS = {0: "autotuned", 1: "phase kernels, 4 waves per geometry column",
-            2: "phase kernels, 2 waves per geometry column", 3: "phase kernels, 1 wave per geometry column",
+VARIANTS = {0: "autotuned", 1: "phase kernels (6 launches per stage)",
+            2: "fused phase kernels (3 launches per stage), 4 waves per geometry column",
+            3: "fused phase kernels (3 launches per stage), 1 wave per geometry column",
             4: "persistent stage kernel, 4 waves per geometry column",
             5: "persistent stage kernel, 1 wave per geometry column"}
 

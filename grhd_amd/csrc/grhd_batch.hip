@@ -32,14 +32,17 @@ constexpr int kMaxGroups = 16;  // metric kinds x face arithmetics
 constexpr int kMaxChunk = 64;   // steps per captured graph
 constexpr int kTuneSteps = 2;   // steps timed per candidate
 
-// Launch forms: phase kernels (graph of 5-6 launches per stage and group) or
-// one persistent kernel per stage and group; gw = waves sharing a
-// face-geometry column.
+// Launch forms: phase kernels (one face launch per direction, update,
+// con2prim, ghosts; or fused: faces, cells, ghosts), groups running
+// concurrently; or one persistent kernel per stage and group.  gw = waves
+// sharing a face-geometry column.
 struct Variant {
     bool persistent;
+    bool fused;
     int gw;
 };
-const Variant kVariants[] = {{false, 4}, {false, 2}, {false, 1}, {true, 4}, {true, 1}};
+const Variant kVariants[] = {{false, false, 4}, {false, true, 4}, {false, true, 1}, {true, false, 4},
+                             {true, false, 1}};
 constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
 
 struct BRun {
@@ -84,6 +87,10 @@ struct grhd_batch {
     int cu = 0;
     uint64_t limit_ticks = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // groups are independent: in the phase form each group's chain of
+    // launches runs on its own stream inside the captured graph
+    hipStream_t side[kMaxGroups] = {};
+    hipEvent_t fork_ev = nullptr, join_ev[kMaxGroups] = {};
     double last_ms = 0.0;
     int64_t last_launches = 0;
     std::string err;
@@ -196,12 +203,20 @@ BatchTables tables_of(const grhd_batch *b, size_t g, int v)
     return T;
 }
 
-// Enqueue K steps (three stages each) of every group.
+// Enqueue K steps (three stages each) of every group: one chain per group,
+// forked onto the side streams in the phase form (the persistent form keeps
+// the groups in sequence: they share the grid barrier and the whole device).
 void enqueue_steps(grhd_batch *b, int v, int K, int64_t &launches)
 {
-    for (int s = 0; s < K; s++)
-        for (int stage = 1; stage <= 3; stage++)
-            for (size_t g = 0; g < b->groups.size(); g++) {
+    const bool fork = !kVariants[v].persistent && b->groups.size() > 1;
+    if (fork) {
+        (void)hipEventRecord(b->fork_ev, b->stream);
+        for (size_t g = 0; g < b->groups.size(); g++) (void)hipStreamWaitEvent(b->side[g], b->fork_ev, 0);
+    }
+    for (size_t g = 0; g < b->groups.size(); g++) {
+        hipStream_t sg = fork ? b->side[g] : b->stream;
+        for (int s = 0; s < K; s++)
+            for (int stage = 1; stage <= 3; stage++) {
                 BatchLaunch L;
                 std::memset(&L, 0, sizeof(L));
                 L.tables = tables_of(b, g, v);
@@ -210,17 +225,24 @@ void enqueue_steps(grhd_batch *b, int v, int K, int64_t &launches)
                 L.stage = stage;
                 L.mk = b->groups[g].mk;
                 L.persistent = kVariants[v].persistent;
+                L.fused = kVariants[v].fused;
                 L.grid = b->groups[g].grid;
                 L.bar = b->d_bar;
                 L.err = b->d_err;
                 L.limit_ticks = b->limit_ticks;
-                if (b->groups[g].mixed) batch_stage_float(L, b->stream);
-                else batch_stage_double(L, b->stream);
+                if (b->groups[g].mixed) batch_stage_float(L, sg);
+                else batch_stage_double(L, sg);
                 const BatchTables &T = L.tables;
                 launches += L.persistent ? 1
+                            : L.fused    ? 2 + (T.total[PH_BOUND] > 0)
                                          : (T.total[PH_FX] > 0) + (T.total[PH_FY] > 0) + (T.total[PH_FZ] > 0) + 2 +
                                                (T.total[PH_BOUND] > 0);
             }
+        if (fork) {
+            (void)hipEventRecord(b->join_ev[g], sg);
+            (void)hipStreamWaitEvent(b->stream, b->join_ev[g], 0);
+        }
+    }
 }
 
 // The captured graph of K steps with variant v (captured on first use; the
@@ -412,6 +434,12 @@ int grhd_batch_create(int32_t capacity, uint64_t hbm_budget, grhd_batch **out)
     if ((e = hipMemset(b->d_bar, 0, 4 * sizeof(unsigned))) != hipSuccess) return bail(e, "memset");
     if ((e = hipEventCreate(&b->ev0)) != hipSuccess) return bail(e, "event");
     if ((e = hipEventCreate(&b->ev1)) != hipSuccess) return bail(e, "event");
+    if ((e = hipEventCreateWithFlags(&b->fork_ev, hipEventDisableTiming)) != hipSuccess) return bail(e, "event");
+    for (int g = 0; g < kMaxGroups; g++) {
+        if ((e = hipStreamCreateWithFlags(&b->side[g], hipStreamNonBlocking)) != hipSuccess) return bail(e, "stream");
+        if ((e = hipEventCreateWithFlags(&b->join_ev[g], hipEventDisableTiming)) != hipSuccess)
+            return bail(e, "event");
+    }
     *out = b;
     return GRHD_OK;
 }
@@ -432,6 +460,11 @@ int grhd_batch_destroy(grhd_batch *b)
     (void)hipFree(b->d_bar);
     if (b->ev0) (void)hipEventDestroy(b->ev0);
     if (b->ev1) (void)hipEventDestroy(b->ev1);
+    if (b->fork_ev) (void)hipEventDestroy(b->fork_ev);
+    for (int g = 0; g < kMaxGroups; g++) {
+        if (b->join_ev[g]) (void)hipEventDestroy(b->join_ev[g]);
+        if (b->side[g]) (void)hipStreamDestroy(b->side[g]);
+    }
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
     return GRHD_OK;

@@ -39,6 +39,7 @@ struct BatchLaunch {
     int stage;                // RK stage 1..3
     int mk;
     bool persistent;
+    bool fused;               // phase form: 3 launches per stage instead of 6
     unsigned grid;            // persistent grid (co-resident blocks)
     unsigned *bar;
     unsigned *err;

@@ -8,7 +8,10 @@
 // [begin[r], begin[r+1]) and runs the same device body as the single-problem
 // kernels on its own block index, so a problem gives bit-identical results
 // alone or in any mix.  Two launch forms:
-//   - one kernel per phase (five launches per stage, captured in a hipGraph);
+//   - kernels per phase, captured in a hipGraph: either faces of all
+//     directions, cells (update and con2prim fused per thread) and ghosts
+//     -- three launches per stage -- or one face launch per direction and
+//     separate update and con2prim launches;
 //   - one persistent kernel per stage: a co-resident grid walks the three
 //     face phases, a grid barrier, the cell phase (update and con2prim fused
 //     per thread), a barrier, and the ghost phase.
@@ -61,6 +64,27 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
     bflux_block<MK, DIR, R>(T, dts, xcd_block(blockIdx.x, gridDim.x), reinterpret_cast<R *>(lds_geo));
 }
 
+// Faces of every problem and direction: logical blocks are the x-face
+// blocks of all problems, then the y and the z ones.
+template <int MK, typename R>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) void kb_flux3(
+    BatchTables T, const double *__restrict__ dts)
+{
+    extern __shared__ double lds_geo[];
+    R *lds = reinterpret_cast<R *>(lds_geo);
+    int64_t L = xcd_block(blockIdx.x, gridDim.x);
+    if (L < T.total[PH_FX]) {
+        bflux_block<MK, 1, R>(T, dts, L, lds);
+        return;
+    }
+    L -= T.total[PH_FX];
+    if (L < T.total[PH_FY]) {
+        bflux_block<MK, 2, R>(T, dts, L, lds);
+        return;
+    }
+    bflux_block<MK, 3, R>(T, dts, L - T.total[PH_FY], lds);
+}
+
 // Buffer holding U^n at step sstep of the chunk: runs alternate their two
 // conserved buffers every step.
 __device__ __forceinline__ double *u_base(const RunDesc &rd, int sstep)
@@ -103,6 +127,28 @@ __global__ __launch_bounds__(256) void kb_c2p(BatchTables T, const double *__res
     const int64_t t = (L - begin[r]) * 256 + threadIdx.x;
     if (t >= (int64_t)rd.P.nx * rd.P.ny * rd.P.nz) return;
     c2p_body<MK, WARM>(rd.P, rd.p, u_stage(rd, sstep), rd.counters, t);
+}
+
+// Cell update and con2prim of the same cell in one thread.
+template <int MK, typename R>
+__global__ __launch_bounds__(256) void kb_cell(BatchTables T, const double *__restrict__ dts, int sstep,
+                                               int stage)
+{
+    const int64_t L = xcd_block(blockIdx.x, gridDim.x);
+    const int64_t *begin = T.begin + PH_CELL * (T.n + 1);
+    const int r = find_run(begin, T.n, L);
+    const int sl = T.slot[r];
+    const double dt = dts[sl];
+    if (dt == 0.0) return;
+    const RunDesc &rd = T.runs[sl];
+    const Params &P = rd.P;
+    const int64_t t = (L - begin[r]) * 256 + threadIdx.x;
+    if (t >= (int64_t)P.nx * P.ny * P.nz) return;
+    double *u1 = u_stage(rd, sstep);
+    update_body<MK, R>(P, rd.p, reinterpret_cast<const R *>(rd.F[0]), reinterpret_cast<const R *>(rd.F[1]),
+                       reinterpret_cast<const R *>(rd.F[2]), rd.fstr[0], rd.fstr[1], rd.fstr[2],
+                       u_base(rd, sstep), u1, stage, dt, t);
+    c2p_body<MK, sizeof(R) == sizeof(float)>(P, rd.p, u1, rd.counters, t);
 }
 
 template <int MK>
@@ -220,11 +266,16 @@ void batch_stage_mk(const BatchLaunch &B, hipStream_t s)
         kb_stage<MK, R><<<B.grid, 256, lds, s>>>(T, dts, B.sstep, B.stage, B.bar, B.err, B.limit_ticks);
         return;
     }
-    if (T.total[PH_FX]) kb_flux<MK, 1, R><<<(unsigned)T.total[PH_FX], 256, lds, s>>>(T, dts);
-    if (T.total[PH_FY]) kb_flux<MK, 2, R><<<(unsigned)T.total[PH_FY], 256, lds, s>>>(T, dts);
-    if (T.total[PH_FZ]) kb_flux<MK, 3, R><<<(unsigned)T.total[PH_FZ], 256, lds, s>>>(T, dts);
-    kb_update<MK, R><<<(unsigned)T.total[PH_CELL], 256, 0, s>>>(T, dts, B.sstep, B.stage);
-    kb_c2p<MK, sizeof(R) == sizeof(float)><<<(unsigned)T.total[PH_CELL], 256, 0, s>>>(T, dts, B.sstep);
+    if (B.fused) {
+        kb_flux3<MK, R><<<(unsigned)(T.total[PH_FX] + T.total[PH_FY] + T.total[PH_FZ]), 256, lds, s>>>(T, dts);
+        kb_cell<MK, R><<<(unsigned)T.total[PH_CELL], 256, 0, s>>>(T, dts, B.sstep, B.stage);
+    } else {
+        if (T.total[PH_FX]) kb_flux<MK, 1, R><<<(unsigned)T.total[PH_FX], 256, lds, s>>>(T, dts);
+        if (T.total[PH_FY]) kb_flux<MK, 2, R><<<(unsigned)T.total[PH_FY], 256, lds, s>>>(T, dts);
+        if (T.total[PH_FZ]) kb_flux<MK, 3, R><<<(unsigned)T.total[PH_FZ], 256, lds, s>>>(T, dts);
+        kb_update<MK, R><<<(unsigned)T.total[PH_CELL], 256, 0, s>>>(T, dts, B.sstep, B.stage);
+        kb_c2p<MK, sizeof(R) == sizeof(float)><<<(unsigned)T.total[PH_CELL], 256, 0, s>>>(T, dts, B.sstep);
+    }
     if (T.total[PH_BOUND]) kb_bound<MK><<<(unsigned)T.total[PH_BOUND], 256, 0, s>>>(T, dts);
 }
 

@@ -304,7 +304,9 @@ int run_steps(grhd_ctx *c, const double *dts, int n)
     int64_t launches = 0;
     launch_boundaries(P, c->stream, c->p, c->injector);  // grhd2.f90:233
     launches++;
-    const int per_stage = 1 + (P.ny > 1) + (P.nz > 1) + 1 + 1;
+    const int per_stage = ((int64_t)P.nx * P.ny * P.nz <= kFusedStageCells)
+                              ? 3  // faces, cells, ghosts
+                              : 1 + (P.ny > 1) + (P.nz > 1) + 1 + 1 + 1;
     for (int s = 0; s < n; s++) {
         double *u0 = c->u[c->ucur], *u1 = c->u[1 - c->ucur];
         for (int rk = 1; rk <= 3; rk++) {

@@ -1,11 +1,11 @@
 // grhd_kernels.hip -- device kernels of the SSP-RK3 finite-volume update.
 //
-// One RK stage = face-flux kernels (one per active direction) + one cell
-// kernel (flux divergence, sources, RK combination, con2prim) + one boundary
-// kernel.  Kernels are templated on the metric kind so each launch evaluates
-// its analytic metric with compile-time structure; reconstruction and the
-// Riemann solver are uniform runtime branches.  This unit instantiates the FP64
-// stage (compiled without FMA contraction) and the boundary, conversion and CFL
+// One RK stage = one face kernel (every active direction) + one cell kernel
+// (flux divergence, sources, RK combination, con2prim) + one boundary kernel.
+// Kernels are templated on the metric kind so each launch evaluates its
+// analytic metric with compile-time structure; reconstruction and the Riemann
+// solver are uniform runtime branches.  This unit instantiates the FP64 stage
+// (compiled without FMA contraction) and the boundary, conversion and CFL
 // kernels.
 #include "grhd_stage.hpp"
 #include "grhd_bound.hpp"
@@ -143,7 +143,7 @@ void launch_boundaries_mk(const Params &P, hipStream_t s, double *p, const doubl
 
 void launch_stage_f64(const Params &P, hipStream_t s, double *p, void *const F[3],
                       const int64_t fstr[3], const double *u0, double *u, int stage, double dt,
-                      unsigned long long *counters)
+                      unsigned long long *counters, bool fused)
 {
     GRHD_STAGE_DISPATCH(double)
 }
@@ -152,8 +152,10 @@ void launch_stage(const Params &P, hipStream_t s, double *p, void *const F[3], c
                   const double *u0, double *u, int stage, double dt, unsigned long long *counters,
                   bool mixed, const double *injector, bool with_boundaries)
 {
-    if (mixed) launch_stage_f32(P, s, p, F, fstr, u0, u, stage, dt, counters);
-    else launch_stage_f64(P, s, p, F, fstr, u0, u, stage, dt, counters);
+    // small grids are launch-bound: fused stage form (3 launches)
+    const bool fused = (int64_t)P.nx * P.ny * P.nz <= kFusedStageCells;
+    if (mixed) launch_stage_f32(P, s, p, F, fstr, u0, u, stage, dt, counters, fused);
+    else launch_stage_f64(P, s, p, F, fstr, u0, u, stage, dt, counters, fused);
     if (with_boundaries) launch_boundaries(P, s, p, injector);
 }
 

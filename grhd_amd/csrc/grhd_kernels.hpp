@@ -10,10 +10,15 @@ void launch_boundaries_mk(const Params &P, hipStream_t s, double *p, const doubl
 // The stage kernels (face fluxes + cell update) with FP64 or FP32 faces.
 void launch_stage_f64(const Params &P, hipStream_t s, double *p, void *const F[3],
                       const int64_t fstr[3], const double *u0, double *u, int stage, double dt,
-                      unsigned long long *counters);
+                      unsigned long long *counters, bool fused);
 void launch_stage_f32(const Params &P, hipStream_t s, double *p, void *const F[3],
                       const int64_t fstr[3], const double *u0, double *u, int stage, double dt,
-                      unsigned long long *counters);
+                      unsigned long long *counters, bool fused);
+
+// Grids of at most this many cells use the fused stage form (one face launch,
+// one cell launch); larger ones one launch per face direction and separate
+// update and con2prim kernels.
+constexpr int64_t kFusedStageCells = 1 << 16;
 
 // One RK stage: face fluxes, cell update (stage 1..3; 0 = rhs only into u),
 // then the ghost fill when with_boundaries.

@@ -8,7 +8,7 @@ namespace grhd {
 
 void launch_stage_f32(const Params &P, hipStream_t s, double *p, void *const F[3],
                       const int64_t fstr[3], const double *u0, double *u, int stage, double dt,
-                      unsigned long long *counters)
+                      unsigned long long *counters, bool fused)
 {
     GRHD_STAGE_DISPATCH(float)
 }

@@ -3,6 +3,7 @@
 // instantiate them: grhd_kernels.hip (FP64 faces, reference rounding sequence)
 // and grhd_kernels_mixed.hip (FP32 faces).
 #pragma once
+#include <algorithm>
 #include "grhd_kernels.hpp"
 
 namespace grhd {
@@ -312,35 +313,101 @@ __global__ __launch_bounds__(256) void k_c2p_stage(Params P, double *__restrict_
     c2p_body<MK, WARM>(P, p, u, counters, t);
 }
 
+// All face fluxes of a stage in one launch: logical blocks [0, nb[0]) are x
+// faces, then nb[1] blocks of y faces and nb[2] of z faces.
+struct Flux3 {
+    int64_t nb[3];
+    int kt[3], gw[3];
+};
+template <int MK, typename R>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_flux3(
+    Params P, const double *__restrict__ p, R *__restrict__ Fx, R *__restrict__ Fy, R *__restrict__ Fz,
+    int64_t fsx, int64_t fsy, int64_t fsz, Flux3 f)
+{
+    extern __shared__ double lds_geo[];
+    R *lds = reinterpret_cast<R *>(lds_geo);
+    int64_t L = xcd_block(blockIdx.x, gridDim.x);
+    if (L < f.nb[0]) {
+        flux_body<MK, 1, R>(P, p, Fx, fsx, L, f.kt[0], f.gw[0], lds);
+        return;
+    }
+    L -= f.nb[0];
+    if (L < f.nb[1]) {
+        flux_body<MK, 2, R>(P, p, Fy, fsy, L, f.kt[1], f.gw[1], lds);
+        return;
+    }
+    flux_body<MK, 3, R>(P, p, Fz, fsz, L - f.nb[1], f.kt[2], f.gw[2], lds);
+}
+
+// Cell update and con2prim of the same cell in one thread (stages 1..3).
+template <int MK, typename R>
+__global__ __launch_bounds__(256) void k_cell(Params P, double *__restrict__ p, const R *__restrict__ Fx,
+                                              const R *__restrict__ Fy, const R *__restrict__ Fz,
+                                              int64_t fsx, int64_t fsy, int64_t fsz,
+                                              const double *__restrict__ u0, double *__restrict__ u,
+                                              int stage, double dt, unsigned long long *__restrict__ counters)
+{
+    const int64_t t = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)P.nx * P.ny * P.nz) return;
+    update_body<MK, R>(P, p, Fx, Fy, Fz, fsx, fsy, fsz, u0, u, stage, dt, t);
+    c2p_body<MK, sizeof(R) == sizeof(float)>(P, p, u, counters, t);
+}
+
+// fused: one face launch for every direction and update + con2prim in one
+// cell kernel (3 launches per stage, for small grids where launch latency
+// dominates); otherwise one face launch per direction and separate update and
+// con2prim kernels -- measured faster on large grids, where the three face
+// code paths sharing a CU's instruction cache and the 142-VGPR fused cell
+// kernel cost more than the launches.
 template <int MK, typename R>
 void launch_stage_mk(const Params &P, hipStream_t s, double *p, void *const F[3], const int64_t fstr[3],
-                     const double *u0, double *u, int stage, double dt, unsigned long long *counters)
+                     const double *u0, double *u, int stage, double dt, unsigned long long *counters,
+                     bool fused)
 {
     const int bs = 256;
     const int64_t ncell = (int64_t)P.nx * P.ny * P.nz;
     R *Fx = (R *)F[0], *Fy = (R *)F[1], *Fz = (R *)F[2];
-    launch_flux<MK, 1, R>(P, s, p, Fx, fstr[0]);
-    if (P.ny > 1) launch_flux<MK, 2, R>(P, s, p, Fy, fstr[1]);
-    if (P.nz > 1) launch_flux<MK, 3, R>(P, s, p, Fz, fstr[2]);
-    k_update<MK, R><<<blocks_for(ncell, bs), bs, 0, s>>>(P, p, Fx, Fy, Fz, fstr[0], fstr[1], fstr[2], u0,
-                                                     u, stage, dt);
-    if (stage > 0) {
-        constexpr bool warm = sizeof(R) == sizeof(float);
-        k_c2p_stage<MK, warm><<<blocks_for(ncell, bs), bs, 0, s>>>(P, p, u, counters);
+    Flux3 f;
+    int groups = 1;
+    for (int d = 0; d < 3; d++) {
+        const bool on = (d == 0) || (d == 1 && P.ny > 1) || (d == 2 && P.nz > 1);
+        const FluxTiling t = flux_tiling(P.nz + (d == 2));
+        f.kt[d] = t.kt;
+        f.gw[d] = t.gw;
+        f.nb[d] = !on ? 0 : (d == 0) ? flux_blocks<1>(P, t) : (d == 1) ? flux_blocks<2>(P, t) : flux_blocks<3>(P, t);
+        if (on) groups = std::max(groups, 4 / t.gw);
     }
+    if (fused) {
+        const size_t lds = (size_t)groups * NFG * 64 * sizeof(R);
+        k_flux3<MK, R><<<(unsigned)(f.nb[0] + f.nb[1] + f.nb[2]), bs, lds, s>>>(P, p, Fx, Fy, Fz, fstr[0],
+                                                                                fstr[1], fstr[2], f);
+    } else {
+        launch_flux<MK, 1, R>(P, s, p, Fx, fstr[0]);
+        if (P.ny > 1) launch_flux<MK, 2, R>(P, s, p, Fy, fstr[1]);
+        if (P.nz > 1) launch_flux<MK, 3, R>(P, s, p, Fz, fstr[2]);
+    }
+    if (stage > 0 && fused) {
+        k_cell<MK, R><<<blocks_for(ncell, bs), bs, 0, s>>>(P, p, Fx, Fy, Fz, fstr[0], fstr[1], fstr[2], u0, u,
+                                                           stage, dt, counters);
+        return;
+    }
+    k_update<MK, R><<<blocks_for(ncell, bs), bs, 0, s>>>(P, p, Fx, Fy, Fz, fstr[0], fstr[1], fstr[2], u0, u,
+                                                     stage, dt);
+    if (stage > 0)
+        k_c2p_stage<MK, sizeof(R) == sizeof(float)><<<blocks_for(ncell, bs), bs, 0, s>>>(P, p, u, counters);
 }
 
 // Instantiates the stage launcher of one face arithmetic for every metric kind.
 #define GRHD_STAGE_DISPATCH(R)                                                                    \
     switch (P.mk) {                                                                               \
-    case MK_MINK_CART: launch_stage_mk<MK_MINK_CART, R>(P, s, p, F, fstr, u0, u, stage, dt, counters); break; \
-    case MK_MINK_CYL: launch_stage_mk<MK_MINK_CYL, R>(P, s, p, F, fstr, u0, u, stage, dt, counters); break;   \
-    case MK_MINK_SPH: launch_stage_mk<MK_MINK_SPH, R>(P, s, p, F, fstr, u0, u, stage, dt, counters); break;   \
-    case MK_MINK_SPH_LOG: launch_stage_mk<MK_MINK_SPH_LOG, R>(P, s, p, F, fstr, u0, u, stage, dt, counters); break; \
-    case MK_EF: launch_stage_mk<MK_EF, R>(P, s, p, F, fstr, u0, u, stage, dt, counters); break;               \
-    case MK_EF_LOG: launch_stage_mk<MK_EF_LOG, R>(P, s, p, F, fstr, u0, u, stage, dt, counters); break;       \
-    case MK_KS: launch_stage_mk<MK_KS, R>(P, s, p, F, fstr, u0, u, stage, dt, counters); break;               \
-    default: launch_stage_mk<MK_KS_LOG, R>(P, s, p, F, fstr, u0, u, stage, dt, counters); break;              \
+    case MK_MINK_CART: launch_stage_mk<MK_MINK_CART, R>(P, s, p, F, fstr, u0, u, stage, dt, counters, fused); break; \
+    case MK_MINK_CYL: launch_stage_mk<MK_MINK_CYL, R>(P, s, p, F, fstr, u0, u, stage, dt, counters, fused); break;   \
+    case MK_MINK_SPH: launch_stage_mk<MK_MINK_SPH, R>(P, s, p, F, fstr, u0, u, stage, dt, counters, fused); break;   \
+    case MK_MINK_SPH_LOG: launch_stage_mk<MK_MINK_SPH_LOG, R>(P, s, p, F, fstr, u0, u, stage, dt, counters, fused); break; \
+    case MK_EF: launch_stage_mk<MK_EF, R>(P, s, p, F, fstr, u0, u, stage, dt, counters, fused); break;               \
+    case MK_EF_LOG: launch_stage_mk<MK_EF_LOG, R>(P, s, p, F, fstr, u0, u, stage, dt, counters, fused); break;       \
+    case MK_KS: launch_stage_mk<MK_KS, R>(P, s, p, F, fstr, u0, u, stage, dt, counters, fused); break;               \
+    default: launch_stage_mk<MK_KS_LOG, R>(P, s, p, F, fstr, u0, u, stage, dt, counters, fused); break;              \
     }
 
 }  // namespace grhd
