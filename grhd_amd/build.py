@@ -22,7 +22,11 @@ SOURCES = ["grhd_kernels.hip", "grhd_kernels_mixed.hip", "grhd_batch_f64.hip", "
 # "fast": the fused operations are then fixed by the source alone, so the
 # single-problem and the batched kernels -- different code around the same
 # device functions -- round identically.
-EXTRA = {"grhd_kernels_mixed.hip": ["-ffp-contract=on"], "grhd_batch_mixed.hip": ["-ffp-contract=on"]}
+# FP32 division and square root use the hardware reciprocal and square-root
+# instructions (1 ulp) instead of the correctly rounded sequences; FP64
+# arithmetic in these units (metric, sources, con2prim, RK) is unaffected.
+MIXED_FLAGS = ["-ffp-contract=on", "-fno-hip-fp32-correctly-rounded-divide-sqrt"]
+EXTRA = {"grhd_kernels_mixed.hip": MIXED_FLAGS, "grhd_batch_mixed.hip": MIXED_FLAGS}
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("GRHD_OFFLOAD_ARCH", "gfx950")
