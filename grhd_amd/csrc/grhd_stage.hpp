@@ -126,8 +126,14 @@ __device__ __forceinline__ void flux_body(const Params &P, const double *__restr
     const int64_t ncg = (nchunk + groups - 1) / groups;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int grp = wave / gw, wig = wave % gw;
-    const int64_t chunk = (blk % ncg) * groups + grp;
-    const int ctile = (int)(blk / ncg);
+    // Block order: the stencil of x and y faces reaches neighbouring rows of
+    // the same plane, so consecutive blocks walk the plane (tiles of 64 faces)
+    // first; the z stencil reaches neighbouring planes, so z-face blocks walk
+    // along z first.  Consecutive blocks share an XCD's L2 (xcd_block).
+    const int nct = (nfz + kt - 1) / kt;
+    const int64_t cg = (DIR == 3) ? blk / nct : blk % ncg;
+    const int ctile = (DIR == 3) ? (int)(blk % nct) : (int)(blk / ncg);
+    const int64_t chunk = cg * groups + grp;
     const int64_t ab = chunk * 64 + lane;
     const bool active = ab < nab;
     const int a = active ? (int)(ab % nfx) : 0, b = active ? (int)(ab / nfx) : 0;
