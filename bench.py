@@ -175,6 +175,20 @@ def run_ensemble(n, rate, time_scale, chunk, seed=3435):
     return s
 
 
+def measured_traffic(precision="fp64"):
+    """HBM bytes per RK3 stage from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary (profiles/c4_hbm_counters.md, scaled by a plain copy of
+    known size); None when the summary is absent."""
+    import re
+    path = ROOT / "profiles" / "c4_hbm_counters.md"
+    try:
+        text = path.read_text()
+    except OSError:
+        return None
+    m = re.search(rf"TRAFFIC {precision} per RK3 stage: [^=]* written = ([0-9.]+) GB", text)
+    return float(m.group(1)) * 1e9 if m else None
+
+
 def copy_calibration(nbytes=1 << 30, reps=5):
     """A plain device-to-device copy of nbytes, reps times: its FETCH_SIZE and
     WRITE_SIZE per dispatch give the counters' scale against known traffic
@@ -253,8 +267,10 @@ def main():
             "peak": HBM_PEAK / 1e9,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK,
-            "traffic": None,
-            "kernel": "RK3 stage (face fluxes + cell update + ghost fill)",
+            "traffic": measured_traffic("fp64" if "fp64" in res else "mixed"),
+            "traffic_unit": "bytes per RK3 stage (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
+            "kernel": "RK3 stage (face fluxes + cell update + con2prim + ghost fill)",
+            "algorithmic_bytes": B_ALG / 3.0 * head["n_cells"],
             "stage_ms": stage_ms,
         },
     }

@@ -193,3 +193,12 @@ reconstruction = weno5
 riemann_solver = hlle
 final_time = 500
 """
+
+
+# Stationary dust (cold free fall) accretion onto Schwarzschild in EF
+# coordinates, physical r in [1, 51]: the preset of problem 8 at nr cells on a
+# single phi plane, to its preset final time t = 500.
+DUST = """problem = dust
+nx = {nr}
+nz = 1
+"""

@@ -7,9 +7,12 @@ and outputs as compressed .npz fixtures:
   <case>.npz : p0 (ghosted primitives fed in), u1 (conserved after one RK3
                step), p1, u10 (after ten steps), dt, t10
   sod_c1_final.npz : C1 to t = 0.4 (u, p row, number of steps)
-  michel_c2_256.npz: C2 at nr = 256 to t = 500 with seeded 1e-3 density
-               noise: p0, final u, the reference's L1/L2/Linf density error
-               against the analytic solution (output.f90:1189-1213)
+  michel_c2_<nr>.npz: C2 at nr = 256 (and 1024) to t = 500 with seeded 1e-3
+               density noise: p0, final u, the reference's L1/L2/Linf density
+               error against the analytic solution (output.f90:1189-1213)
+  dust_<nr>.npz: stationary dust accretion (problem 8, EF, r in [1, 51]) at
+               nr = 128, 256, 512 to t = 500: final u, p row and the dust
+               norms against the analytic solution (output.f90:1214-1235)
 
 Usage: python tests/golden/make_golden.py [case ...]   (OMP_NUM_THREADS=1)
 """
@@ -24,7 +27,8 @@ sys.path.insert(0, str(ROOT))
 
 import oracle  # noqa: E402
 from grhd_amd.params import config_from_par  # noqa: E402
-from tests.cases import CASES, MICHEL_C2, NOISY  # noqa: E402
+from tests.cases import CASES, DUST, MICHEL_C2, NOISY  # noqa: E402
+from tests.helpers import dust_exact_density, dust_norms  # noqa: E402
 from tests.helpers import michel_norms as helpers_michel_norms  # noqa: E402
 from tests.helpers import seeded_noise  # noqa: E402
 
@@ -97,6 +101,24 @@ def michel_c2(nr=256):
     print(f"michel nr={nr}: {n} steps, L1/L2/Linf = {norms}")
 
 
+def dust(nr):
+    cfg = config_from_par(DUST.format(nr=nr))
+    R = oracle.Reference(cfg)
+    try:
+        R.set_time(0.0, cfg.final_time)
+        t, dt, n = R.step(10 ** 8)
+        u, p = R.get_cons(), R.get_prim()
+        ng = R.ng
+    finally:
+        R.close()
+    x_min, dx = cfg.r_min, (cfg.r_max - cfg.r_min) / nr
+    r = x_min + (np.arange(1, nr + 1) - 0.5) * dx
+    exact = dust_exact_density(r, cfg.bh_mass, cfg.dust_accretion_constant)
+    norms = dust_norms(p, exact, nr, cfg.ny, ng)
+    np.savez_compressed(OUT / f"dust_{nr}.npz", u=u, p_row=p[ng + 1, ng + 1], t=t, n_steps=n, norms=norms)
+    print(f"dust nr={nr}: {n} steps, L1/L2/Linf = {norms}")
+
+
 if __name__ == "__main__":
     os.environ.setdefault("OMP_NUM_THREADS", "1")
     which = sys.argv[1:] or STEP_CASES + ["sod_final", "michel_c2"]
@@ -105,6 +127,10 @@ if __name__ == "__main__":
             sod_final()
         elif w == "michel_c2":
             michel_c2(256)
+        elif w.startswith("michel_c2_"):
+            michel_c2(int(w.split("_")[-1]))
+        elif w.startswith("dust_"):
+            dust(int(w.split("_")[-1]))
         else:
             step_case(w)
         print("wrote", w, flush=True)

@@ -64,3 +64,21 @@ def michel_norms(p, exact, nx, ny, ng):
     d = np.abs(p[ng + 1, ng + j, rows, 0] - exact[ng + 1, ng + j, rows, 0])
     n = i1 - i0 + 1
     return np.array([d.sum() / n, np.sqrt((d ** 2).sum() / n), d.max()])
+
+
+def dust_exact_density(r, bh_mass, accretion_constant):
+    """Density of the stationary dust solution (evaluate_dust_accretion_state,
+    conditions.f90:249-278): rho = -C / (r^2 sqrt(2 M / r))."""
+    return -accretion_constant / (r ** 2 * np.sqrt(2.0 * bh_mass / r))
+
+
+def dust_norms(p, rho_exact, nx, ny, ng):
+    """The dust branch of calc_convergence_norms (output.f90:1214-1235): cells
+    max(1, int(0.10 nx)) .. min(nx, max(i0, int(0.95 nx))) of row
+    j = max(1, ny // 2), k = 1, against rho_exact[i - 1]; (L1, L2, Linf)."""
+    i0 = max(1, int(0.10 * nx))
+    i1 = min(nx, max(i0, int(0.95 * nx)))
+    j = max(1, ny // 2)
+    d = np.abs(p[ng + 1, ng + j, ng + i0:ng + i1 + 1, 0] - rho_exact[i0 - 1:i1])
+    n = i1 - i0 + 1
+    return np.array([d.sum() / n, np.sqrt((d ** 2).sum() / n), d.max()])

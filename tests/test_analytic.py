@@ -1,5 +1,11 @@
 """Analytic-solution errors of the device path against the reference's.
 
+Dust (problem 8: cold free fall onto Schwarzschild in EF coordinates, r in
+[1, 51], WENO5 + HLLE + sensor, t = 500) at nr = 128, 256, 512: from the
+initial data the device reaches the reference's final state and the same
+L1/L2/Linf density errors against the stationary analytic solution
+(output.f90:1214-1235), stored in tests/golden/dust_<nr>.npz.
+
 C2 Michel accretion (EF, log r, nr = 256, WENO5 + HLLE, t = 500, 1e-3 seeded
 density noise): the device run from the reference's perturbed initial state
 reaches the reference's final state and the same L1/L2/Linf density errors
@@ -16,8 +22,8 @@ import numpy as np
 import pytest
 
 from grhd_amd.params import config_from_par
-from tests.cases import MICHEL_C2
-from tests.helpers import michel_norms, rel_errors
+from tests.cases import DUST, MICHEL_C2
+from tests.helpers import dust_exact_density, dust_norms, michel_norms, rel_errors
 
 GOLDEN = Path(__file__).resolve().parent / "golden"
 
@@ -70,3 +76,48 @@ def test_michel_c2_256(precision):
         # FP32 faces: the error against the analytic solution stays that of
         # the scheme (within 10 % of the FP64 reference's L1).
         assert abs(norms[0] - g["norms"][0]) <= 0.1 * g["norms"][0]
+
+
+DUST_NR = [128, 256, 512]
+
+
+@pytest.mark.parametrize("nr", DUST_NR)
+def test_dust_golden_norms_follow_from_final_row(nr):
+    g = np.load(GOLDEN / f"dust_{nr}.npz")
+    cfg = config_from_par(DUST.format(nr=nr))
+    ng = 3
+    r = cfg.r_min + (np.arange(1, nr + 1) - 0.5) * ((cfg.r_max - cfg.r_min) / nr)
+    exact = dust_exact_density(r, cfg.bh_mass, cfg.dust_accretion_constant)
+    p = np.zeros((1, 1 + 2 * ng + 1, nr + 2 * ng + 1, 5))
+    p[0, ng + 1] = g["p_row"]
+    p = np.concatenate([np.zeros_like(p)] * (ng + 1) + [p], axis=0)
+    assert np.array_equal(dust_norms(p, exact, nr, 1, ng), g["norms"])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("nr", DUST_NR)
+def test_dust(nr, precision):
+    from grhd_amd.solver import Solver
+    g = np.load(GOLDEN / f"dust_{nr}.npz")
+    cfg = config_from_par(DUST.format(nr=nr) + f"precision = {precision}\n")
+    print(f"dust nr={nr} {precision}: {int(g['n_steps'])} steps", flush=True)
+    with Solver(cfg) as S:
+        S.initial_conditions()
+        n, _ = S.advance()
+        t = S.t
+        u = S.get_conserved()
+        p = S.get_primitives()
+        ng = S.ng
+    r = cfg.r_min + (np.arange(1, nr + 1) - 0.5) * ((cfg.r_max - cfg.r_min) / nr)
+    norms = dust_norms(p, dust_exact_density(r, cfg.bh_mass, cfg.dust_accretion_constant), nr, 1, ng)
+    print(f"  device norms {norms}  reference {g['norms']}", flush=True)
+    assert n == int(g["n_steps"]) and t == float(g["t"])
+    if precision == "fp64":
+        l1, li = rel_errors(u, g["u"])
+        assert np.all(l1 <= 1e-9) and np.all(li <= 1e-7), (l1, li)
+        assert np.allclose(norms, g["norms"], rtol=1e-3, atol=0)
+    else:
+        # FP32 faces: the L1 error against the analytic solution stays within
+        # a factor 2 of the FP64 reference's at every resolution
+        assert norms[0] <= 2.0 * g["norms"][0] + 1e-8, (norms, g["norms"])
