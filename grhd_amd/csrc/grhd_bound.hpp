@@ -74,14 +74,8 @@ __device__ __forceinline__ void bound_body(const Params &P, double *__restrict__
         case 8:  // inner outflow towards the hole, outer dust injector (conditions.f90:896-907)
             if (hi) {
                 double r_loc, jac;
-                const double xr = xc(P, nx + g);
-                if (MK == MK_EF_LOG || MK == MK_KS_LOG || MK == MK_MINK_SPH_LOG) {
-                    r_loc = exp(xr);
-                    jac = r_loc;
-                } else {
-                    r_loc = xr;
-                    jac = 1.0;
-                }
+                r_loc = tab_xc(P, nx + g)[1];  // r (exp of x on logarithmic grids)
+                jac = (MK == MK_EF_LOG || MK == MK_KS_LOG || MK == MK_MINK_SPH_LOG) ? r_loc : 1.0;
                 const double M = P.mass, ff = sqrt(2.0 * M / r_loc);
                 q[EQ_DE] = -P.dust_accretion_constant / ((r_loc * r_loc) * ff);
                 q[EQ_PR] = P.p_floor;
@@ -99,7 +93,7 @@ __device__ __forceinline__ void bound_body(const Params &P, double *__restrict__
         default: {  // 9-12: shift-aware radial outflow (conditions.f90:918-938, 951-973, 1007-1026)
             FaceMetric m;
             const int ic = hi ? nx : 1;
-            metric_face<MK>(P, xc(P, ic), yc(P, gj), m);
+            metric_face<MK>(P, coord_cc(P, ic, gj), m);
             const double vlim = m.beta[0] / m.alpha;
             load(ic, gj, gk);
             const double rho_edge = q[EQ_DE];

@@ -340,6 +340,7 @@ uint64_t ctx_footprint(const grhd_config &cfg)
     for (int d = 0; d < 3; d++) b += nf[d] * (d == 0 || nf[d] > 1 ? NEQ : 1) * rsz;
     b += (uint64_t)NEQ * P.ng * P.ny * P.nz * sizeof(double) + 8;
     b += (((uint64_t)P.nx * P.ny + 255) / 256) * 4 * sizeof(double) + 4 * sizeof(unsigned long long);
+    b += (uint64_t)coord_table_size(P.nx, P.ny, P.ng) * sizeof(double);
     return b;
 }
 }  // namespace grhd
@@ -424,6 +425,30 @@ int grhd_create(const grhd_config *cfg, grhd_ctx **out)
         return bail(e, "hipMalloc partial");
     if ((e = hipMalloc(&c->counters, 4 * sizeof(unsigned long long))) != hipSuccess)
         return bail(e, "hipMalloc counters");
+    {
+        // coordinate table: the grid's x, r, e^{2x}, sin and cos evaluated once
+        // on the host (initialization.f90:84-263)
+        std::vector<double> t((size_t)coord_table_size(P.nx, P.ny, P.ng));
+        size_t o = 0;
+        auto put_x = [&](double x) {
+            const Coord q = coord_of(P.mk, x, 0.0);
+            t[o++] = q.x;
+            t[o++] = q.r;
+            t[o++] = q.e2;
+        };
+        auto put_y = [&](double y) {
+            t[o++] = std::sin(y);
+            t[o++] = std::cos(y);
+        };
+        for (int i = -P.ng; i <= P.nx + P.ng; i++) put_x(xc(P, i));
+        for (int i = 0; i <= P.nx; i++) put_x(xf(P, i));
+        for (int j = -P.ng; j <= P.ny + P.ng; j++) put_y(yc(P, j));
+        for (int j = 0; j <= P.ny; j++) put_y(yf(P, j));
+        if ((e = hipMalloc(&c->coord, t.size() * sizeof(double))) != hipSuccess) return bail(e, "hipMalloc coord");
+        if ((e = hipMemcpy(c->coord, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice)) != hipSuccess)
+            return bail(e, "hipMemcpy coord");
+        c->P.coord = c->coord;
+    }
     if ((e = hipMemsetAsync(c->counters, 0, 4 * sizeof(unsigned long long), c->stream)) != hipSuccess)
         return bail(e, "hipMemset");
     if ((e = hipMemsetAsync(c->p, 0, (size_t)P.vstride * NEQ * sizeof(double), c->stream)) != hipSuccess)
@@ -446,6 +471,7 @@ int grhd_destroy(grhd_ctx *c)
     (void)hipFree(c->injector);
     (void)hipFree(c->partial);
     (void)hipFree(c->counters);
+    (void)hipFree(c->coord);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);

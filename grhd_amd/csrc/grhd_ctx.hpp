@@ -23,6 +23,7 @@ struct grhd_ctx {
     double *partial = nullptr;
     int partial_n = 0;
     unsigned long long *counters = nullptr;
+    double *coord = nullptr;  // coordinate table (Params::coord)
     int64_t steps = 0;
     double dt_cfl = 0.0;
     bool dt_ready = false;

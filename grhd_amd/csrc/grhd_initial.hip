@@ -26,7 +26,7 @@ struct HostGrid {
 
 template <int MK> void face_metric(const Params &P, double x, double y, FaceMetric &m)
 {
-    metric_face<MK>(P, x, y, m);
+    metric_face<MK>(P, coord_of(MK, x, y), m);
 }
 
 void metric_any(const Params &P, double x, double y, FaceMetric &m)

@@ -24,7 +24,7 @@ __global__ __launch_bounds__(256) void k_p2c(Params P, const double *__restrict_
     const int64_t rem = t / P.nx;
     const int j = (int)(rem % P.ny) + 1, k = (int)(rem / P.ny) + 1;
     double g[3][3], gi[3][3], sqg, alpha, beta[3], q[NEQ], U[NEQ];
-    metric_c2p<MK>(P, xc(P, i), yc(P, j), g, gi, sqg, alpha, beta);
+    metric_c2p<MK>(P, coord_cc(P, i, j), g, gi, sqg, alpha, beta);
     const int64_t pc = pindex(P, i, j, k);
     for (int v = 0; v < NEQ; v++) q[v] = p[v * P.vstride + pc];
     p2c<double>(P.g1, q, Mat3<double>{g}, sqg, U);
@@ -43,7 +43,7 @@ __global__ __launch_bounds__(256) void k_c2p(Params P, double *__restrict__ p, d
     const int64_t rem = t / P.nx;
     const int j = (int)(rem % P.ny) + 1, k = (int)(rem / P.ny) + 1;
     double g[3][3], gi[3][3], sqg, alpha, beta[3], q[NEQ], c[NEQ];
-    metric_c2p<MK>(P, xc(P, i), yc(P, j), g, gi, sqg, alpha, beta);
+    metric_c2p<MK>(P, coord_cc(P, i, j), g, gi, sqg, alpha, beta);
     for (int v = 0; v < NEQ; v++) c[v] = u[v * ncell + t];
     int brent = 0;
     if (c2p(P.g1, P.rho_floor, P.p_floor, g, gi, sqg, c, q, brent)) atomicAdd(&counters[0], 1ull);
@@ -77,7 +77,7 @@ __global__ __launch_bounds__(256) void k_cfl(Params P, double *__restrict__ part
     if (t < (int64_t)P.nx * P.ny) {
         const int i = (int)(t % P.nx) + 1, j = (int)(t / P.nx) + 1;
         CentreMetric m;
-        metric_centre<MK>(P, xc(P, i), yc(P, j), m);
+        metric_centre<MK>(P, coord_cc(P, i, j), m);
         const double a = m.alpha;
         if (!isfinite(a) || a <= 0.0 || !isfinite(m.beta[0]) || !isfinite(m.beta[1]) ||
             !isfinite(m.beta[2]) || !isfinite(m.ginv[0][0]) || m.ginv[0][0] <= 0.0 ||

@@ -13,6 +13,8 @@
 #include <hip/hip_runtime.h>
 #include <float.h>
 #include <math.h>
+
+#include <cmath>
 #include <stdint.h>
 
 #define GRHD_HD __host__ __device__ __forceinline__
@@ -54,6 +56,11 @@ struct Params {
     double jet_ambient_density, jet_ambient_pressure, jet_ambient_velocity;
     double jet_nozzle_radius, jet_density, jet_pressure, jet_velocity;
     double dust_accretion_constant;
+    // coordinate table (coord_table_size doubles, device memory): per radial
+    // centre i = -ng..nx+ng: x, r, e^{2x}; per radial face i = 0..nx: x, r,
+    // e^{2x}; per theta centre j = -ng..ny+ng: sin, cos; per theta face
+    // j = 0..ny: sin, cos
+    const double *coord;
 };
 
 GRHD_HD int64_t pindex(const Params &P, int i, int j, int k)
@@ -67,6 +74,43 @@ GRHD_HD double xf(const Params &P, int i) { return P.x_min + (double)i * P.dx; }
 GRHD_HD double yc(const Params &P, int j) { return P.y_min + ((double)j - 0.5) * P.dy; }
 GRHD_HD double yf(const Params &P, int j) { return P.y_min + (double)j * P.dy; }
 GRHD_HD double zc(const Params &P, int k) { return P.z_min + ((double)k - 0.5) * P.dz; }
+
+// ----------------------------------------------------------- coordinates --
+// A metric evaluation point: x (r, or ln r on logarithmic grids), r, e^{2x},
+// sin(theta), cos(theta).  The kernels read these from the coordinate table,
+// which the host fills once per problem with the same transcendental
+// functions the reference calls (initialization.f90:84-263); everything the
+// metric computes from them is evaluated in registers.
+struct Coord {
+    double x, r, e2, s, c;
+};
+GRHD_HD constexpr bool log_radius_kind(int mk) { return mk == 3 || mk == 5 || mk == 7; }
+
+// Direct evaluation (host set-up code).
+inline Coord coord_of(int mk, double x, double y)
+{
+    return {x, log_radius_kind(mk) ? std::exp(x) : x, std::exp(2.0 * x), std::sin(y), std::cos(y)};
+}
+
+GRHD_HD int64_t coord_table_size(int nx, int ny, int ng)
+{
+    return 3LL * (nx + 2 * ng + 1) + 3LL * (nx + 1) + 2LL * (ny + 2 * ng + 1) + 2LL * (ny + 1);
+}
+GRHD_HD const double *tab_xc(const Params &P, int i) { return P.coord + 3 * (i + P.ng); }
+GRHD_HD const double *tab_xf(const Params &P, int i) { return P.coord + 3 * (P.nx + 2 * P.ng + 1) + 3 * i; }
+GRHD_HD const double *tab_yc(const Params &P, int j)
+{
+    return P.coord + 3 * (P.nx + 2 * P.ng + 1) + 3 * (P.nx + 1) + 2 * (j + P.ng);
+}
+GRHD_HD const double *tab_yf(const Params &P, int j)
+{
+    return P.coord + 3 * (P.nx + 2 * P.ng + 1) + 3 * (P.nx + 1) + 2 * (P.ny + 2 * P.ng + 1) + 2 * j;
+}
+GRHD_HD Coord coord_at(const double *xr, const double *ys) { return {xr[0], xr[1], xr[2], ys[0], ys[1]}; }
+// cell centre (i, j); x face i at centre j; centre i at theta face j
+GRHD_HD Coord coord_cc(const Params &P, int i, int j) { return coord_at(tab_xc(P, i), tab_yc(P, j)); }
+GRHD_HD Coord coord_fc(const Params &P, int i, int j) { return coord_at(tab_xf(P, i), tab_yc(P, j)); }
+GRHD_HD Coord coord_cf(const Params &P, int i, int j) { return coord_at(tab_xc(P, i), tab_yf(P, j)); }
 
 template <typename R> struct Lim;
 template <> struct Lim<double> {
@@ -112,8 +156,9 @@ struct CentreMetric {
 // metrics.f90:753-838 (physical r), 998-1085 (x = ln r); Eddington-Finkelstein:
 // 492-551, 612-670; Minkowski: 209-239, 265-294, 338-368, 419-450.
 template <int MK>
-GRHD_HD void metric_face(const Params &P, double xp, double yp, FaceMetric &m)
+GRHD_HD void metric_face(const Params &P, const Coord &q, FaceMetric &m)
 {
+    const double xp = q.x;
     for (int a = 0; a < 3; a++) {
         m.beta[a] = 0.0;
         for (int b = 0; b < 3; b++) m.g[a][b] = 0.0;
@@ -129,14 +174,14 @@ GRHD_HD void metric_face(const Params &P, double xp, double yp, FaceMetric &m)
         m.g[2][2] = 1.0;
         m.det = xp * xp;
     } else if (MK == MK_MINK_SPH) {
-        const double r2 = xp * xp, s = sin(yp), s2 = s * s;
+        const double r2 = xp * xp, s = q.s, s2 = s * s;
         m.alpha = 1.0;
         m.g[0][0] = 1.0;
         m.g[1][1] = r2;
         m.g[2][2] = r2 * s2;
         m.det = r2 * r2 * s2;
     } else if (MK == MK_MINK_SPH_LOG) {
-        const double r = exp(xp), r2 = r * r, s = sin(yp), s2 = s * s;
+        const double r = q.r, r2 = r * r, s = q.s, s2 = s * s;
         m.alpha = 1.0;
         m.g[0][0] = r2;
         m.g[1][1] = r2;
@@ -144,8 +189,8 @@ GRHD_HD void metric_face(const Params &P, double xp, double yp, FaceMetric &m)
         m.det = r2 * r2 * r2 * s2;
     } else if (MK == MK_EF || MK == MK_EF_LOG) {
         const double M = P.mass;
-        const double r = (MK == MK_EF_LOG) ? exp(xp) : xp;
-        const double s = sin(yp);
+        const double r = q.r;
+        const double s = q.s;
         m.alpha = 1.0 / sqrt(1.0 + 2.0 * M / r);
         const double g11 = (MK == MK_EF_LOG) ? (1.0 + 2.0 * M / r) * (r * r) : 1.0 + 2.0 * M / r;
         m.beta[0] = (MK == MK_EF_LOG) ? (2.0 * M / r / (1.0 + 2.0 * M / r)) / r
@@ -157,8 +202,8 @@ GRHD_HD void metric_face(const Params &P, double xp, double yp, FaceMetric &m)
         m.det = g11 * g22 * g33;
     } else {  // Kerr-Schild
         const double M = P.mass, a = P.spin;
-        const double r = (MK == MK_KS_LOG) ? exp(xp) : xp;
-        const double s = sin(yp), c = cos(yp), s2 = s * s, c2 = c * c;
+        const double r = q.r;
+        const double s = q.s, c = q.c, s2 = s * s, c2 = c * c;
         const double Sig = r * r + (a * a) * c2;
         const double H = M * r / Sig;
         m.alpha = 1.0 / sqrt(1.0 + 2.0 * H);
@@ -184,10 +229,11 @@ GRHD_HD void metric_face(const Params &P, double xp, double yp, FaceMetric &m)
 // (initialization.f90:283-317).  ny1: the grid has ny == 1, which zeroes the
 // theta derivatives (metrics.f90:309, 385, 609, 745, 919, 1178).
 template <int MK>
-GRHD_HD void metric_centre(const Params &P, double xp, double yp, CentreMetric &m)
+GRHD_HD void metric_centre(const Params &P, const Coord &q, CentreMetric &m)
 {
+    const double xp = q.x;
     FaceMetric f;
-    metric_face<MK>(P, xp, yp, f);
+    metric_face<MK>(P, q, f);
     m.alpha = f.alpha;
     for (int a = 0; a < 3; a++) {
         m.beta[a] = f.beta[a];
@@ -214,7 +260,7 @@ GRHD_HD void metric_centre(const Params &P, double xp, double yp, CentreMetric &
         m.gm[3][3] = 1.0;
         m.dg[2][2][0] = 2.0 * xp;
     } else if (MK == MK_MINK_SPH) {  // metrics.f90:287-310
-        const double r2 = xp * xp, s = sin(yp), c = cos(yp), s2 = s * s;
+        const double r2 = xp * xp, s = q.s, c = q.c, s2 = s * s;
         m.gm[0][0] = -1.0;
         m.gm[1][1] = 1.0;
         m.gm[2][2] = 1.0 / r2;
@@ -223,8 +269,8 @@ GRHD_HD void metric_centre(const Params &P, double xp, double yp, CentreMetric &
         m.dg[3][3][0] = 2.0 * xp * (s * s);
         m.dg[3][3][1] = ny1 ? 0.0 : 2.0 * (xp * xp) * s * c;
     } else if (MK == MK_MINK_SPH_LOG) {  // metrics.f90:361-386
-        const double r = exp(xp), r2 = r * r, s = sin(yp), c = cos(yp), s2 = s * s;
-        const double e2 = exp(2.0 * xp);
+        const double r = q.r, r2 = r * r, s = q.s, c = q.c, s2 = s * s;
+        const double e2 = q.e2;
         m.gm[0][0] = -1.0;
         m.gm[1][1] = 1.0 / r2;
         m.gm[2][2] = 1.0 / r2;
@@ -235,8 +281,8 @@ GRHD_HD void metric_centre(const Params &P, double xp, double yp, CentreMetric &
         m.dg[3][3][1] = ny1 ? 0.0 : 2.0 * e2 * s * c;
     } else if (MK == MK_EF || MK == MK_EF_LOG) {  // metrics.f90:525-610, 649-746
         const double M = P.mass;
-        const double r = (MK == MK_EF_LOG) ? exp(xp) : xp;
-        const double s = sin(yp), c = cos(yp);
+        const double r = q.r;
+        const double s = q.s, c = q.c;
         const double a2 = m.alpha * m.alpha;
         m.gm[0][0] = -1.0 / a2;
         for (int i = 0; i < 3; i++) {
@@ -278,8 +324,8 @@ GRHD_HD void metric_centre(const Params &P, double xp, double yp, CentreMetric &
     } else {  // Kerr-Schild: metrics.f90:800-829 / 1045-1078, derivatives 844-921 / 1091-1180
         const double M = P.mass, a = P.spin;
         const bool lg = (MK == MK_KS_LOG);
-        const double r = lg ? exp(xp) : xp;
-        const double s = sin(yp), c = cos(yp), s2 = s * s, c2 = c * c;
+        const double r = q.r;
+        const double s = q.s, c = q.c, s2 = s * s, c2 = c * c;
         const double Sig = r * r + (a * a) * c2;
         const double H = M * r / Sig;
         const double Del = r * r - 2.0 * M * r + a * a;
@@ -370,11 +416,11 @@ GRHD_HD void metric_centre(const Params &P, double xp, double yp, CentreMetric &
 
 // Lighter centre metric for con2prim and prim_to_cons: gamma_ij, gamma^ij, sqrt(det).
 template <int MK>
-GRHD_HD void metric_c2p(const Params &P, double xp, double yp, double g[3][3], double ginv[3][3],
+GRHD_HD void metric_c2p(const Params &P, const Coord &q, double g[3][3], double ginv[3][3],
                         double &sqg, double &alpha, double beta[3])
 {
     CentreMetric m;
-    metric_centre<MK>(P, xp, yp, m);
+    metric_centre<MK>(P, q, m);
     for (int a = 0; a < 3; a++) {
         beta[a] = m.beta[a];
         for (int b = 0; b < 3; b++) {

@@ -38,13 +38,13 @@ __device__ __forceinline__ void face_geometry(const Params &P, int i, int j, R *
     const double eps64 = 64.0 * DBL_EPSILON;
     bool zero = false;
     if (DIR == 1 && P.cyl) zero = fabs(xf(P, i)) <= eps64;
-    if (DIR == 2 && P.polar) zero = fabs(sin(yf(P, j))) <= eps64;
+    if (DIR == 2 && P.polar) zero = fabs(tab_yf(P, j)[0]) <= eps64;  // |sin(theta_face)|
     int flags = zero ? FG_ZERO : 0;
     if (!zero) {
         FaceMetric m;
-        if (DIR == 1) metric_face<MK>(P, xf(P, i), yc(P, j), m);
-        else if (DIR == 2) metric_face<MK>(P, xc(P, i), yf(P, j), m);
-        else metric_face<MK>(P, xc(P, i), yc(P, j), m);  // z faces: centre metric
+        if (DIR == 1) metric_face<MK>(P, coord_fc(P, i, j), m);
+        else if (DIR == 2) metric_face<MK>(P, coord_cf(P, i, j), m);
+        else metric_face<MK>(P, coord_cc(P, i, j), m);  // z faces: centre metric
         R g[3][3], beta[3];
         const R alpha = (R)m.alpha, sqg = (R)sqrt(m.det);
         for (int x = 0; x < 3; x++) {
@@ -216,7 +216,7 @@ __device__ __forceinline__ void update_body(const Params &P, const double *__res
     for (int v = 0; v < NEQ; v++) q[v] = p[v * P.vstride + pc];
     if (MK != MK_MINK_CART) {
         CentreMetric m;
-        metric_centre<MK>(P, xc(P, i), yc(P, j), m);
+        metric_centre<MK>(P, coord_cc(P, i, j), m);
         sources(P.g1, m, q, src);
     } else {
 #pragma unroll
@@ -283,7 +283,7 @@ __device__ __forceinline__ void c2p_body(const Params &P, double *__restrict__ p
     const int64_t rem = t / P.nx;
     const int j = (int)(rem % P.ny) + 1, k = (int)(rem / P.ny) + 1;
     double g[3][3], gi[3][3], sqg, alpha, beta[3], q[NEQ], c[NEQ];
-    metric_c2p<MK>(P, xc(P, i), yc(P, j), g, gi, sqg, alpha, beta);
+    metric_c2p<MK>(P, coord_cc(P, i, j), g, gi, sqg, alpha, beta);
     const int64_t pc = pindex(P, i, j, k);
     double w_start = 0.0;
     if (WARM) {

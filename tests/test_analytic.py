@@ -118,6 +118,9 @@ def test_dust(nr, precision):
         assert np.all(l1 <= 1e-9) and np.all(li <= 1e-7), (l1, li)
         assert np.allclose(norms, g["norms"], rtol=1e-3, atol=0)
     else:
-        # FP32 faces: the L1 error against the analytic solution stays within
-        # a factor 2 of the FP64 reference's at every resolution
-        assert norms[0] <= 2.0 * g["norms"][0] + 1e-8, (norms, g["norms"])
+        # FP32 faces: the flux difference F(i+1) - F(i) of a smooth stationary
+        # flow cancels most digits of two FP32 fluxes, which leaves an error
+        # floor of a few 1e-6 in rho: at coarse grids the L1 error stays that
+        # of the scheme, at fine grids it is the floor (nr 256: 4.8e-6 against
+        # the FP64 path's 2.2e-7)
+        assert norms[0] <= max(2.0 * g["norms"][0], 1.0e-5), (norms, g["norms"])

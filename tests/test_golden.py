@@ -15,8 +15,7 @@ from tests.cases import CASES
 from tests.helpers import read_set_mask, rel_errors
 
 GOLD = Path(__file__).resolve().parent / "golden"
-STEP_CASES = sorted(p.stem for p in GOLD.glob("*.npz")
-                    if not p.stem.endswith("_final") and not p.stem.startswith("michel_c2"))
+STEP_CASES = sorted(p.stem for p in GOLD.glob("*.npz") if p.stem in CASES)
 
 
 def _oracle_for(name, p0):
