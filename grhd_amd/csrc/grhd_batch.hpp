@@ -40,15 +40,42 @@ struct BatchLaunch {
     int mk;
     bool persistent;
     bool fused;               // phase form: 3 launches per stage instead of 6
+    bool small;               // one block per problem runs K whole steps
+    int K;                    // steps of the launch (small form)
+    int cap;                  // row length of the dts table (small form)
     unsigned grid;            // persistent grid (co-resident blocks)
     unsigned *bar;
     unsigned *err;
     uint64_t limit_ticks;
 };
 
-void batch_stage_double(const BatchLaunch &B, hipStream_t s);
-void batch_stage_float(const BatchLaunch &B, hipStream_t s);
-int batch_occupancy_double(int mk, size_t lds);
-int batch_occupancy_float(int mk, size_t lds);
+// Instantiated in four units (face arithmetic x metric-kind half) so that
+// they compile in parallel; the dispatchers are in grhd_batch.hip.
+void batch_stage_double_a(const BatchLaunch &B, hipStream_t s);
+void batch_stage_double_b(const BatchLaunch &B, hipStream_t s);
+void batch_stage_float_a(const BatchLaunch &B, hipStream_t s);
+void batch_stage_float_b(const BatchLaunch &B, hipStream_t s);
+int batch_occupancy_double_a(int mk, size_t lds);
+int batch_occupancy_double_b(int mk, size_t lds);
+int batch_occupancy_float_a(int mk, size_t lds);
+int batch_occupancy_float_b(int mk, size_t lds);
+inline void batch_stage_double(const BatchLaunch &B, hipStream_t s)
+{
+    if (B.mk <= 3) batch_stage_double_a(B, s);
+    else batch_stage_double_b(B, s);
+}
+inline void batch_stage_float(const BatchLaunch &B, hipStream_t s)
+{
+    if (B.mk <= 3) batch_stage_float_a(B, s);
+    else batch_stage_float_b(B, s);
+}
+inline int batch_occupancy_double(int mk, size_t lds)
+{
+    return mk <= 3 ? batch_occupancy_double_a(mk, lds) : batch_occupancy_double_b(mk, lds);
+}
+inline int batch_occupancy_float(int mk, size_t lds)
+{
+    return mk <= 3 ? batch_occupancy_float_a(mk, lds) : batch_occupancy_float_b(mk, lds);
+}
 
 }  // namespace grhd

@@ -255,6 +255,51 @@ __global__ __launch_bounds__(256) void kb_stage(BatchTables T, const double *__r
     }
 }
 
+// ------------------------------------------------------- small problems --
+// One block per problem advances it K whole steps in one launch: faces of
+// every direction, update and con2prim, ghost fill, three stages a step, with
+// block barriers between the phases.  For problems of a few thousand cells
+// the step then costs the block's arithmetic instead of nine kernel launches,
+// and many such problems run side by side on different CUs.  dts holds K rows
+// of cap entries (row s: every slot's dt at step s, 0 once finished).
+constexpr int kSmallThreads = 512;  // 8 waves: two per SIMD hide part of the latency
+
+template <int MK, typename R>
+__global__ __launch_bounds__(kSmallThreads) void kb_small(BatchTables T, const double *__restrict__ dts, int K,
+                                                          int cap)
+{
+    extern __shared__ double lds_geo[];
+    R *lds = reinterpret_cast<R *>(lds_geo);
+    const int sl = T.slot[blockIdx.x];
+    const RunDesc &rd = T.runs[sl];
+    const Params &P = rd.P;
+    const int64_t ncell = (int64_t)P.nx * P.ny * P.nz, nghost = ghost_cells(P);
+    const FluxTiling tx = flux_tiling(faces_z<1>(P), T.gw), ty = flux_tiling(faces_z<2>(P), T.gw),
+                     tz = flux_tiling(faces_z<3>(P), T.gw);
+    constexpr int W = kSmallThreads / 64;
+    const int64_t nbx = flux_blocks<1>(P, tx, W), nby = P.ny > 1 ? flux_blocks<2>(P, ty, W) : 0,
+                  nbz = P.nz > 1 ? flux_blocks<3>(P, tz, W) : 0;
+    R *Fx = reinterpret_cast<R *>(rd.F[0]), *Fy = reinterpret_cast<R *>(rd.F[1]),
+      *Fz = reinterpret_cast<R *>(rd.F[2]);
+    for (int s = 0; s < K; s++) {
+        const double dt = dts[(size_t)s * cap + sl];
+        if (dt == 0.0) break;  // finished: every later row is 0 as well
+        double *u0 = u_base(rd, s), *u1 = u_stage(rd, s);
+        for (int stage = 1; stage <= 3; stage++) {
+            for (int64_t b = 0; b < nbx; b++) flux_body<MK, 1, R>(P, rd.p, Fx, rd.fstr[0], b, tx.kt, tx.gw, lds);
+            for (int64_t b = 0; b < nby; b++) flux_body<MK, 2, R>(P, rd.p, Fy, rd.fstr[1], b, ty.kt, ty.gw, lds);
+            for (int64_t b = 0; b < nbz; b++) flux_body<MK, 3, R>(P, rd.p, Fz, rd.fstr[2], b, tz.kt, tz.gw, lds);
+            for (int64_t t = threadIdx.x; t < ncell; t += blockDim.x) {
+                update_body<MK, R>(P, rd.p, Fx, Fy, Fz, rd.fstr[0], rd.fstr[1], rd.fstr[2], u0, u1, stage, dt, t);
+                c2p_body<MK, sizeof(R) == sizeof(float)>(P, rd.p, u1, rd.counters, t);
+            }
+            __syncthreads();
+            for (int64_t t = threadIdx.x; t < nghost; t += blockDim.x) bound_body<MK>(P, rd.p, rd.injector, t);
+            __syncthreads();
+        }
+    }
+}
+
 // ----------------------------------------------------------------- launch --
 template <int MK, typename R>
 void batch_stage_mk(const BatchLaunch &B, hipStream_t s)
@@ -262,6 +307,11 @@ void batch_stage_mk(const BatchLaunch &B, hipStream_t s)
     const BatchTables &T = B.tables;
     const double *dts = B.dts;
     const size_t lds = (size_t)T.lds_groups * NFG * 64 * sizeof(R);
+    if (B.small) {
+        const size_t lds_small = (size_t)T.lds_groups * (kSmallThreads / 256) * NFG * 64 * sizeof(R);
+        kb_small<MK, R><<<(unsigned)T.n, kSmallThreads, lds_small, s>>>(T, dts, B.K, B.cap);
+        return;
+    }
     if (B.persistent) {
         kb_stage<MK, R><<<B.grid, 256, lds, s>>>(T, dts, B.sstep, B.stage, B.bar, B.err, B.limit_ticks);
         return;
@@ -287,31 +337,30 @@ int batch_occupancy_mk(size_t lds)
     return nb;
 }
 
-#define GRHD_BATCH_DISPATCH(R)                                                         \
-    void batch_stage_##R(const BatchLaunch &B, hipStream_t s)                          \
+// Instantiates the batched stage of one face arithmetic R for the metric
+// kinds of one half: PART a = Minkowski kinds (0-3), b = EF and KS (4-7).
+#define GRHD_BATCH_CASE(MKV, R, CALL) case MKV: CALL(MKV, R); break;
+#define GRHD_BATCH_STAGE_CALL(MKV, R) batch_stage_mk<MKV, R>(B, s)
+#define GRHD_BATCH_OCC_CALL(MKV, R) return batch_occupancy_mk<MKV, R>(lds)
+#define GRHD_BATCH_DISPATCH(R, PART, M0, M1, M2, M3)                                   \
+    void batch_stage_##R##_##PART(const BatchLaunch &B, hipStream_t s)                \
     {                                                                                  \
         switch (B.mk) {                                                                \
-        case MK_MINK_CART: batch_stage_mk<MK_MINK_CART, R>(B, s); break;               \
-        case MK_MINK_CYL: batch_stage_mk<MK_MINK_CYL, R>(B, s); break;                 \
-        case MK_MINK_SPH: batch_stage_mk<MK_MINK_SPH, R>(B, s); break;                 \
-        case MK_MINK_SPH_LOG: batch_stage_mk<MK_MINK_SPH_LOG, R>(B, s); break;         \
-        case MK_EF: batch_stage_mk<MK_EF, R>(B, s); break;                             \
-        case MK_EF_LOG: batch_stage_mk<MK_EF_LOG, R>(B, s); break;                     \
-        case MK_KS: batch_stage_mk<MK_KS, R>(B, s); break;                             \
-        default: batch_stage_mk<MK_KS_LOG, R>(B, s); break;                            \
+            GRHD_BATCH_CASE(M0, R, GRHD_BATCH_STAGE_CALL)                              \
+            GRHD_BATCH_CASE(M1, R, GRHD_BATCH_STAGE_CALL)                              \
+            GRHD_BATCH_CASE(M2, R, GRHD_BATCH_STAGE_CALL)                              \
+            GRHD_BATCH_CASE(M3, R, GRHD_BATCH_STAGE_CALL)                              \
+        default: break;                                                                \
         }                                                                              \
     }                                                                                  \
-    int batch_occupancy_##R(int mk, size_t lds)                                        \
+    int batch_occupancy_##R##_##PART(int mk, size_t lds)                               \
     {                                                                                  \
         switch (mk) {                                                                  \
-        case MK_MINK_CART: return batch_occupancy_mk<MK_MINK_CART, R>(lds);               \
-        case MK_MINK_CYL: return batch_occupancy_mk<MK_MINK_CYL, R>(lds);                 \
-        case MK_MINK_SPH: return batch_occupancy_mk<MK_MINK_SPH, R>(lds);                 \
-        case MK_MINK_SPH_LOG: return batch_occupancy_mk<MK_MINK_SPH_LOG, R>(lds);         \
-        case MK_EF: return batch_occupancy_mk<MK_EF, R>(lds);                             \
-        case MK_EF_LOG: return batch_occupancy_mk<MK_EF_LOG, R>(lds);                     \
-        case MK_KS: return batch_occupancy_mk<MK_KS, R>(lds);                             \
-        default: return batch_occupancy_mk<MK_KS_LOG, R>(lds);                            \
+        case M0: return batch_occupancy_mk<M0, R>(lds);                                \
+        case M1: return batch_occupancy_mk<M1, R>(lds);                                \
+        case M2: return batch_occupancy_mk<M2, R>(lds);                                \
+        case M3: return batch_occupancy_mk<M3, R>(lds);                                \
+        default: return 0;                                                             \
         }                                                                              \
     }
 

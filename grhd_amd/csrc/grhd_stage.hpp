@@ -113,7 +113,7 @@ __host__ __device__ inline FluxTiling flux_tiling(int nfz, int gw_pref = 4)
 }
 
 // Faces of logical block blk of one problem (0 <= blk < flux_blocks()).
-// lds holds (4 / gw) * NFG * 64 values of R.
+// lds holds (waves / gw) * NFG * 64 values of R, waves = blockDim.x / 64.
 template <int MK, int DIR, typename R>
 __device__ __forceinline__ void flux_body(const Params &P, const double *__restrict__ p,
                                           R *__restrict__ F, int64_t fstride, int64_t blk, int kt,
@@ -122,7 +122,7 @@ __device__ __forceinline__ void flux_body(const Params &P, const double *__restr
     const int nfx = P.nx + (DIR == 1), nfy = P.ny + (DIR == 2), nfz = P.nz + (DIR == 3);
     const int64_t nab = (int64_t)nfx * nfy;
     const int64_t nchunk = (nab + 63) / 64;
-    const int groups = 4 / gw;
+    const int groups = (int)(blockDim.x >> 6) / gw;
     const int64_t ncg = (nchunk + groups - 1) / groups;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int grp = wave / gw, wig = wave % gw;
@@ -164,11 +164,12 @@ __device__ __forceinline__ void flux_body(const Params &P, const double *__restr
     __syncthreads();  // the LDS columns are reused by the block's next tile
 }
 
+// Logical blocks of the faces of direction DIR for blocks of `waves` waves.
 template <int DIR>
-inline int64_t flux_blocks(const Params &P, FluxTiling t)
+__host__ __device__ inline int64_t flux_blocks(const Params &P, FluxTiling t, int waves = 4)
 {
     const int nfx = P.nx + (DIR == 1), nfy = P.ny + (DIR == 2), nfz = P.nz + (DIR == 3);
-    const int groups = 4 / t.gw;
+    const int groups = waves / t.gw;
     const int64_t nchunk = ((int64_t)nfx * nfy + 63) / 64;
     return ((nchunk + groups - 1) / groups) * ((nfz + t.kt - 1) / t.kt);
 }
