@@ -107,7 +107,10 @@ program grhd2_hip
       perturbed = .true.
       write(*,'(A,ES14.6)') '>>> perturbation injected at t = ', t
     end if
+    ! up to the next multiple of the monitor stride (grhd2.f90:282-301)
     n_adv = chunk
+    if (rc%mass_monitor_stride > 0) &
+      n_adv = int(rc%mass_monitor_stride - mod(nsteps, int(rc%mass_monitor_stride, 8)), c_int32_t)
     if (.not. perturbed) n_adv = min(n_adv, steps_until(rc%perturbation_time))
     call require(grhd_advance(ctx, cfg%final_time, n_adv, t, done, last_dt), 'grhd_advance')
     nsteps = nsteps + done

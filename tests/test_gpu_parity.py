@@ -155,3 +155,33 @@ def test_mixed_precision_error_is_stated():
     S.close()
     O.close()
     assert l1[0] <= 1e-5, f"mixed density L1 {l1[0]}"
+
+
+def test_large_grid_one_step_matches_oracle():
+    """Grids above 65,536 cells take the per-direction stage form (one face
+    launch per direction, separate update and con2prim) that the 3D torus
+    benchmark runs; smaller grids the fused form the other tests cover."""
+    text = CASES["fm3d_ks_mp5_hllc_sensor"].replace("nx = 24", "nx = 64").replace(
+        "ny = 16", "ny = 32").replace("nz = 8", "nz = 40")
+    cfg = config_from_par(text)
+    assert cfg.nx * cfg.ny * cfg.nz > 65536
+    print("large grid one step", cfg.nx, cfg.ny, cfg.nz, flush=True)
+    with Solver(cfg) as S:
+        O = oracle.Oracle(cfg)
+        S.initial_conditions()
+        p = seeded_noise(S.get_primitives(), S.nx, S.ny, S.nz, S.ng, 2, 1.0e-2, 3435)
+        S.set_primitives(p)
+        u = O.prim_to_cons(p)
+        dt = O.dt()
+        p_o = p.copy()
+        O.boundaries(p_o)
+        S.step(dt, 1)
+        O.step(dt, u, p_o)
+        l1, li = rel_errors(S.get_conserved(), u)
+        O.close()
+    # L1 <= 1e-12 for every variable.  On 82k cells one cell's con2prim root
+    # solve can stop at a different point inside its 1e-10 tolerance (an input
+    # one rounding apart, from the device's exp in WENO3 faces of the sensor
+    # cascade), which shows in the Linf of the small S_theta: 2e-12 of its
+    # maximum.  Linf is held to 1e-11.
+    assert np.all(l1 <= TOL_STEP) and np.all(li <= 1e-11), (l1, li)
