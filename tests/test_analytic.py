@@ -6,11 +6,12 @@ initial data the device reaches the reference's final state and the same
 L1/L2/Linf density errors against the stationary analytic solution
 (output.f90:1214-1235), stored in tests/golden/dust_<nr>.npz.
 
-C2 Michel accretion (EF, log r, nr = 256, WENO5 + HLLE, t = 500, 1e-3 seeded
-density noise): the device run from the reference's perturbed initial state
-reaches the reference's final state and the same L1/L2/Linf density errors
-against the analytic solution (output.f90:1189-1213), stored in
-tests/golden/michel_c2_256.npz by tests/golden/make_golden.py.
+C2 Michel accretion (EF, log r, nr = 256 and 1024, WENO5 + HLLE, t = 500,
+1e-3 seeded density noise): the device run from the reference's perturbed
+initial state reaches the reference's final state and the same L1/L2/Linf
+density errors against the analytic solution (output.f90:1189-1213), stored
+in tests/golden/michel_c2_<nr>.npz by tests/golden/make_golden.py.  nr = 4096
+(945k steps) is not pinned: the reference needs about 19 CPU-hours for it.
 
 The analytic solution is the problem's initial data; the device's initial
 data equals the reference's to 1e-13 (test_gpu_parity), so the norms are
@@ -51,11 +52,12 @@ def test_michel_golden_norms_consistent():
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
-def test_michel_c2_256(precision):
+@pytest.mark.parametrize("nr", [256, 1024])
+def test_michel_c2(nr, precision):
     from grhd_amd.solver import Solver
-    g = np.load(GOLDEN / "michel_c2_256.npz")
-    cfg = config_from_par(MICHEL_C2.format(nr=256) + f"precision = {precision}\n")
-    print(f"michel C2 nr=256 {precision}: {int(g['n_steps'])} steps", flush=True)
+    g = np.load(GOLDEN / f"michel_c2_{nr}.npz")
+    cfg = config_from_par(MICHEL_C2.format(nr=nr) + f"precision = {precision}\n")
+    print(f"michel C2 nr={nr} {precision}: {int(g['n_steps'])} steps", flush=True)
     with Solver(cfg) as S:
         S.initial_conditions()
         exact = S.get_primitives()
@@ -74,8 +76,10 @@ def test_michel_c2_256(precision):
         assert np.allclose(norms, g["norms"], rtol=1e-4, atol=0)
     else:
         # FP32 faces: the error against the analytic solution stays that of
-        # the scheme (within 10 % of the FP64 reference's L1).
-        assert abs(norms[0] - g["norms"][0]) <= 0.1 * g["norms"][0]
+        # the scheme (within 10 % of the FP64 reference's L1) or, where the
+        # scheme's error is below it, the FP32 floor of a smooth stationary
+        # flow (a few 1e-6 in rho).
+        assert abs(norms[0] - g["norms"][0]) <= max(0.1 * g["norms"][0], 5.0e-6), (norms, g["norms"])
 
 
 DUST_NR = [128, 256, 512]
