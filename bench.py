@@ -189,6 +189,46 @@ def measured_traffic(precision="fp64"):
     return float(m.group(1)) * 1e9 if m else None
 
 
+def michel_convergence(resolutions=(256, 1024, 4096)):
+    """C2: Michel accretion (EF, log r in [1.8, 20], r_c = 8, WENO5 + HLLE) to
+    t = 500 from the analytic state with 1e-3 seeded density noise, FP64 and
+    mixed, at each resolution: L1/L2/Linf density error against the analytic
+    solution (output.f90:1189-1213) beside the reference's where
+    tests/golden holds it.  Each run is a batch of one problem (graph-replayed
+    chunks; bit-identical to grhd_advance)."""
+    import numpy as np
+    from grhd_amd.batch import Batch
+    from grhd_amd.params import config_from_par
+    from tests.cases import MICHEL_C2
+    from tests.helpers import michel_norms, seeded_noise
+    out = []
+    for nr in resolutions:
+        gpath = ROOT / "tests" / "golden" / f"michel_c2_{nr}.npz"
+        g = np.load(gpath) if gpath.exists() else None
+        for prec in ("fp64", "mixed"):
+            cfg = config_from_par(MICHEL_C2.format(nr=nr) + f"precision = {prec}\n")
+            with Batch(capacity=1) as B:
+                slot = B.add(cfg)
+                view = B.problem(slot)
+                exact = view.get_primitives()
+                p0 = g["p0"] if g is not None else seeded_noise(exact, view.nx, view.ny, view.nz, view.ng, 2,
+                                                                  1.0e-3, 3435)
+                view.set_primitives(p0)
+                t0 = time.perf_counter()
+                while not B.status(slot)[0]:
+                    B.step(4096)
+                wall = time.perf_counter() - t0
+                done, t, steps = B.status(slot)
+                norms = michel_norms(view.get_primitives(), exact, cfg.nx, cfg.ny, view.ng)
+            rec = {"nr": nr, "precision": prec, "steps": steps, "t": t, "wall_s": wall,
+                   "L1": norms[0], "L2": norms[1], "Linf": norms[2],
+                   "reference_L1": float(g["norms"][0]) if g is not None else None,
+                   "reference_steps": int(g["n_steps"]) if g is not None else None}
+            print(json.dumps(rec), flush=True)
+            out.append(rec)
+    return out
+
+
 def copy_calibration(nbytes=1 << 30, reps=5):
     """A plain device-to-device copy of nbytes, reps times: its FETCH_SIZE and
     WRITE_SIZE per dispatch give the counters' scale against known traffic
@@ -218,11 +258,16 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=16.0)
     ap.add_argument("--serve-time-scale", type=float, default=1.0)
     ap.add_argument("--serve-chunk", type=int, default=32)
+    ap.add_argument("--michel", action="store_true",
+                    help="only the Michel C2 convergence study (nr 256/1024/4096, FP64 and mixed)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain device copy of known size (for HBM counter scaling)")
     a = ap.parse_args()
     if a.copy_calibration:
         copy_calibration()
+        return
+    if a.michel:
+        michel_convergence()
         return
     cnx, cny, cnz = (int(v) for v in a.cpu_grid.split(","))
     if a.cpu_child:
