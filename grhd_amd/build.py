@@ -16,7 +16,7 @@ CSRC = PKG / "csrc"
 LIB = PKG / "libgrhd_hip.so"
 SOURCES = ["grhd_batch_f64_a.hip", "grhd_batch_f64_b.hip", "grhd_batch_mixed_a.hip", "grhd_batch_mixed_b.hip",
            "grhd_kernels.hip", "grhd_kernels_mixed.hip", "grhd_initial.hip", "grhd_capi.hip", "grhd_batch.hip"]
-# The mixed-precision stage is judged by its error against the FP64 path, so
+# The mixed-precision stage is held to an error bound against the FP64 path, so
 # it may contract multiply-adds; every other unit keeps the reference's
 # rounding sequence.  Contraction is "on" (within one source expression), not
 # "fast": the fused operations are then fixed by the source alone, so the

@@ -1,7 +1,7 @@
 // grhd_kernels_mixed.hip -- the RK stage with FP32 reconstruction and Riemann
 // solver (face fluxes stored as FP32); the cell update, sources, con2prim and
 // the RK accumulation stay FP64.  Compiled with FMA contraction: this path is
-// judged by its error against the FP64 path, not by bitwise agreement.
+// held to an error bound against the FP64 path, not to bitwise agreement.
 #include "grhd_stage.hpp"
 
 namespace grhd {

@@ -545,7 +545,7 @@ def _validate_initial(cfg: RunConfig) -> None:
 
 
 def config_to_par(cfg: RunConfig, output_folder: str = "/tmp/grhd_out") -> str:
-    """Write a .par text that makes the reference set up ``cfg`` (for checkers)."""
+    """Write a .par text that makes the reference set up ``cfg`` (for the oracle)."""
     inv_metric = {MINKOWSKI: "minkowski", EDDINGTON_FINKELSTEIN: "eddington_finkelstein",
                   KERR_SCHILD: "kerr_schild"}
     inv_geom = {CARTESIAN: "cartesian", CYLINDRICAL: "cylindrical", SPHERICAL: "spherical",

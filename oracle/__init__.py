@@ -1,4 +1,4 @@
-"""ctypes front-ends for the two CPU checkers.
+"""ctypes front-ends for the two CPU oracles (the C restatement and the reference build).
 
 TEST INFRASTRUCTURE ONLY.  Import from tests/, ``__graft_entry__.smoke`` and the
 CPU-baseline leg of bench.py -- never from the product package.

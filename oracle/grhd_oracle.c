@@ -2,7 +2,7 @@
  * grhd_oracle.c -- CPU restatement (FP64, scalar C) of the GRHD2 per-step
  * finite-volume update.
  *
- * TEST INFRASTRUCTURE ONLY: this is the checker the HIP path is compared
+ * TEST INFRASTRUCTURE ONLY: this is the oracle the HIP path is compared
  * against; nothing in the product (libgrhd_hip.so, the Fortran driver, the
  * serving loop) links or calls it.
  *

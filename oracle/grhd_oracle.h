@@ -1,7 +1,7 @@
 /*
  * grhd_oracle.h -- CPU restatement of the GRHD2 per-step finite-volume update.
  *
- * TEST INFRASTRUCTURE ONLY.  This library is the checker: only tests/, the
+ * TEST INFRASTRUCTURE ONLY.  This library is the oracle: only tests/, the
  * smoke entry point and the bench CPU-baseline leg may load it.  The product
  * path (libgrhd_hip.so) never links or calls it.
  *

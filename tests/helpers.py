@@ -35,7 +35,7 @@ def rel_errors(a, b):
 def seeded_noise(p, nx, ny, nz, ng, kind, amp, seed):
     """Multiplicative noise on the interior (kind 1 pressure, 2 density with
     p scaled by f^Gamma is left to the caller); numpy generator, for feeding the
-    same perturbed state to two checkers."""
+    same perturbed state to the oracle and the device."""
     rng = np.random.default_rng(seed)
     q = p.copy()
     sl = (slice(ng + 1, ng + 1 + nz), slice(ng + 1, ng + 1 + ny), slice(ng + 1, ng + 1 + nx))
