@@ -133,6 +133,26 @@ template <typename R> GRHD_HD R sgn1(R a) { return copysign((R)1, a); }
 GRHD_HD bool finite(double a) { return isfinite(a); }
 GRHD_HD bool finite(float a) { return isfinite(a); }
 
+// a / b, given rb = 1 / b, for divisions that share a divisor or divide by a
+// constant.  An IEEE FP64 division costs a reciprocal and about ten dependent
+// VALU operations; this costs one multiply and two FMAs and returns the same
+// correctly rounded quotient: q = RN(a rb) is within one ulp of a / b, the
+// remainder a - b q is exact in an FMA, and RN(q + (a - b q) rb) is RN(a / b)
+// when rb = RN(1 / b) (P. Markstein, IBM J. Res. Dev. 34 (1990) 111-119).
+// Valid for finite non-zero b and a quotient in the normal range; a zero
+// quotient comes back as +0 where the division gives -0.  A non-finite
+// quotient stays non-finite (inf may come back as NaN), so finite() tests on
+// the result keep their outcome.  FP32 (the mixed path) divides as compiled.
+template <typename R> GRHD_HD R quot(R a, R b, R rb)
+{
+    if constexpr (sizeof(R) == 8) {
+        const R q = a * rb;
+        return fma(fma(-q, b, a), rb, q);
+    } else {
+        return a / b;
+    }
+}
+
 // ---------------------------------------------------------------- metric --
 // The 3+1 quantities on a face: lapse, shift, spatial metric, sqrt(det).
 struct FaceMetric {
@@ -408,9 +428,10 @@ GRHD_HD void metric_centre(const Params &P, const Coord &q, CentreMetric &m)
                 m.ch0[nu][mu] = acc;
             }
     }
+    const double a2 = m.alpha * m.alpha, ra2 = 1.0 / a2;
     for (int a = 0; a < 3; a++)
         for (int b = 0; b < 3; b++)
-            m.ginv[a][b] = m.gm[a + 1][b + 1] + (m.beta[a] * m.beta[b]) / (m.alpha * m.alpha);
+            m.ginv[a][b] = m.gm[a + 1][b + 1] + quot(m.beta[a] * m.beta[b], a2, ra2);
     m.sqg = sqrt(m.det);
 }
 
@@ -586,8 +607,9 @@ GRHD_HD void hlle(R g1, R gam, const R qL[NEQ], const R qR[NEQ], const FG &f, R 
     if (fabs(ap - am) < (R)1.0e-14) {
         for (int e = 0; e < NEQ; e++) F[e] = 0;
     } else {
+        const R den = ap - am, rden = (R)1 / den;
         for (int e = 0; e < NEQ; e++)
-            F[e] = (ap * fL[e] - am * fR[e] + ap * am * (uR[e] - uL[e])) / (ap - am);
+            F[e] = quot(ap * fL[e] - am * fR[e] + ap * am * (uR[e] - uL[e]), den, rden);
     }
 }
 
@@ -619,22 +641,24 @@ GRHD_HD bool face_frame(const R g[3][3], R cof[3][3], R &gdd)
                   g[0][1] * (g[1][0] * g[2][2] - g[1][2] * g[2][0]) +
                   g[0][2] * (g[1][0] * g[2][1] - g[1][1] * g[2][0]);
     if (!finite(det) || det <= (R)1.0e-30) return false;
-    ginv[0][0] = (g[1][1] * g[2][2] - g[1][2] * g[2][1]) / det;
-    ginv[0][1] = (g[0][2] * g[2][1] - g[0][1] * g[2][2]) / det;
-    ginv[0][2] = (g[0][1] * g[1][2] - g[0][2] * g[1][1]) / det;
-    ginv[1][0] = (g[1][2] * g[2][0] - g[1][0] * g[2][2]) / det;
-    ginv[1][1] = (g[0][0] * g[2][2] - g[0][2] * g[2][0]) / det;
-    ginv[1][2] = (g[0][2] * g[1][0] - g[0][0] * g[1][2]) / det;
-    ginv[2][0] = (g[1][0] * g[2][1] - g[1][1] * g[2][0]) / det;
-    ginv[2][1] = (g[0][1] * g[2][0] - g[0][0] * g[2][1]) / det;
-    ginv[2][2] = (g[0][0] * g[1][1] - g[0][1] * g[1][0]) / det;
+    const R rdet = (R)1 / det;
+    ginv[0][0] = quot(g[1][1] * g[2][2] - g[1][2] * g[2][1], det, rdet);
+    ginv[0][1] = quot(g[0][2] * g[2][1] - g[0][1] * g[2][2], det, rdet);
+    ginv[0][2] = quot(g[0][1] * g[1][2] - g[0][2] * g[1][1], det, rdet);
+    ginv[1][0] = quot(g[1][2] * g[2][0] - g[1][0] * g[2][2], det, rdet);
+    ginv[1][1] = quot(g[0][0] * g[2][2] - g[0][2] * g[2][0], det, rdet);
+    ginv[1][2] = quot(g[0][2] * g[1][0] - g[0][0] * g[1][2], det, rdet);
+    ginv[2][0] = quot(g[1][0] * g[2][1] - g[1][1] * g[2][0], det, rdet);
+    ginv[2][1] = quot(g[0][1] * g[2][0] - g[0][0] * g[2][1], det, rdet);
+    ginv[2][2] = quot(g[0][0] * g[1][1] - g[0][1] * g[1][0], det, rdet);
     for (int i = 0; i < 3; i++)
         for (int j = 0; j < 3; j++)
             if (!finite(ginv[i][j])) return false;
     const int d = DIR - 1;
     gdd = ginv[d][d];
     if (gdd <= (R)1.0e-30 || !finite(gdd)) return false;
-    for (int i = 0; i < 3; i++) frame[i][0] = ginv[i][d] / sqrt(gdd);
+    const R sgdd = sqrt(gdd), rsgdd = (R)1 / sgdd;
+    for (int i = 0; i < 3; i++) frame[i][0] = quot(ginv[i][d], sgdd, rsgdd);
     const int t0 = (DIR == 1) ? 1 : (DIR == 2) ? 2 : 0;
     const int t1 = (DIR == 1) ? 2 : (DIR == 2) ? 0 : 1;
     for (int b = 1; b < 3; b++) {
@@ -647,7 +671,8 @@ GRHD_HD bool face_frame(const R g[3][3], R cof[3][3], R &gdd)
         }
         const R nsq = metric_dot(seed, seed, g);
         if (!finite(nsq) || nsq <= (R)1.0e-28) return false;
-        for (int i = 0; i < 3; i++) frame[i][b] = seed[i] / sqrt(nsq);
+        const R snsq = sqrt(nsq), rsnsq = (R)1 / snsq;
+        for (int i = 0; i < 3; i++) frame[i][b] = quot(seed[i], snsq, rsnsq);
     }
     for (int a = 0; a < 3; a++)
         for (int i = 0; i < 3; i++)
@@ -699,8 +724,9 @@ GRHD_HD bool srhd_local(R g1, R gam, const R ql[NEQ], R U[NEQ], R F[NEQ], R &wmi
     const R rad = ((R)1 - vsq) * ((R)1 - vsq * cs2 - (v0 * v0) * ((R)1 - cs2));
     if (den <= (R)1.0e-14 || rad < (R)-1.0e-12) return false;
     const R ac = sqrt(cs2) * sqrt(rmax((R)0, rad));
-    wmin = (v0 * ((R)1 - cs2) - ac) / den;
-    wmax = (v0 * ((R)1 - cs2) + ac) / den;
+    const R rden = (R)1 / den;
+    wmin = quot(v0 * ((R)1 - cs2) - ac, den, rden);
+    wmax = quot(v0 * ((R)1 - cs2) + ac, den, rden);
     for (int e = 0; e < NEQ; e++)
         if (!finite(U[e]) || !finite(F[e])) return false;
     if (!finite(wmin) || !finite(wmax)) return false;
@@ -777,9 +803,10 @@ GRHD_HD bool hllc_try(R g1, R gam, const R qL[NEQ], const R qR[NEQ], const FG &f
     if (w >= lr) return local_to_coord(uR, fR, w, alpha, sqg, gdd, f, F);
     R uh[NEQ], fh[NEQ];
     R den = lr - ll;
+    const R rhll = (R)1 / den;
     for (int e = 0; e < NEQ; e++) {
-        uh[e] = (lr * uR[e] - ll * uL[e] + fL[e] - fR[e]) / den;
-        fh[e] = (lr * fL[e] - ll * fR[e] + lr * ll * (uR[e] - uL[e])) / den;
+        uh[e] = quot(lr * uR[e] - ll * uL[e] + fL[e] - fR[e], den, rhll);
+        fh[e] = quot(lr * fL[e] - ll * fR[e] + lr * ll * (uR[e] - uL[e]), den, rhll);
     }
     for (int e = 0; e < NEQ; e++)
         if (!finite(uh[e]) || !finite(fh[e])) return false;
@@ -841,9 +868,10 @@ GRHD_HD bool hllc_try(R g1, R gam, const R qL[NEQ], const R qR[NEQ], const FG &f
     if (!finite(ps) || fabs(ll - ls) <= (R)1.0e-13 || fabs(lr - ls) <= (R)1.0e-13) return false;
     const R pv[NEQ] = {0, ls, 1, 0, 0};
     R usL[NEQ], usR[NEQ], fsL[NEQ], fsR[NEQ];
+    const R dsL = ll - ls, dsR = lr - ls, rdsL = (R)1 / dsL, rdsR = (R)1 / dsR;
     for (int e = 0; e < NEQ; e++) {
-        usL[e] = (ll * uL[e] - fL[e] + ps * pv[e]) / (ll - ls);
-        usR[e] = (lr * uR[e] - fR[e] + ps * pv[e]) / (lr - ls);
+        usL[e] = quot(ll * uL[e] - fL[e] + ps * pv[e], dsL, rdsL);
+        usR[e] = quot(lr * uR[e] - fR[e] + ps * pv[e], dsR, rdsR);
     }
     for (int e = 0; e < NEQ; e++) {
         fsL[e] = fL[e] + ll * (usL[e] - uL[e]);
@@ -897,7 +925,8 @@ template <typename R> GRHD_HD R weno3(R qm1, R q0, R qp1)
     const R epsh = eps_min + (qm1 * qm1 + q0 * q0 + qp1 * qp1) * exp((R)-10 * sens);
     const R a0 = d0 / ((epsh + b0) * (epsh + b0));
     const R a1 = d1 / ((epsh + b1) * (epsh + b1));
-    const R w0 = a0 / (a0 + a1), w1 = a1 / (a0 + a1);
+    const R as = a0 + a1, ras = (R)1 / as;
+    const R w0 = quot(a0, as, ras), w1 = quot(a1, as, ras);
     return w0 * p0 + w1 * p1;
 }
 // weno5_interface, WENO5-Z (reconstruction.f90:187-215).
@@ -915,7 +944,8 @@ template <typename R> GRHD_HD R weno5(R qm2, R qm1, R q0, R qp1, R qp2)
     const R a0 = d0 * ((R)1 + (tau5 / (eps + b0)));
     const R a1 = d1 * ((R)1 + (tau5 / (eps + b1)));
     const R a2 = d2 * ((R)1 + (tau5 / (eps + b2)));
-    const R w0 = a0 / (a0 + a1 + a2), w1 = a1 / (a0 + a1 + a2), w2 = a2 / (a0 + a1 + a2);
+    const R as = a0 + a1 + a2, ras = (R)1 / as;
+    const R w0 = quot(a0, as, ras), w1 = quot(a1, as, ras), w2 = quot(a2, as, ras);
     const R p0 = (R)(1.0 / 3.0) * q0 + (R)(5.0 / 6.0) * qp1 - (R)(1.0 / 6.0) * qp2;
     const R p1 = -(R)(1.0 / 6.0) * qm1 + (R)(5.0 / 6.0) * q0 + (R)(1.0 / 3.0) * qp1;
     const R p2 = (R)(1.0 / 3.0) * qm2 - (R)(7.0 / 6.0) * qm1 + (R)(11.0 / 6.0) * q0;
@@ -925,7 +955,8 @@ template <typename R> GRHD_HD R weno5(R qm2, R qm1, R q0, R qp1, R qp2)
 template <typename R> GRHD_HD R mp5(R qm2, R qm1, R q0, R qp1, R qp2)
 {
     const R alpha = (R)4, eps = (R)1.0e-10;
-    const R vL = ((R)2 * qm2 - (R)13 * qm1 + (R)47 * q0 + (R)27 * qp1 - (R)3 * qp2) / (R)60;
+    const R vL = quot((R)2 * qm2 - (R)13 * qm1 + (R)47 * q0 + (R)27 * qp1 - (R)3 * qp2, (R)60,
+                      (R)(1.0 / 60.0));
     const R vmp = q0 + minmod2(qp1 - q0, alpha * (q0 - qm1));
     if ((vL - q0) * (vL - vmp) < eps) return vL;
     const R djm1 = qm2 - (R)2 * qm1 + q0;
@@ -936,7 +967,7 @@ template <typename R> GRHD_HD R mp5(R qm2, R qm1, R q0, R qp1, R qp2)
     const R vul = q0 + alpha * (q0 - qm1);
     const R vav = (R)0.5 * (q0 + qp1);
     const R vmd = vav - (R)0.5 * dm4p;
-    const R vlc = q0 + (R)0.5 * (q0 - qm1) + (R)4 * dm4m / (R)3;
+    const R vlc = q0 + (R)0.5 * (q0 - qm1) + quot((R)4 * dm4m, (R)3, (R)(1.0 / 3.0));
     const R vmin = rmax(rmin(rmin(q0, qp1), vmd), rmin(rmin(q0, vul), vlc));
     const R vmax = rmin(rmax(rmax(q0, qp1), vmd), rmax(rmax(q0, vul), vlc));
     return vL + minmod2(vmin - vL, vmax - vL);
@@ -1205,9 +1236,10 @@ GRHD_HD bool c2p(double g1, double rho_floor, double p_floor, const double g[3][
             } else {
                 q[EQ_DE] = c[EQ_DE] / (W * sqg);
                 q[EQ_PR] = (V - c[EQ_PR] - c[EQ_DE]) / sqg;
-                q[EQ_VX] = (gi[0][0] * c[EQ_VX] + gi[0][1] * c[EQ_VY] + gi[0][2] * c[EQ_VZ]) / V;
-                q[EQ_VY] = (gi[1][0] * c[EQ_VX] + gi[1][1] * c[EQ_VY] + gi[1][2] * c[EQ_VZ]) / V;
-                q[EQ_VZ] = (gi[2][0] * c[EQ_VX] + gi[2][1] * c[EQ_VY] + gi[2][2] * c[EQ_VZ]) / V;
+                const double rV = 1.0 / V;
+                q[EQ_VX] = quot(gi[0][0] * c[EQ_VX] + gi[0][1] * c[EQ_VY] + gi[0][2] * c[EQ_VZ], V, rV);
+                q[EQ_VY] = quot(gi[1][0] * c[EQ_VX] + gi[1][1] * c[EQ_VY] + gi[1][2] * c[EQ_VZ], V, rV);
+                q[EQ_VZ] = quot(gi[2][0] * c[EQ_VX] + gi[2][1] * c[EQ_VY] + gi[2][2] * c[EQ_VZ], V, rV);
                 double vf = 0.0;
                 for (int a = 0; a < 3; a++)
                     for (int b = 0; b < 3; b++) vf = vf + g[a][b] * q[EQ_VX + a] * q[EQ_VX + b];
