@@ -306,7 +306,7 @@ int run_steps(grhd_ctx *c, const double *dts, int n)
     launches++;
     const int per_stage = ((int64_t)P.nx * P.ny * P.nz <= kFusedStageCells)
                               ? 3  // faces, cells, ghosts
-                              : 1 + (P.ny > 1) + (P.nz > 1) + 1 + 1 + 1;
+                              : 1 + (P.ny > 1) + (P.nz > 1) + (c->mixed ? 2 : 1) + 1;
     for (int s = 0; s < n; s++) {
         double *u0 = c->u[c->ucur], *u1 = c->u[1 - c->ucur];
         for (int rk = 1; rk <= 3; rk++) {

@@ -393,7 +393,10 @@ void launch_stage_mk(const Params &P, hipStream_t s, double *p, void *const F[3]
         if (P.ny > 1) launch_flux<MK, 2, R>(P, s, p, Fy, fstr[1]);
         if (P.nz > 1) launch_flux<MK, 3, R>(P, s, p, Fz, fstr[2]);
     }
-    if (stage > 0 && fused) {
+    // FP64 faces: update and con2prim in one kernel at every size (u does not
+    // go through HBM between them).  FP32 faces on large grids: the separate
+    // warm-started con2prim kernel is faster.
+    if (stage > 0 && (fused || sizeof(R) == sizeof(double))) {
         k_cell<MK, R><<<blocks_for(ncell, bs), bs, 0, s>>>(P, p, Fx, Fy, Fz, fstr[0], fstr[1], fstr[2], u0, u,
                                                            stage, dt, counters);
         return;
