@@ -349,24 +349,26 @@ GRHD_HD void metric_centre(const Params &P, const Coord &q, CentreMetric &m)
         const double Sig = r * r + (a * a) * c2;
         const double H = M * r / Sig;
         const double Del = r * r - 2.0 * M * r + a * a;
+        const double rSig = 1.0 / Sig, rr = 1.0 / r;
         m.gm[0][0] = -(1.0 + 2.0 * H);
         if (lg) {
-            m.gm[0][1] = (2.0 * H) / r;
-            m.gm[1][1] = (Del / Sig) / (r * r);
-            m.gm[1][3] = (a / Sig) / r;
+            m.gm[0][1] = quot(2.0 * H, r, rr);
+            m.gm[1][1] = quot(Del, Sig, rSig) / (r * r);
+            m.gm[1][3] = quot(quot(a, Sig, rSig), r, rr);
         } else {
             m.gm[0][1] = 2.0 * H;
-            m.gm[1][1] = Del / Sig;
-            m.gm[1][3] = a / Sig;
+            m.gm[1][1] = quot(Del, Sig, rSig);
+            m.gm[1][3] = quot(a, Sig, rSig);
         }
         m.gm[1][0] = m.gm[0][1];
         m.gm[3][1] = m.gm[1][3];
-        m.gm[2][2] = 1.0 / Sig;
+        m.gm[2][2] = rSig;
         m.gm[3][3] = (fabs(s) > 1.0e-15) ? 1.0 / (Sig * s2) : 1.0 / (Sig * 1.0e-30);
-        const double dH_dr = M * ((a * a) * c2 - r * r) / (Sig * Sig);
-        const double dH_dth = 2.0 * M * r * (a * a) * s * c / (Sig * Sig);
-        m.dlna[1] = lg ? -(dH_dr * r) / (1.0 + 2.0 * H) : -dH_dr / (1.0 + 2.0 * H);
-        m.dlna[2] = -dH_dth / (1.0 + 2.0 * H);
+        const double S2 = Sig * Sig, rS2 = 1.0 / S2, h2 = 1.0 + 2.0 * H, rh2 = 1.0 / h2;
+        const double dH_dr = quot(M * ((a * a) * c2 - r * r), S2, rS2);
+        const double dH_dth = quot(2.0 * M * r * (a * a) * s * c, S2, rS2);
+        m.dlna[1] = lg ? quot(-(dH_dr * r), h2, rh2) : quot(-dH_dr, h2, rh2);
+        m.dlna[2] = quot(-dH_dth, h2, rh2);
         double (*dg)[4][2] = m.dg;
         const double dS_dth = -2.0 * (a * a) * s * c;
         if (lg) {
@@ -1271,8 +1273,9 @@ GRHD_HD void sources(double g1, const CentreMetric &m, const double q[NEQ], doub
     const double vsq = vsq_clamp<double>(Mat3<double>{m.g}, v);
     const double h = 1.0 + g1 * q[EQ_PR] / q[EQ_DE];
     const double W = 1.0 / sqrt(1.0 - vsq);
-    u4[0] = W / m.alpha;
-    for (int i = 0; i < 3; i++) u4[i + 1] = W * (v[i] - m.beta[i] / m.alpha);
+    const double ra = 1.0 / m.alpha;
+    u4[0] = quot(W, m.alpha, ra);
+    for (int i = 0; i < 3; i++) u4[i + 1] = W * (v[i] - quot(m.beta[i], m.alpha, ra));
     for (int mu = 0; mu < 4; mu++)
         for (int nu = mu; nu < 4; nu++) {
             T[mu][nu] = (q[EQ_DE] * h * u4[mu] * u4[nu]) + (q[EQ_PR] * m.gm[mu][nu]);

@@ -227,19 +227,19 @@ __device__ __forceinline__ void update_body(const Params &P, const double *__res
         const int64_t fa = ((int64_t)k0 * P.ny + j0) * (P.nx + 1) + i0;
 #pragma unroll
         for (int v = 0; v < NEQ; v++)
-            rhs[v] = -((double)Fx[v * fsx + fa + 1] - (double)Fx[v * fsx + fa]) / P.dx + src[v];
+            rhs[v] = -quot((double)Fx[v * fsx + fa + 1] - (double)Fx[v * fsx + fa], P.dx, P.inv_dx) + src[v];
     }
     if (P.ny > 1) {
         const int64_t fa = ((int64_t)k0 * (P.ny + 1) + j0) * P.nx + i0;
 #pragma unroll
         for (int v = 0; v < NEQ; v++)
-            rhs[v] = rhs[v] - ((double)Fy[v * fsy + fa + P.nx] - (double)Fy[v * fsy + fa]) / P.dy;
+            rhs[v] = rhs[v] - quot((double)Fy[v * fsy + fa + P.nx] - (double)Fy[v * fsy + fa], P.dy, P.inv_dy);
     }
     if (P.nz > 1) {
         const int64_t plane = (int64_t)P.nx * P.ny;
 #pragma unroll
         for (int v = 0; v < NEQ; v++)
-            rhs[v] = rhs[v] - ((double)Fz[v * fsz + t + plane] - (double)Fz[v * fsz + t]) / P.dz;
+            rhs[v] = rhs[v] - quot((double)Fz[v * fsz + t + plane] - (double)Fz[v * fsz + t], P.dz, P.inv_dz);
     }
     if (stage == 0) {
 #pragma unroll
@@ -252,7 +252,7 @@ __device__ __forceinline__ void update_body(const Params &P, const double *__res
         const double a0 = u0[v * ncell + t];
         if (stage == 1) c[v] = a0 + rhs[v] * dt;
         else if (stage == 2) c[v] = 0.75 * a0 + 0.25 * (u[v * ncell + t] + rhs[v] * dt);
-        else c[v] = (a0 + 2.0 * (u[v * ncell + t] + rhs[v] * dt)) / 3.0;
+        else c[v] = quot(a0 + 2.0 * (u[v * ncell + t] + rhs[v] * dt), 3.0, 1.0 / 3.0);
     }
 #pragma unroll
     for (int v = 0; v < NEQ; v++) u[v * ncell + t] = c[v];
