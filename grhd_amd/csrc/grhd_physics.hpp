@@ -142,16 +142,48 @@ GRHD_HD bool finite(float a) { return isfinite(a); }
 // Valid for finite non-zero b and a quotient in the normal range; a zero
 // quotient comes back as +0 where the division gives -0.  A non-finite
 // quotient stays non-finite (inf may come back as NaN), so finite() tests on
-// the result keep their outcome.  FP32 (the mixed path) divides as compiled.
+// the result keep their outcome.  FP32 (the mixed path): the same two FMAs
+// from the refined hardware reciprocal below.
 template <typename R> GRHD_HD R quot(R a, R b, R rb)
 {
+    const R q = a * rb;
+    return fma(fma(-q, b, a), rb, q);
+}
+
+// Reciprocal, division and reciprocal square root in the face arithmetic R.
+// FP64: the IEEE operations the reference performs.  FP32 (the mixed path,
+// held to an error bound against FP64): the hardware v_rcp_f32 (about 1 ulp)
+// refined by one Newton step, y + y (1 - b y), then the remainder correction
+// of quot: within about half an ulp, in four instructions instead of the eight
+// of the range-scaled division.  Every FP32 operand of the path is a normal
+// number (the floors 1e-10 and 1e-13 included), where v_rcp_f32 holds its
+// accuracy.
+template <typename R> GRHD_HD R rcp(R b)
+{
     if constexpr (sizeof(R) == 8) {
-        const R q = a * rb;
-        return fma(fma(-q, b, a), rb, q);
+        return (R)1 / b;
     } else {
-        return a / b;
+#if defined(__HIP_DEVICE_COMPILE__)
+        const float y = __builtin_amdgcn_rcpf(b);
+        return fmaf(fmaf(-b, y, 1.0f), y, y);
+#else
+        return 1.0f / b;
+#endif
     }
 }
+template <typename R> GRHD_HD R rdiv(R a, R b)
+{
+    if constexpr (sizeof(R) == 8) {
+        return a / b;
+    } else {
+#if defined(__HIP_DEVICE_COMPILE__)
+        return quot(a, b, __builtin_amdgcn_rcpf(b));
+#else
+        return a / b;
+#endif
+    }
+}
+template <typename R> GRHD_HD R rrsqrt(R x) { return rcp(sqrt(x)); }
 
 // ---------------------------------------------------------------- metric --
 // The 3+1 quantities on a face: lapse, shift, spatial metric, sqrt(det).
@@ -473,9 +505,9 @@ GRHD_HD R vsq_clamp(const G &g, R v[3])
     for (int a = 0; a < 3; a++)
         for (int b = 0; b < 3; b++) vsq = vsq + g.gam(a, b) * v[a] * v[b];
     if (vsq >= vmax) {
-        v[0] = v[0] * sqrt(vmax / vsq);
-        v[1] = v[1] * sqrt(vmax / vsq);
-        v[2] = v[2] * sqrt(vmax / vsq);
+        v[0] = v[0] * sqrt(rdiv(vmax, vsq));
+        v[1] = v[1] * sqrt(rdiv(vmax, vsq));
+        v[2] = v[2] * sqrt(rdiv(vmax, vsq));
         vsq = vmax;
     }
     return vsq;
@@ -497,8 +529,8 @@ GRHD_HD void p2c(R g1, const R q[NEQ], const G &g, R sqg, R U[NEQ])
     R v[3] = {q[EQ_VX], q[EQ_VY], q[EQ_VZ]}, vc[3];
     const R vsq = vsq_clamp<R>(g, v);
     lower<R>(g, v, vc);
-    const R h = (R)1 + g1 * q[EQ_PR] / q[EQ_DE];
-    const R W = (R)1 / sqrt((R)1 - vsq);
+    const R h = (R)1 + rdiv(g1 * q[EQ_PR], q[EQ_DE]);
+    const R W = rrsqrt((R)1 - vsq);
     U[EQ_DE] = q[EQ_DE] * W;
     U[EQ_PR] = q[EQ_DE] * h * (W * W) - q[EQ_PR] - q[EQ_DE] * W;
     U[EQ_VX] = q[EQ_DE] * h * (W * W) * vc[0];
@@ -549,8 +581,8 @@ GRHD_HD void phys_flux(R g1, const R q[NEQ], const FG &f, R F[NEQ])
     const R vsq = vsq_clamp<R>(f, v);
     lower<R>(f, v, vc);
     const R vn = v[DIR - 1], bn = f.beta(DIR - 1);
-    const R h = (R)1 + g1 * q[EQ_PR] / q[EQ_DE];
-    const R W = (R)1 / sqrt((R)1 - vsq);
+    const R h = (R)1 + rdiv(g1 * q[EQ_PR], q[EQ_DE]);
+    const R W = rrsqrt((R)1 - vsq);
     const R veff = f.alpha() * vn - bn;
     F[EQ_DE] = q[EQ_DE] * W * veff;
     F[EQ_PR] = q[EQ_DE] * h * (W * W) * veff + q[EQ_PR] * bn - F[EQ_DE];
@@ -568,21 +600,21 @@ GRHD_HD void wave_speeds(R g1, R gam, const R q[NEQ], const FG &f, R &wmin, R &w
 {
     R v[3] = {q[EQ_VX], q[EQ_VY], q[EQ_VZ]};
     const R vsq = vsq_clamp<R>(f, v);
-    const R enth = (R)1 + g1 * q[EQ_PR] / q[EQ_DE];
-    const R cs2 = gam * q[EQ_PR] / (q[EQ_DE] * enth);
+    const R enth = (R)1 + rdiv(g1 * q[EQ_PR], q[EQ_DE]);
+    const R cs2 = rdiv(gam * q[EQ_PR], q[EQ_DE] * enth);
     const R cs = sqrt(cs2);
     const R vn = v[DIR - 1];
     R e1, e2, e3;
     if (SRHD) {
-        const R t1 = (R)1 / ((R)1 - vsq * cs2);
-        const R t2 = cs * sqrt(((R)1 - vsq) * ((R)1 / t1 - (vn * vn) * ((R)1 - cs2)));
+        const R t1 = rcp((R)1 - vsq * cs2);
+        const R t2 = cs * sqrt(((R)1 - vsq) * (rcp(t1) - (vn * vn) * ((R)1 - cs2)));
         e1 = vn;
         e2 = t1 * (vn * ((R)1 - cs2) + t2);
         e3 = t1 * (vn * ((R)1 - cs2) - t2);
     } else {
         const R gii = f.gii();
         const R bn = f.beta(DIR - 1);
-        const R t1 = f.alpha() / ((R)1 - vsq * cs2);
+        const R t1 = rdiv(f.alpha(), (R)1 - vsq * cs2);
         const R t2 = cs * sqrt(rmax((R)0, ((R)1 - vsq) * (gii * ((R)1 - vsq * cs2) -
                                                         (vn * vn) * ((R)1 - cs2))));
         e1 = f.alpha() * vn - bn;
@@ -609,7 +641,7 @@ GRHD_HD void hlle(R g1, R gam, const R qL[NEQ], const R qR[NEQ], const FG &f, R 
     if (fabs(ap - am) < (R)1.0e-14) {
         for (int e = 0; e < NEQ; e++) F[e] = 0;
     } else {
-        const R den = ap - am, rden = (R)1 / den;
+        const R den = ap - am, rden = rcp(den);
         for (int e = 0; e < NEQ; e++)
             F[e] = quot(ap * fL[e] - am * fR[e] + ap * am * (uR[e] - uL[e]), den, rden);
     }
@@ -643,7 +675,7 @@ GRHD_HD bool face_frame(const R g[3][3], R cof[3][3], R &gdd)
                   g[0][1] * (g[1][0] * g[2][2] - g[1][2] * g[2][0]) +
                   g[0][2] * (g[1][0] * g[2][1] - g[1][1] * g[2][0]);
     if (!finite(det) || det <= (R)1.0e-30) return false;
-    const R rdet = (R)1 / det;
+    const R rdet = rcp(det);
     ginv[0][0] = quot(g[1][1] * g[2][2] - g[1][2] * g[2][1], det, rdet);
     ginv[0][1] = quot(g[0][2] * g[2][1] - g[0][1] * g[2][2], det, rdet);
     ginv[0][2] = quot(g[0][1] * g[1][2] - g[0][2] * g[1][1], det, rdet);
@@ -659,7 +691,7 @@ GRHD_HD bool face_frame(const R g[3][3], R cof[3][3], R &gdd)
     const int d = DIR - 1;
     gdd = ginv[d][d];
     if (gdd <= (R)1.0e-30 || !finite(gdd)) return false;
-    const R sgdd = sqrt(gdd), rsgdd = (R)1 / sgdd;
+    const R sgdd = sqrt(gdd), rsgdd = rcp(sgdd);
     for (int i = 0; i < 3; i++) frame[i][0] = quot(ginv[i][d], sgdd, rsgdd);
     const int t0 = (DIR == 1) ? 1 : (DIR == 2) ? 2 : 0;
     const int t1 = (DIR == 1) ? 2 : (DIR == 2) ? 0 : 1;
@@ -673,7 +705,7 @@ GRHD_HD bool face_frame(const R g[3][3], R cof[3][3], R &gdd)
         }
         const R nsq = metric_dot(seed, seed, g);
         if (!finite(nsq) || nsq <= (R)1.0e-28) return false;
-        const R snsq = sqrt(nsq), rsnsq = (R)1 / snsq;
+        const R snsq = sqrt(nsq), rsnsq = rcp(snsq);
         for (int i = 0; i < 3; i++) frame[i][b] = quot(seed[i], snsq, rsnsq);
     }
     for (int a = 0; a < 3; a++)
@@ -704,11 +736,11 @@ GRHD_HD bool srhd_local(R g1, R gam, const R ql[NEQ], R U[NEQ], R F[NEQ], R &wmi
     const R vsq = v0 * v0 + v1 * v1 + v2 * v2;
     if (!finite(vsq) || vsq < (R)0) return false;
     if (vsq >= vmax) return false;
-    const R h = (R)1 + g1 * pr / rho;
+    const R h = (R)1 + rdiv(g1 * pr, rho);
     if (!finite(h) || h <= (R)0) return false;
-    const R cs2 = gam * pr / (rho * h);
+    const R cs2 = rdiv(gam * pr, rho * h);
     if (!finite(cs2) || cs2 < (R)0 || cs2 >= (R)1) return false;
-    const R W = (R)1 / sqrt((R)1 - vsq);
+    const R W = rrsqrt((R)1 - vsq);
     const R D = rho * W;
     const R tau = rho * h * (W * W) - pr - D;
     const R m0 = rho * h * (W * W) * v0, m1 = rho * h * (W * W) * v1, m2 = rho * h * (W * W) * v2;
@@ -726,7 +758,7 @@ GRHD_HD bool srhd_local(R g1, R gam, const R ql[NEQ], R U[NEQ], R F[NEQ], R &wmi
     const R rad = ((R)1 - vsq) * ((R)1 - vsq * cs2 - (v0 * v0) * ((R)1 - cs2));
     if (den <= (R)1.0e-14 || rad < (R)-1.0e-12) return false;
     const R ac = sqrt(cs2) * sqrt(rmax((R)0, rad));
-    const R rden = (R)1 / den;
+    const R rden = rcp(den);
     wmin = quot(v0 * ((R)1 - cs2) - ac, den, rden);
     wmax = quot(v0 * ((R)1 - cs2) + ac, den, rden);
     for (int e = 0; e < NEQ; e++)
@@ -805,7 +837,7 @@ GRHD_HD bool hllc_try(R g1, R gam, const R qL[NEQ], const R qR[NEQ], const FG &f
     if (w >= lr) return local_to_coord(uR, fR, w, alpha, sqg, gdd, f, F);
     R uh[NEQ], fh[NEQ];
     R den = lr - ll;
-    const R rhll = (R)1 / den;
+    const R rhll = rcp(den);
     for (int e = 0; e < NEQ; e++) {
         uh[e] = quot(lr * uR[e] - ll * uL[e] + fL[e] - fR[e], den, rhll);
         fh[e] = quot(lr * fL[e] - ll * fR[e] + lr * ll * (uR[e] - uL[e]), den, rhll);
@@ -823,22 +855,22 @@ GRHD_HD bool hllc_try(R g1, R gam, const R qL[NEQ], const R qR[NEQ], const FG &f
     R ls;
     if (!quad) {
         if (fabs(B) <= (R)1.0e-14 * rmax(Lim<R>::tiny(), fabs(Cq))) return false;
-        ls = -Cq / B;
+        ls = rdiv(-Cq, B);
     } else {
         den = -B - sqrt(disc);
         if (fabs(den) > (R)1.0e-14 * rmax(rmax(Lim<R>::tiny(), fabs(B)), sqrt(disc))) {
-            ls = den / ((R)2 * A);
+            ls = rdiv(den, (R)2 * A);
         } else {
             den = -B + sqrt(disc);
             if (fabs(den) <= (R)1.0e-14 * rmax(rmax(Lim<R>::tiny(), fabs(B)), sqrt(disc)))
                 return false;
-            ls = ((R)2 * Cq) / den;
+            ls = rdiv((R)2 * Cq, den);
         }
     }
     const R rtol = (R)1.0e-10 * rmax(rmax((R)1, fabs(ll)), fabs(lr));
     bool valid = finite(ls) && ls >= ll - rtol && ls <= lr + rtol && fabs(ls) <= (R)1 + rtol;
     if (!valid && quad) {
-        const R ra = (-B + sqrt(disc)) / ((R)2 * A);
+        const R ra = rdiv(-B + sqrt(disc), (R)2 * A);
         if (finite(ra) && ra >= ll - rtol && ra <= lr + rtol && fabs(ra) <= (R)1 + rtol) {
             ls = ra;
             valid = true;
@@ -852,10 +884,10 @@ GRHD_HD bool hllc_try(R g1, R gam, const R qL[NEQ], const R qR[NEQ], const FG &f
     const R BR = uR[EQ_VX] * (lr - qRh[EQ_VX]) - qRh[EQ_PR];
     den = (R)1 - ll * ls;
     if (fabs(den) <= (R)1.0e-13) return false;
-    const R psL = (AL * ls - BL) / den;
+    const R psL = rdiv(AL * ls - BL, den);
     den = (R)1 - lr * ls;
     if (fabs(den) <= (R)1.0e-13) return false;
-    const R psR = (AR * ls - BR) / den;
+    const R psR = rdiv(AR * ls - BR, den);
     const R sscale = rmax(rmax(rmax(rmax(Lim<R>::tiny(), fabs(uL[EQ_PR] + uL[EQ_DE])),
                                     fabs(uR[EQ_PR] + uR[EQ_DE])),
                                fabs(uL[EQ_VX])),
@@ -870,7 +902,7 @@ GRHD_HD bool hllc_try(R g1, R gam, const R qL[NEQ], const R qR[NEQ], const FG &f
     if (!finite(ps) || fabs(ll - ls) <= (R)1.0e-13 || fabs(lr - ls) <= (R)1.0e-13) return false;
     const R pv[NEQ] = {0, ls, 1, 0, 0};
     R usL[NEQ], usR[NEQ], fsL[NEQ], fsR[NEQ];
-    const R dsL = ll - ls, dsR = lr - ls, rdsL = (R)1 / dsL, rdsR = (R)1 / dsR;
+    const R dsL = ll - ls, dsR = lr - ls, rdsL = rcp(dsL), rdsR = rcp(dsR);
     for (int e = 0; e < NEQ; e++) {
         usL[e] = quot(ll * uL[e] - fL[e] + ps * pv[e], dsL, rdsL);
         usR[e] = quot(lr * uR[e] - fR[e] + ps * pv[e], dsR, rdsR);
@@ -923,11 +955,11 @@ template <typename R> GRHD_HD R weno3(R qm1, R q0, R qp1)
     const R p1 = (R)-0.5 * qm1 + (R)1.5 * q0;
     const R b0 = (q0 - qp1) * (q0 - qp1);
     const R b1 = (qm1 - q0) * (qm1 - q0);
-    const R sens = fabs(b0 - b1) / (rmax(b0, b1) + eps_min);
+    const R sens = rdiv(fabs(b0 - b1), rmax(b0, b1) + eps_min);
     const R epsh = eps_min + (qm1 * qm1 + q0 * q0 + qp1 * qp1) * exp((R)-10 * sens);
-    const R a0 = d0 / ((epsh + b0) * (epsh + b0));
-    const R a1 = d1 / ((epsh + b1) * (epsh + b1));
-    const R as = a0 + a1, ras = (R)1 / as;
+    const R a0 = rdiv(d0, (epsh + b0) * (epsh + b0));
+    const R a1 = rdiv(d1, (epsh + b1) * (epsh + b1));
+    const R as = a0 + a1, ras = rcp(as);
     const R w0 = quot(a0, as, ras), w1 = quot(a1, as, ras);
     return w0 * p0 + w1 * p1;
 }
@@ -943,10 +975,10 @@ template <typename R> GRHD_HD R weno5(R qm2, R qm1, R q0, R qp1, R qp2)
     const R b1 = c13 * (t1 * t1) + (R)0.25 * (u1 * u1);
     const R b2 = c13 * (t2 * t2) + (R)0.25 * (u2 * u2);
     const R tau5 = fabs(b0 - b2);
-    const R a0 = d0 * ((R)1 + (tau5 / (eps + b0)));
-    const R a1 = d1 * ((R)1 + (tau5 / (eps + b1)));
-    const R a2 = d2 * ((R)1 + (tau5 / (eps + b2)));
-    const R as = a0 + a1 + a2, ras = (R)1 / as;
+    const R a0 = d0 * ((R)1 + rdiv(tau5, eps + b0));
+    const R a1 = d1 * ((R)1 + rdiv(tau5, eps + b1));
+    const R a2 = d2 * ((R)1 + rdiv(tau5, eps + b2));
+    const R as = a0 + a1 + a2, ras = rcp(as);
     const R w0 = quot(a0, as, ras), w1 = quot(a1, as, ras), w2 = quot(a2, as, ras);
     const R p0 = (R)(1.0 / 3.0) * q0 + (R)(5.0 / 6.0) * qp1 - (R)(1.0 / 6.0) * qp2;
     const R p1 = -(R)(1.0 / 6.0) * qm1 + (R)(5.0 / 6.0) * q0 + (R)(1.0 / 3.0) * qp1;
@@ -1011,8 +1043,8 @@ GRHD_HD void recon_face(const Params &P, const R s[NEQ][6], int i, int nodes, in
             rmx = rmax(rmx, S_(EQ_DE, o));
             rmn = rmin(rmn, S_(EQ_DE, o));
         }
-        const R pj = (pmax > (R)1.0e3 * pf) ? (pmax - pmin) / rmax(pmin, pf) : (R)0;
-        const R rj = (rmx > (R)1.0e3 * rf) ? (rmx - rmn) / rmax(rmn, rf) : (R)0;
+        const R pj = (pmax > (R)1.0e3 * pf) ? rdiv(pmax - pmin, rmax(pmin, pf)) : (R)0;
+        const R rj = (rmx > (R)1.0e3 * rf) ? rdiv(rmx - rmn, rmax(rmn, rf)) : (R)0;
         const bool god = radial0 || S_(EQ_DE, 0) <= (R)1.0e2 * rf || S_(EQ_DE, 1) <= (R)1.0e2 * rf ||
                          pj >= (R)10 || rj >= (R)10;
         const bool tvdmc = (pj >= (R)0.80 || rj >= (R)0.50);

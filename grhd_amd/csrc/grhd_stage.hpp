@@ -55,13 +55,13 @@ __device__ __forceinline__ void face_geometry(const Params &P, int i, int j, R *
                       g[0][1] * (g[1][0] * g[2][2] - g[1][2] * g[2][0]) +
                       g[0][2] * (g[1][0] * g[2][1] - g[1][1] * g[2][0]);
         R gii;
-        if (DIR == 1) gii = (g[1][1] * g[2][2] - g[1][2] * g[2][1]) / det;
-        else if (DIR == 2) gii = (g[0][0] * g[2][2] - g[0][2] * g[2][0]) / det;
-        else gii = (g[0][0] * g[1][1] - g[0][1] * g[1][0]) / det;
+        if (DIR == 1) gii = rdiv(g[1][1] * g[2][2] - g[1][2] * g[2][1], det);
+        else if (DIR == 2) gii = rdiv(g[0][0] * g[2][2] - g[0][2] * g[2][0], det);
+        else gii = rdiv(g[0][0] * g[1][1] - g[0][1] * g[1][0], det);
         R cof[3][3], gdd = 0;
         const bool ok = !(alpha <= (R)0 || sqg <= (R)0 || !finite(alpha) || !finite(sqg)) &&
                         face_frame<R, DIR>(g, cof, gdd);
-        const R w = ok ? beta[DIR - 1] / (alpha * sqrt(gdd)) : (R)0;
+        const R w = ok ? rdiv(beta[DIR - 1], alpha * sqrt(gdd)) : (R)0;
         if (ok) flags |= FG_FRAME_OK;
         col[FG_ALPHA * 64] = alpha;
         for (int x = 0; x < 3; x++) col[(FG_BETA + x) * 64] = beta[x];
@@ -86,11 +86,17 @@ __device__ __forceinline__ void flux_face(const Params &P, const double *__restr
     const int lo = (P.ng >= 2) ? -2 : 0, hi = (P.ng >= 3) ? 3 : (P.ng == 2 ? 2 : 1);
     const int64_t step = (DIR == 1) ? 1 : (DIR == 2) ? (int64_t)P.NXs : (int64_t)P.NXs * P.NYs;
     const int64_t base = pindex(P, DIR == 1 ? fi : i, DIR == 2 ? fi : j, DIR == 3 ? fi : k);
+    // Offsets outside the ghost depth read a clamped (valid) address and are
+    // then zeroed, so all 30 loads issue together without branches.
 #pragma unroll
     for (int o = -2; o <= 3; o++) {
         const bool in = (o >= lo && o <= hi);
+        const int64_t off = base + (int64_t)(o < lo ? lo : (o > hi ? hi : o)) * step;
 #pragma unroll
-        for (int v = 0; v < NEQ; v++) s[v][o + 2] = in ? (R)p[v * P.vstride + base + o * step] : (R)0;
+        for (int v = 0; v < NEQ; v++) {
+            const R x = (R)p[v * P.vstride + off];
+            s[v][o + 2] = in ? x : (R)0;
+        }
     }
     R qL[NEQ], qR[NEQ];
     recon_face<R>(P, s, fi, nodes, DIR, qL, qR);
@@ -223,23 +229,40 @@ __device__ __forceinline__ void update_body(const Params &P, const double *__res
 #pragma unroll
         for (int v = 0; v < NEQ; v++) src[v] = 0.0;
     }
-    {
-        const int64_t fa = ((int64_t)k0 * P.ny + j0) * (P.nx + 1) + i0;
+    // Every face value and both RK operands are loaded in one batch, so their
+    // latencies overlap: the addresses of an inactive direction fall back to
+    // the x faces (read and discarded), which keeps the loads branch-free.
+    const bool ya = P.ny > 1, za = P.nz > 1;
+    const int64_t fax = ((int64_t)k0 * P.ny + j0) * (P.nx + 1) + i0;
+    const int64_t fay = ya ? ((int64_t)k0 * (P.ny + 1) + j0) * P.nx + i0 : fax;
+    const R *Fyp = ya ? Fy : Fx, *Fzp = za ? Fz : Fx;
+    const int64_t fsyp = ya ? fsy : fsx, fszp = za ? fsz : fsx;
+    const int64_t sy = ya ? (int64_t)P.nx : 1, faz = za ? t : fax, sz = za ? (int64_t)P.nx * P.ny : 1;
+    R fx0[NEQ], fx1[NEQ], fy0[NEQ], fy1[NEQ], fz0[NEQ], fz1[NEQ];
+    double a0[NEQ], a1[NEQ];
+    const bool rk = stage > 0;
+    const int64_t tr = rk ? t : 0;
+    const double *u0p = rk ? u0 : u;
 #pragma unroll
-        for (int v = 0; v < NEQ; v++)
-            rhs[v] = -quot((double)Fx[v * fsx + fa + 1] - (double)Fx[v * fsx + fa], P.dx, P.inv_dx) + src[v];
+    for (int v = 0; v < NEQ; v++) {
+        fx0[v] = Fx[v * fsx + fax];
+        fx1[v] = Fx[v * fsx + fax + 1];
+        fy0[v] = Fyp[v * fsyp + fay];
+        fy1[v] = Fyp[v * fsyp + fay + sy];
+        fz0[v] = Fzp[v * fszp + faz];
+        fz1[v] = Fzp[v * fszp + faz + sz];
+        a0[v] = u0p[v * ncell + tr];
+        a1[v] = u[v * ncell + tr];
     }
-    if (P.ny > 1) {
-        const int64_t fa = ((int64_t)k0 * (P.ny + 1) + j0) * P.nx + i0;
 #pragma unroll
-        for (int v = 0; v < NEQ; v++)
-            rhs[v] = rhs[v] - quot((double)Fy[v * fsy + fa + P.nx] - (double)Fy[v * fsy + fa], P.dy, P.inv_dy);
+    for (int v = 0; v < NEQ; v++) rhs[v] = -quot((double)fx1[v] - (double)fx0[v], P.dx, P.inv_dx) + src[v];
+    if (ya) {
+#pragma unroll
+        for (int v = 0; v < NEQ; v++) rhs[v] = rhs[v] - quot((double)fy1[v] - (double)fy0[v], P.dy, P.inv_dy);
     }
-    if (P.nz > 1) {
-        const int64_t plane = (int64_t)P.nx * P.ny;
+    if (za) {
 #pragma unroll
-        for (int v = 0; v < NEQ; v++)
-            rhs[v] = rhs[v] - quot((double)Fz[v * fsz + t + plane] - (double)Fz[v * fsz + t], P.dz, P.inv_dz);
+        for (int v = 0; v < NEQ; v++) rhs[v] = rhs[v] - quot((double)fz1[v] - (double)fz0[v], P.dz, P.inv_dz);
     }
     if (stage == 0) {
 #pragma unroll
@@ -249,10 +272,9 @@ __device__ __forceinline__ void update_body(const Params &P, const double *__res
     double c[NEQ];
 #pragma unroll
     for (int v = 0; v < NEQ; v++) {
-        const double a0 = u0[v * ncell + t];
-        if (stage == 1) c[v] = a0 + rhs[v] * dt;
-        else if (stage == 2) c[v] = 0.75 * a0 + 0.25 * (u[v * ncell + t] + rhs[v] * dt);
-        else c[v] = quot(a0 + 2.0 * (u[v * ncell + t] + rhs[v] * dt), 3.0, 1.0 / 3.0);
+        if (stage == 1) c[v] = a0[v] + rhs[v] * dt;
+        else if (stage == 2) c[v] = 0.75 * a0[v] + 0.25 * (a1[v] + rhs[v] * dt);
+        else c[v] = quot(a0[v] + 2.0 * (a1[v] + rhs[v] * dt), 3.0, 1.0 / 3.0);
     }
 #pragma unroll
     for (int v = 0; v < NEQ; v++) u[v * ncell + t] = c[v];
