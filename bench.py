@@ -15,6 +15,7 @@ mixed kinds and sizes, continuously batched on the device, with p50/p99
 time-to-solution and aggregate cell-updates/s ("ensemble").
 
 python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline] [--serve-requests N]
+                [--serve-rates 4,16,64] [--serve-mix quick|full]
 """
 from __future__ import annotations
 
@@ -156,22 +157,37 @@ def cpu_baseline(seconds=15.0, nx=64, ny=32, nz=32):
     }
 
 
-def run_ensemble(n, rate, time_scale, chunk, seed=3435):
+# Served mixes.  "quick": Michel nr 256/512 and KH 128^2/256^2, so that the
+# default run finishes in seconds; "full": the ranges of the serving workload
+# (Michel nr 256-4096, KH 256^2-1024^2; the Michel nr = 4096 runs take about
+# 945k steps each).
+MIXES = {
+    "quick": dict(shock_nx=(200, 400, 800, 1600, 3200, 6400), michel_nr=(256, 512), khi_n=(128, 256)),
+    "full": dict(shock_nx=(200, 400, 800, 1600, 3200, 6400), michel_nr=(256, 512, 1024, 2048, 4096),
+                 khi_n=(256, 512, 1024)),
+}
+
+
+def run_ensemble(n, rate, time_scale, chunk, seed=3435, mix="quick"):
     """Served request mix: Poisson arrivals of independent problems (shock
-    tubes nx 200-6400, Michel nr 256/512 with spin 0 or in [0, 0.99), KH
-    128^2/256^2), continuously batched on the device; p50/p99 time-to-solution
-    and aggregate cell-updates/s."""
+    tubes, Michel accretion with spin 0 or in [0, 0.99), Kelvin-Helmholtz with
+    seeded v^z noise), continuously batched on the device; p50/p99
+    time-to-solution and aggregate cell-updates/s."""
     from grhd_amd.batch import Batch, footprint
     from grhd_amd.serving import Server, request_mix
-    reqs = request_mix(n, rate=rate, seed=seed, time_scale=time_scale)
+    m = MIXES[mix]
+    reqs = request_mix(n, rate=rate, seed=seed, time_scale=time_scale, **m)
     with Batch(capacity=64) as B:
         rep = Server(B, footprint, hbm_budget=64 << 30, chunk=chunk).run(reqs)
         info = B.info()
     s = rep.summary()
+    lo, hi = min(m["shock_nx"]), max(m["shock_nx"])
     s.update({"arrival_rate_per_s": rate, "final_time_scale": time_scale, "chunk_steps": chunk,
               "launch_form": info["variant"], "seed": seed,
-              "mix": "sod/strong_shock/shu_osher nx 200-6400, michel nr 256/512 a=0 or U[0,0.99), "
-                     "kelvin_helmholtz 128^2/256^2 v=+-0.5 Gamma=4/3, each to its preset final_time"})
+              "mix": f"{mix}: sod/strong_shock/shu_osher nx {lo}-{hi}, michel nr "
+                     f"{'/'.join(map(str, m['michel_nr']))} a=0 or U[0,0.99), kelvin_helmholtz "
+                     f"{'/'.join(f'{k}^2' for k in m['khi_n'])} v=+-0.5 Gamma=4/3 with 1e-2 seeded v^z "
+                     "noise, each to its preset final_time"})
     return s
 
 
@@ -255,7 +271,10 @@ def main():
     ap.add_argument("--cpu-child", default=None)
     ap.add_argument("--serve-requests", type=int, default=32,
                     help="requests of the served ensemble (0: skip it)")
-    ap.add_argument("--serve-rate", type=float, default=16.0)
+    ap.add_argument("--serve-rates", default="4,16,64",
+                    help="Poisson arrival rates (requests/s) of the served ensemble, swept in order; "
+                         "the line's 'ensemble' is the one at 16/s (or the first)")
+    ap.add_argument("--serve-mix", choices=sorted(MIXES), default="quick")
     ap.add_argument("--serve-time-scale", type=float, default=1.0)
     ap.add_argument("--serve-chunk", type=int, default=32)
     ap.add_argument("--michel", action="store_true",
@@ -327,7 +346,13 @@ def main():
                                 if "fp64" in res else None),
         }
     if a.serve_requests > 0:
-        line["ensemble"] = run_ensemble(a.serve_requests, a.serve_rate, a.serve_time_scale, a.serve_chunk)
+        rates = [float(r) for r in a.serve_rates.split(",") if r]
+        sweep = [run_ensemble(a.serve_requests, r, a.serve_time_scale, a.serve_chunk, mix=a.serve_mix)
+                 for r in rates]
+        line["ensemble"] = next((e for e in sweep if e["arrival_rate_per_s"] == 16.0), sweep[0])
+        line["ensemble_sweep"] = [{k: e[k] for k in ("arrival_rate_per_s", "p50_s", "p99_s", "mean_wait_s",
+                                                     "cell_updates_per_s", "wall_s", "peak_active")}
+                                  for e in sweep]
     line["cpu_baseline"] = None if a.no_cpu_baseline else cpu_baseline(a.cpu_seconds, cnx, cny, cnz)
     line["counters"] = head["counters"]
     print(json.dumps(line), flush=True)

@@ -500,8 +500,8 @@ int grhd_initial_conditions(grhd_ctx *c)
 
 int grhd_perturb(grhd_ctx *c, int32_t type, double amplitude, uint64_t seed)
 {
-    if (!c || (type != 1 && type != 2) || !(amplitude >= 0.0 && amplitude < 1.0))
-        return c ? fail(c, GRHD_ERR_INVALID_ARGUMENT, "perturbation type must be 1 or 2 and 0 <= A < 1")
+    if (!c || (type < 1 || type > 3) || !(amplitude >= 0.0 && amplitude < 1.0))
+        return c ? fail(c, GRHD_ERR_INVALID_ARGUMENT, "perturbation type must be 1, 2 or 3 and 0 <= A < 1")
                  : GRHD_ERR_INVALID_ARGUMENT;
     RefLayout L(c->P);
     std::vector<double> h(L.psize());

@@ -344,7 +344,10 @@ void perturb(const Params &P, int type, double amplitude, uint64_t seed, std::ve
     // inject_pressure_noise / inject_density_noise (conditions.f90:603-636, 730-768):
     // multiplicative white noise on the cells denser than 5 % of the maximum.
     // The deviate comes from this counter-based generator (the reference uses the
-    // compiler runtime's random_number, whose stream is not portable).
+    // compiler runtime's random_number, whose stream is not portable).  Type 3
+    // (no reference counterpart) adds white noise A (2 U - 1) to v^z on the same
+    // cells: the random part of the Kelvin-Helmholtz seed perturbation, whose
+    // sinusoidal part the initial data carries (conditions.f90:73-103).
     HostGrid G(P);
     double rho_max = -1.0;
     for (int k = 1; k <= P.nz; k++)
@@ -358,6 +361,8 @@ void perturb(const Params &P, int type, double amplitude, uint64_t seed, std::ve
                 const double f = 1.0 + amplitude * (2.0 * uniform01(seed, idx) - 1.0);
                 if (type == 1) {
                     p[G.at(EQ_PR, i, j, k)] *= f;
+                } else if (type == 3) {
+                    p[G.at(EQ_VZ, i, j, k)] += f - 1.0;
                 } else {
                     p[G.at(EQ_DE, i, j, k)] *= f;
                     p[G.at(EQ_PR, i, j, k)] *= std::pow(f, P.gam);

@@ -1017,15 +1017,28 @@ template <typename R> GRHD_HD R tvd_slope(R a, R b, int lim)
     return minmod2(a, b);
 }
 
+// Face schemes compiled as constants: scheme 0 reads the reconstruction, the
+// shock sensor and the Riemann solver from Params at run time (every
+// combination); the others fix them, which removes the untaken branches and
+// their registers from the face kernels of the configurations run most:
+// 1 = MP5 + shock sensor + HLLC, 2 = WENO5 + HLLE without the sensor.
+struct Scheme {
+    int rec, sensor, rs;
+};
+constexpr Scheme kSchemes[] = {{0, 0, 0}, {4, 1, 2}, {5, 0, 1}};
+// ghost depth of a reconstruction (initialization.f90:27-52)
+GRHD_HD constexpr int ghost_depth(int rec) { return rec == 1 ? 1 : (rec == 2 || rec == 3) ? 2 : 3; }
+
 // One face of reconstruct_1d_core (reconstruction.f90:285-657).
 // s[v][o + 2] holds the primitive v at offset o = -2..3 from the face's left
 // cell.  i is the face index (0..nodes) along direction dir (1..3).
-template <typename R>
+template <typename R, int SCH = 0>
 GRHD_HD void recon_face(const Params &P, const R s[NEQ][6], int i, int nodes, int dir,
                         R qL[NEQ], R qR[NEQ])
 {
 #define S_(v, o) s[v][(o) + 2]
-    const int rec = P.rec;
+    const int rec = SCH ? kSchemes[SCH].rec : P.rec;
+    const bool sensor = SCH ? (kSchemes[SCH].sensor != 0) : (P.sensor != 0);
     const bool mink = (P.mk <= MK_MINK_SPH_LOG);
     const R rf = (R)P.rho_floor, pf = (R)P.p_floor;
     const bool radial0 = (!mink && dir == 1 && i <= 0);
@@ -1035,7 +1048,7 @@ GRHD_HD void recon_face(const Params &P, const R s[NEQ][6], int i, int nodes, in
             qL[v] = S_(v, 0);
             qR[v] = S_(v, 1);
         }
-    } else if (P.sensor) {
+    } else if (sensor) {
         R pmax = S_(EQ_PR, -2), pmin = S_(EQ_PR, -2), rmx = S_(EQ_DE, -2), rmn = S_(EQ_DE, -2);
         for (int o = -1; o <= 2; o++) {
             pmax = rmax(pmax, S_(EQ_PR, o));

@@ -77,13 +77,14 @@ __device__ __forceinline__ void face_geometry(const Params &P, int i, int j, R *
     col[FG_FLAGS * 64] = (R)flags;
 }
 
-template <int MK, int DIR, typename R>
+template <int MK, int DIR, typename R, int SCH = 0>
 __device__ __forceinline__ void flux_face(const Params &P, const double *__restrict__ p,
                                           const FaceGeo<R> &f, int i, int j, int k, int fi, int nodes,
                                           R out[NEQ])
 {
     R s[NEQ][6];
-    const int lo = (P.ng >= 2) ? -2 : 0, hi = (P.ng >= 3) ? 3 : (P.ng == 2 ? 2 : 1);
+    const int ng = SCH ? ghost_depth(kSchemes[SCH].rec) : P.ng;
+    const int lo = (ng >= 2) ? -2 : 0, hi = (ng >= 3) ? 3 : (ng == 2 ? 2 : 1);
     const int64_t step = (DIR == 1) ? 1 : (DIR == 2) ? (int64_t)P.NXs : (int64_t)P.NXs * P.NYs;
     const int64_t base = pindex(P, DIR == 1 ? fi : i, DIR == 2 ? fi : j, DIR == 3 ? fi : k);
     // Offsets outside the ghost depth read a clamped (valid) address and are
@@ -99,8 +100,8 @@ __device__ __forceinline__ void flux_face(const Params &P, const double *__restr
         }
     }
     R qL[NEQ], qR[NEQ];
-    recon_face<R>(P, s, fi, nodes, DIR, qL, qR);
-    riemann<R, DIR, kSRHD<MK>>(P.rs, (R)P.g1, (R)P.gam, qL, qR, f, out);
+    recon_face<R, SCH>(P, s, fi, nodes, DIR, qL, qR);
+    riemann<R, DIR, kSRHD<MK>>(SCH ? kSchemes[SCH].rs : P.rs, (R)P.g1, (R)P.gam, qL, qR, f, out);
 }
 
 // Tiling of the face kernels: a tile is 64 consecutive faces of one (x, y)
@@ -120,7 +121,7 @@ __host__ __device__ inline FluxTiling flux_tiling(int nfz, int gw_pref = 4)
 
 // Faces of logical block blk of one problem (0 <= blk < flux_blocks()).
 // lds holds (waves / gw) * NFG * 64 values of R, waves = blockDim.x / 64.
-template <int MK, int DIR, typename R>
+template <int MK, int DIR, typename R, int SCH = 0>
 __device__ __forceinline__ void flux_body(const Params &P, const double *__restrict__ p,
                                           R *__restrict__ F, int64_t fstride, int64_t blk, int kt,
                                           int gw, R *lds)
@@ -161,7 +162,7 @@ __device__ __forceinline__ void flux_body(const Params &P, const double *__restr
 #pragma unroll
             for (int v = 0; v < NEQ; v++) out[v] = 0;
         } else {
-            flux_face<MK, DIR, R>(P, p, fg, i, j, k, fi, nodes, out);
+            flux_face<MK, DIR, R, SCH>(P, p, fg, i, j, k, fi, nodes, out);
         }
         const int64_t fo = (int64_t)c * nab + ab;
 #pragma unroll
@@ -182,22 +183,44 @@ __host__ __device__ inline int64_t flux_blocks(const Params &P, FluxTiling t, in
 
 // Four waves per SIMD: the compiler keeps the face kernels at 128 VGPRs with a
 // few doubles of scratch spill, measured faster than 151 VGPRs at three waves.
-template <int MK, int DIR, typename R>
+template <int MK, int DIR, typename R, int SCH>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_flux(
     Params P, const double *__restrict__ p, R *__restrict__ F, int64_t fstride, int kt, int gw)
 {
     extern __shared__ double lds_geo[];
-    flux_body<MK, DIR, R>(P, p, F, fstride, xcd_block(blockIdx.x, gridDim.x), kt, gw,
-                          reinterpret_cast<R *>(lds_geo));
+    flux_body<MK, DIR, R, SCH>(P, p, F, fstride, xcd_block(blockIdx.x, gridDim.x), kt, gw,
+                               reinterpret_cast<R *>(lds_geo));
 }
 
-template <int MK, int DIR, typename R>
+template <int MK, int DIR, typename R, int SCH>
 void launch_flux(const Params &P, hipStream_t s, const double *p, R *F, int64_t fstr)
 {
     const int nfz = P.nz + (DIR == 3);
     const FluxTiling t = flux_tiling(nfz);
     const size_t lds = (size_t)(4 / t.gw) * NFG * 64 * sizeof(R);
-    k_flux<MK, DIR, R><<<(unsigned)flux_blocks<DIR>(P, t), 256, lds, s>>>(P, p, F, fstr, t.kt, t.gw);
+    k_flux<MK, DIR, R, SCH><<<(unsigned)flux_blocks<DIR>(P, t), 256, lds, s>>>(P, p, F, fstr, t.kt, t.gw);
+}
+
+// The compiled face scheme for a problem: the Kerr-Schild torus
+// configurations get their own kernels, everything else the run-time one.
+template <int MK>
+inline int face_scheme(const Params &P)
+{
+    if constexpr (MK == MK_KS || MK == MK_KS_LOG) {
+        for (int sch = 1; sch < (int)(sizeof(kSchemes) / sizeof(kSchemes[0])); sch++)
+            if (P.rec == kSchemes[sch].rec && (P.sensor != 0) == (kSchemes[sch].sensor != 0) &&
+                P.rs == kSchemes[sch].rs)
+                return sch;
+    }
+    return 0;
+}
+
+template <int MK, typename R, int SCH>
+void launch_faces(const Params &P, hipStream_t s, const double *p, R *Fx, R *Fy, R *Fz, const int64_t fstr[3])
+{
+    launch_flux<MK, 1, R, SCH>(P, s, p, Fx, fstr[0]);
+    if (P.ny > 1) launch_flux<MK, 2, R, SCH>(P, s, p, Fy, fstr[1]);
+    if (P.nz > 1) launch_flux<MK, 3, R, SCH>(P, s, p, Fz, fstr[2]);
 }
 
 // --------------------------------------------------------- cell update --
@@ -293,9 +316,8 @@ __global__ __launch_bounds__(256) void k_update(Params P, const double *__restri
 }
 
 // cons_to_prim after the RK combination (grhd2.f90:252-260) of cell t: a
-// serial, divide-heavy root solve per cell, kept apart from the sources so
-// that it runs at a higher occupancy.  WARM starts the root solve from the
-// cell's previous Lorentz factor (mixed-precision path).
+// serial, divide-heavy root solve per cell.  WARM starts the root solve from
+// the cell's previous Lorentz factor (mixed-precision path).
 template <int MK, bool WARM>
 __device__ __forceinline__ void c2p_body(const Params &P, double *__restrict__ p,
                                          double *__restrict__ u,
@@ -332,15 +354,6 @@ __device__ __forceinline__ void c2p_body(const Params &P, double *__restrict__ p
     }
 }
 
-template <int MK, bool WARM>
-__global__ __launch_bounds__(256) void k_c2p_stage(Params P, double *__restrict__ p,
-                                                   double *__restrict__ u,
-                                                   unsigned long long *__restrict__ counters)
-{
-    const int64_t t = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    if (t >= (int64_t)P.nx * P.ny * P.nz) return;
-    c2p_body<MK, WARM>(P, p, u, counters, t);
-}
 
 // All face fluxes of a stage in one launch: logical blocks [0, nb[0]) are x
 // faces, then nb[1] blocks of y faces and nb[2] of z faces.
@@ -369,8 +382,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
 }
 
 // Cell update and con2prim of the same cell in one thread (stages 1..3).
+// Held to 128 VGPRs (four waves per SIMD): the kernel streams about 440 B per
+// cell, and the extra wave hides more of that latency than the few spilled
+// values cost (FP32 faces: 144 VGPRs and three waves otherwise).
 template <int MK, typename R>
-__global__ __launch_bounds__(256) void k_cell(Params P, double *__restrict__ p, const R *__restrict__ Fx,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_cell(Params P, double *__restrict__ p, const R *__restrict__ Fx,
                                               const R *__restrict__ Fy, const R *__restrict__ Fz,
                                               int64_t fsx, int64_t fsy, int64_t fsz,
                                               const double *__restrict__ u0, double *__restrict__ u,
@@ -382,12 +398,11 @@ __global__ __launch_bounds__(256) void k_cell(Params P, double *__restrict__ p, 
     c2p_body<MK, sizeof(R) == sizeof(float)>(P, p, u, counters, t);
 }
 
-// fused: one face launch for every direction and update + con2prim in one
-// cell kernel (3 launches per stage, for small grids where launch latency
-// dominates); otherwise one face launch per direction and separate update and
-// con2prim kernels -- measured faster on large grids, where the three face
-// code paths sharing a CU's instruction cache and the 142-VGPR fused cell
-// kernel cost more than the launches.
+// fused: one face launch for every direction (for small grids, where launch
+// latency dominates); otherwise one face launch per direction -- measured
+// faster on large grids, where the three face code paths share a CU's
+// instruction cache.  Either way the flux divergence, RK combination and
+// con2prim of a cell run in one thread of one cell kernel.
 template <int MK, typename R>
 void launch_stage_mk(const Params &P, hipStream_t s, double *p, void *const F[3], const int64_t fstr[3],
                      const double *u0, double *u, int stage, double dt, unsigned long long *counters,
@@ -411,22 +426,25 @@ void launch_stage_mk(const Params &P, hipStream_t s, double *p, void *const F[3]
         k_flux3<MK, R><<<(unsigned)(f.nb[0] + f.nb[1] + f.nb[2]), bs, lds, s>>>(P, p, Fx, Fy, Fz, fstr[0],
                                                                                 fstr[1], fstr[2], f);
     } else {
-        launch_flux<MK, 1, R>(P, s, p, Fx, fstr[0]);
-        if (P.ny > 1) launch_flux<MK, 2, R>(P, s, p, Fy, fstr[1]);
-        if (P.nz > 1) launch_flux<MK, 3, R>(P, s, p, Fz, fstr[2]);
+        const int sch = face_scheme<MK>(P);
+        if constexpr (MK == MK_KS || MK == MK_KS_LOG) {
+            if (sch == 1) launch_faces<MK, R, 1>(P, s, p, Fx, Fy, Fz, fstr);
+            else if (sch == 2) launch_faces<MK, R, 2>(P, s, p, Fx, Fy, Fz, fstr);
+            else launch_faces<MK, R, 0>(P, s, p, Fx, Fy, Fz, fstr);
+        } else {
+            launch_faces<MK, R, 0>(P, s, p, Fx, Fy, Fz, fstr);
+        }
     }
-    // FP64 faces: update and con2prim in one kernel at every size (u does not
-    // go through HBM between them).  FP32 faces on large grids: the separate
-    // warm-started con2prim kernel is faster.
-    if (stage > 0 && (fused || sizeof(R) == sizeof(double))) {
+    // Update and con2prim in one kernel at every size: u does not go through
+    // HBM between them (80 B per cell of traffic saved).
+    if (stage > 0) {
         k_cell<MK, R><<<blocks_for(ncell, bs), bs, 0, s>>>(P, p, Fx, Fy, Fz, fstr[0], fstr[1], fstr[2], u0, u,
                                                            stage, dt, counters);
         return;
     }
+    // stage 0: the rhs alone (parity entry point)
     k_update<MK, R><<<blocks_for(ncell, bs), bs, 0, s>>>(P, p, Fx, Fy, Fz, fstr[0], fstr[1], fstr[2], u0, u,
                                                      stage, dt);
-    if (stage > 0)
-        k_c2p_stage<MK, sizeof(R) == sizeof(float)><<<blocks_for(ncell, bs), bs, 0, s>>>(P, p, u, counters);
 }
 
 // Instantiates the stage launcher of one face arithmetic for every metric kind.

@@ -176,13 +176,14 @@ def _kelvin_helmholtz(n: int) -> RunConfig:
 
 def request_mix(n: int, rate: float, seed: int = 3435, time_scale: float = 1.0,
                 shock_nx=(200, 400, 800, 1600, 3200, 6400), michel_nr=(256, 512),
-                khi_n=(128, 256)) -> list[Request]:
+                khi_n=(128, 256), khi_noise: float = 1.0e-2) -> list[Request]:
     """n requests with Poisson arrivals (rate per second), drawn uniformly over
     three kinds: a shock tube (sod, strong_shock or shu_osher) with nx from
     shock_nx; Michel accretion with nr from michel_nr and spin in [0, 0.99)
     (Kerr-Schild spheroidal when a > 0); Kelvin-Helmholtz with N x N cells
-    from khi_n, v = +-0.5, Gamma = 4/3.  Each runs to its preset final time
-    times time_scale."""
+    from khi_n, v = +-0.5, Gamma = 4/3, whose sinusoidal seed perturbation
+    gets seeded white noise of amplitude khi_noise on v^z at admission.  Each
+    runs to its preset final time times time_scale."""
     rng = np.random.default_rng(seed)
     t = 0.0
     out = []
@@ -205,7 +206,8 @@ def request_mix(n: int, rate: float, seed: int = 3435, time_scale: float = 1.0,
             label = f"kelvin_helmholtz {nk}x{nk}"
         if time_scale != 1.0:
             cfg = cfg.replace(final_time=cfg.final_time * time_scale)
-        out.append(Request(rid, cfg, t, label))
+        noise = (3, khi_noise, seed * 1000003 + rid) if kind == 2 and khi_noise > 0.0 else None
+        out.append(Request(rid, cfg, t, label, perturb=noise))
     return out
 
 

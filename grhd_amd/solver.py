@@ -91,7 +91,8 @@ class Solver:
         self._chk(self._L.grhd_initial_conditions(self.h))
 
     def perturb(self, kind: int, amplitude: float, seed: int):
-        """Seeded multiplicative noise: kind 1 pressure, 2 isentropic density."""
+        """Seeded noise: kind 1 pressure, 2 isentropic density (multiplicative),
+        3 additive v^z (Kelvin-Helmholtz random seed)."""
         self._chk(self._L.grhd_perturb(self.h, int(kind), float(amplitude), int(seed)))
 
     def set_primitives(self, p: np.ndarray):

@@ -131,11 +131,13 @@ int grhd_dims(const grhd_ctx *ctx, int32_t *dims);
  * the interior (grhd2.f90:126-134) and the Michel injector cache
  * (conditions.f90:229-246). */
 int grhd_initial_conditions(grhd_ctx *ctx);
-/* Multiplicative white noise on the cells denser than 5 % of the maximum
- * density, followed by U = P2C(p): type 1 pressure noise
+/* White noise on the cells denser than 5 % of the maximum density,
+ * followed by U = P2C(p): type 1 pressure noise
  * (inject_pressure_noise, conditions.f90:603-636), type 2 isentropic density
- * noise (inject_density_noise, conditions.f90:730-768).  The deviates come
- * from a counter-based generator keyed by (seed, cell index); the reference's
+ * noise (inject_density_noise, conditions.f90:730-768); type 3 additive
+ * noise A (2 U - 1) on v^z (no reference counterpart: the random part of the
+ * Kelvin-Helmholtz seed perturbation).  The deviates come from a
+ * counter-based generator keyed by (seed, cell index); the reference's
  * compiler-runtime random_number stream is not portable. */
 int grhd_perturb(grhd_ctx *ctx, int32_t type, double amplitude, uint64_t seed);
 /* Polytropic constant K of the Fishbone-Moncrief torus that puts rho = 1 at

@@ -157,15 +157,22 @@ def test_mixed_precision_error_is_stated():
     assert l1[0] <= 1e-5, f"mixed density L1 {l1[0]}"
 
 
-def test_large_grid_one_step_matches_oracle():
-    """Grids above 65,536 cells take the per-direction stage form (one face
-    launch per direction, separate update and con2prim) that the 3D torus
-    benchmark runs; smaller grids the fused form the other tests cover."""
-    text = CASES["fm3d_ks_mp5_hllc_sensor"].replace("nx = 24", "nx = 64").replace(
-        "ny = 16", "ny = 32").replace("nz = 8", "nz = 40")
+def _large(name):
+    text = CASES[name].replace("nx = 24", "nx = 64").replace("nx = 20", "nx = 64").replace(
+        "ny = 16", "ny = 32").replace("ny = 12", "ny = 32").replace("nz = 8", "nz = 40")
     cfg = config_from_par(text)
     assert cfg.nx * cfg.ny * cfg.nz > 65536
-    print("large grid one step", cfg.nx, cfg.ny, cfg.nz, flush=True)
+    return cfg
+
+
+@pytest.mark.parametrize("name", ["fm3d_ks_mp5_hllc_sensor", "fm3d_ks_weno5_hlle"])
+def test_large_grid_one_step_matches_oracle(name):
+    """Grids above 65,536 cells take the per-direction stage form (one face
+    launch per direction, the cell kernel) that the 3D torus benchmark runs,
+    with the face kernels compiled for the scheme (MP5 + sensor + HLLC, WENO5
+    + HLLE); smaller grids the fused form the other tests cover."""
+    cfg = _large(name)
+    print("large grid one step", name, cfg.nx, cfg.ny, cfg.nz, flush=True)
     with Solver(cfg) as S:
         O = oracle.Oracle(cfg)
         S.initial_conditions()
@@ -185,3 +192,26 @@ def test_large_grid_one_step_matches_oracle():
     # cascade), which shows in the Linf of the small S_theta: 2e-12 of its
     # maximum.  Linf is held to 1e-11.
     assert np.all(l1 <= TOL_STEP) and np.all(li <= 1e-11), (l1, li)
+
+
+def test_large_grid_mixed_precision_error_is_stated():
+    """Mixed path on the per-direction stage form (FP32 face kernels compiled
+    for MP5 + sensor + HLLC, FP64 cell kernel): L1 density error vs the FP64
+    oracle after 10 steps of the noisy 3D torus on 82k cells stays below 1e-5."""
+    cfg = _large("fm3d_ks_mp5_hllc_sensor").replace(precision=1)
+    print("large grid mixed 10 steps", cfg.nx, cfg.ny, cfg.nz, flush=True)
+    S = Solver(cfg)
+    O = oracle.Oracle(cfg)
+    S.initial_conditions()
+    p = seeded_noise(S.get_primitives(), S.nx, S.ny, S.nz, S.ng, 2, 1.0e-2, 3435)
+    S.set_primitives(p)
+    u_o = O.prim_to_cons(p)
+    p_o = p.copy()
+    S.advance(1.0, 10)
+    O.run(u_o, p_o, 10, t=0.0, t_final=1.0)
+    m = read_set_mask(S.nx, S.ny, S.nz, S.ng)
+    l1, li = rel_errors(S.get_primitives()[m], p_o[m])
+    S.close()
+    O.close()
+    print("  mixed L1", l1, flush=True)
+    assert l1[0] <= 1e-5, f"mixed density L1 {l1[0]}"

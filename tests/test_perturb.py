@@ -3,7 +3,9 @@ inject_pressure_noise / inject_density_noise (conditions.f90:603-636,
 730-768): only interior cells with rho > 0.05 rho_max change; the factor
 f = 1 + A (2 U - 1) lies in [1 - A, 1 + A]; pressure noise scales p by f;
 density noise scales rho by f and p by f^Gamma; velocities and ghost cells
-are untouched; U is rebuilt from the perturbed primitives.  The deviates come
+are untouched; U is rebuilt from the perturbed primitives.  Type 3 (no
+reference counterpart) adds A (2 U - 1) to v^z of the same cells, the random
+part of the Kelvin-Helmholtz seed perturbation.  The deviates come
 from a counter-based generator keyed by (seed, cell) -- the reference's
 runtime random_number stream is not reproduced (parity unpinned for the
 deviates themselves) -- so the same seed gives the same field and different
@@ -72,3 +74,24 @@ def test_seeded_and_reproducible():
     _, c, _, _, _, _ = _perturbed(2, 3436)
     assert np.array_equal(a, b)
     assert not np.array_equal(a, c)
+
+
+def test_transverse_velocity_noise():
+    print("perturb kind 3 on khi_weno3_sensor", flush=True)
+    p0, p1, u1, u_ref, ng, _ = _perturbed(3, 3435, "khi_weno3_sensor")
+    i0, i1 = _interior(p0, ng), _interior(p1, ng)
+    rho0 = i0[..., 0]
+    hot = rho0 > 0.05 * rho0.max()
+    assert hot.any()
+    assert np.array_equal(i1[~hot], i0[~hot])
+    assert np.array_equal(i1[..., :4], i0[..., :4])
+    d = i1[..., 4][hot] - i0[..., 4][hot]
+    assert np.all(np.abs(d) <= A * (1 + 1e-12))
+    assert np.std(d) > 0.3 * A / np.sqrt(3)
+    # ghosts of the active x and z axes untouched (the case has ny = 1: the
+    # reference layout's y ghost slots hold copies of the single row)
+    row0, row1 = p0[:, ng + 1], p1[:, ng + 1]
+    mask = np.ones(row0.shape[:2], bool)
+    mask[ng + 1:-ng, ng + 1:-ng] = False
+    assert np.array_equal(row1[mask], row0[mask])
+    assert np.array_equal(u1, u_ref)

@@ -115,6 +115,11 @@ def test_request_mix_is_seeded_and_valid():
     assert all(r2.arrival > r1.arrival for r1, r2 in zip(a, a[1:]))
     kinds = {r.label.split()[0] for r in a}
     assert {"michel", "kelvin_helmholtz"} & kinds
+    # Kelvin-Helmholtz requests carry seeded v^z noise (type 3), distinct per request
+    khi = [r for r in a if r.label.startswith("kelvin_helmholtz")]
+    assert khi and all(r.perturb is not None and r.perturb[:2] == (3, 1.0e-2) for r in khi)
+    assert len({r.perturb[2] for r in khi}) == len(khi)
+    assert all(r.perturb is None for r in a if not r.label.startswith("kelvin_helmholtz"))
 
 
 @pytest.mark.gpu
@@ -135,6 +140,8 @@ def test_served_results_equal_alone():
     for c in rep.completed:
         with Solver(reqs[c.rid].cfg) as S:
             S.initial_conditions()
+            if reqs[c.rid].perturb is not None:
+                S.perturb(*reqs[c.rid].perturb)
             n, _ = S.advance()
             assert n == c.steps, c.label
             assert np.array_equal(S.get_conserved(), c.state), c.label
