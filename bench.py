@@ -168,7 +168,11 @@ MIXES = {
 }
 
 
-def run_ensemble(n, rate, time_scale, chunk, seed=3435, mix="quick"):
+def _progress(c):
+    print(f"served {c.label}: {c.steps} steps, latency {c.latency:.2f} s", file=sys.stderr, flush=True)
+
+
+def run_ensemble(n, rate, time_scale, chunk, seed=3435, mix="quick", progress=False):
     """Served request mix: Poisson arrivals of independent problems (shock
     tubes, Michel accretion with spin 0 or in [0, 0.99), Kelvin-Helmholtz with
     seeded v^z noise), continuously batched on the device; p50/p99
@@ -178,7 +182,8 @@ def run_ensemble(n, rate, time_scale, chunk, seed=3435, mix="quick"):
     m = MIXES[mix]
     reqs = request_mix(n, rate=rate, seed=seed, time_scale=time_scale, **m)
     with Batch(capacity=64) as B:
-        rep = Server(B, footprint, hbm_budget=64 << 30, chunk=chunk).run(reqs)
+        rep = Server(B, footprint, hbm_budget=64 << 30, chunk=chunk,
+                     on_complete=_progress if progress else None).run(reqs)
         info = B.info()
     s = rep.summary()
     lo, hi = min(m["shock_nx"]), max(m["shock_nx"])
@@ -275,6 +280,10 @@ def main():
                     help="Poisson arrival rates (requests/s) of the served ensemble, swept in order; "
                          "the line's 'ensemble' is the one at 16/s (or the first)")
     ap.add_argument("--serve-mix", choices=sorted(MIXES), default="quick")
+    ap.add_argument("--serve-progress", action="store_true",
+                    help="print each served request to stderr as it completes")
+    ap.add_argument("--serve-only", action="store_true",
+                    help="only the served ensemble (no torus throughput, no CPU baseline)")
     ap.add_argument("--serve-time-scale", type=float, default=1.0)
     ap.add_argument("--serve-chunk", type=int, default=32)
     ap.add_argument("--michel", action="store_true",
@@ -296,6 +305,11 @@ def main():
         print(f"bench.py: this build measures one GPU only (asked for --gpus {a.gpus}); "
               "multi-GPU sharding is designed (radial slabs) but not built", file=sys.stderr)
         sys.exit(2)
+    if a.serve_only:
+        for r in (float(r) for r in a.serve_rates.split(",") if r):
+            print(json.dumps(run_ensemble(a.serve_requests, r, a.serve_time_scale, a.serve_chunk,
+                                          mix=a.serve_mix, progress=a.serve_progress)), flush=True)
+        return
     nx, ny, nz = (int(v) for v in a.grid.split(","))
     cfg = c4_config(nx, ny, nz)
     res = {}
@@ -347,7 +361,8 @@ def main():
         }
     if a.serve_requests > 0:
         rates = [float(r) for r in a.serve_rates.split(",") if r]
-        sweep = [run_ensemble(a.serve_requests, r, a.serve_time_scale, a.serve_chunk, mix=a.serve_mix)
+        sweep = [run_ensemble(a.serve_requests, r, a.serve_time_scale, a.serve_chunk, mix=a.serve_mix,
+                              progress=a.serve_progress)
                  for r in rates]
         line["ensemble"] = next((e for e in sweep if e["arrival_rate_per_s"] == 16.0), sweep[0])
         line["ensemble_sweep"] = [{k: e[k] for k in ("arrival_rate_per_s", "p50_s", "p99_s", "mean_wait_s",

@@ -263,9 +263,11 @@ __device__ __forceinline__ void update_body(const Params &P, const double *__res
     const int64_t sy = ya ? (int64_t)P.nx : 1, faz = za ? t : fax, sz = za ? (int64_t)P.nx * P.ny : 1;
     R fx0[NEQ], fx1[NEQ], fy0[NEQ], fy1[NEQ], fz0[NEQ], fz1[NEQ];
     double a0[NEQ], a1[NEQ];
+    // stage 1 does not read u (its a1 loads repeat the u0 addresses and hit
+    // the cache); stage 0 reads neither operand
     const bool rk = stage > 0;
     const int64_t tr = rk ? t : 0;
-    const double *u0p = rk ? u0 : u;
+    const double *u0p = rk ? u0 : u, *u1p = (stage >= 2) ? u : u0p;
 #pragma unroll
     for (int v = 0; v < NEQ; v++) {
         fx0[v] = Fx[v * fsx + fax];
@@ -275,7 +277,7 @@ __device__ __forceinline__ void update_body(const Params &P, const double *__res
         fz0[v] = Fzp[v * fszp + faz];
         fz1[v] = Fzp[v * fszp + faz + sz];
         a0[v] = u0p[v * ncell + tr];
-        a1[v] = u[v * ncell + tr];
+        a1[v] = u1p[v * ncell + tr];
     }
 #pragma unroll
     for (int v = 0; v < NEQ; v++) rhs[v] = -quot((double)fx1[v] - (double)fx0[v], P.dx, P.inv_dx) + src[v];

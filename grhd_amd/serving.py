@@ -88,8 +88,9 @@ class Report:
 
 class Server:
     def __init__(self, batch, footprint, hbm_budget: int, chunk: int = 32, clock=time.perf_counter,
-                 sleep=time.sleep):
+                 sleep=time.sleep, on_complete=None):
         self.batch = batch
+        self.on_complete = on_complete     # called with each Completed record
         self.footprint = footprint
         self.budget = int(hbm_budget)
         self.chunk = int(chunk)
@@ -147,6 +148,8 @@ class Server:
                 used -= need
                 rep.completed.append(Completed(r.rid, r.label, r.arrival, admitted, self.clock() - t0, steps,
                                                cells, state))
+                if self.on_complete is not None:
+                    self.on_complete(rep.completed[-1])
         rep.wall = self.clock() - t0
         return rep
 
