@@ -107,6 +107,9 @@ _SIGS = {
     "grhd_rhs": ([ctypes.c_void_p, ctypes.POINTER(_D)], ctypes.c_int),
     "grhd_con2prim": ([ctypes.c_void_p], ctypes.c_int),
     "grhd_counters": ([ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64)], ctypes.c_int),
+    "grhd_total_mass": ([ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
+    "grhd_convergence_norms": ([ctypes.c_void_p, ctypes.c_double, ctypes.POINTER(ctypes.c_double),
+                                ctypes.POINTER(ctypes.c_int32)], ctypes.c_int),
     "grhd_set_stream": ([ctypes.c_void_p, ctypes.c_void_p], ctypes.c_int),
     "grhd_synchronize": ([ctypes.c_void_p], ctypes.c_int),
     "grhd_last_timing": ([ctypes.c_void_p, ctypes.POINTER(_D), ctypes.POINTER(ctypes.c_int64)],

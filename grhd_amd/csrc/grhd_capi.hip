@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cfloat>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -676,6 +677,79 @@ int grhd_con2prim(grhd_ctx *c)
     launch_c2p(c->P, c->stream, c->p, c->u[c->ucur], c->counters);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    return GRHD_OK;
+}
+
+int grhd_total_mass(grhd_ctx *c, double *mass)
+{
+    if (!c || !mass) return GRHD_ERR_INVALID_ARGUMENT;
+    const int nb = launch_mass(c->P, c->stream, c->u[c->ucur], c->partial, c->partial_n * 4);
+    HIPCHK(c, hipGetLastError());
+    std::vector<double> h((size_t)nb);
+    HIPCHK(c, hipMemcpyAsync(h.data(), c->partial, h.size() * sizeof(double), hipMemcpyDeviceToHost,
+                             c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    double m = 0.0;
+    for (int b = 0; b < nb; b++) m = m + h[(size_t)b];
+    *mass = m;
+    return GRHD_OK;
+}
+
+int grhd_convergence_norms(grhd_ctx *c, double t, double *norms, int32_t *imax)
+{
+    if (!c || !norms) return GRHD_ERR_INVALID_ARGUMENT;
+    const Params &P = c->P;
+    if (P.problem != 6 && P.problem != 7 && P.problem != 8)
+        return fail(c, GRHD_ERR_UNSUPPORTED,
+                    "convergence norms exist for the advected wave, Michel and dust problems only");
+    // row j = max(1, ny / 2), k = 1 of rho (output.f90:1166); one contiguous
+    // run of the SoA density plane
+    const int jr = std::max(1, P.ny / 2);
+    const int js = (P.ny > 1) ? jr : 1, ks = 1;
+    std::vector<double> rho((size_t)P.nx);
+    HIPCHK(c, hipMemcpyAsync(rho.data(), c->p + pindex(P, 1, js, ks), rho.size() * sizeof(double),
+                             hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    std::vector<double> exact((size_t)P.nx);
+    int i0 = 1, i1 = P.nx;
+    if (P.problem == 6) {
+        // exact cell averages of the advected wave at time t (output.f90:1166-1185)
+        const double pi = M_PI;
+        const double arg = 0.5 * c->cfg.convergence_wave_number * pi * P.dx;
+        const double sinc = (std::fabs(arg) > DBL_EPSILON) ? std::sin(arg) / arg : 1.0;
+        for (int i = 1; i <= P.nx; i++) {
+            const double xa = xc(P, i) - c->cfg.convergence_velocity * t;
+            exact[(size_t)(i - 1)] = c->cfg.convergence_density +
+                                     c->cfg.convergence_amplitude * sinc *
+                                         std::sin(c->cfg.convergence_wave_number * pi * xa);
+        }
+    } else {
+        // Michel (the initial data, output.f90:1186-1211) and dust (the
+        // stationary free fall, 1212-1235): cells int(0.1 nx)..int(0.95 nx)
+        std::vector<double> h;
+        const int rc = initial_conditions(P, c->cfg, h, c->err);
+        if (rc != GRHD_OK) return rc;
+        const RefLayout L(P);
+        for (int i = 1; i <= P.nx; i++) exact[(size_t)(i - 1)] = h[L.p(EQ_DE, i, jr, 1)];
+        i0 = std::max(1, (int)(0.10 * (double)P.nx));
+        i1 = std::min(P.nx, std::max(i0, (int)(0.95 * (double)P.nx)));
+    }
+    double s1 = 0.0, s2 = 0.0, linf = 0.0;
+    int im = i0;
+    for (int i = i0; i <= i1; i++) {
+        const double diff = std::fabs(rho[(size_t)(i - 1)] - exact[(size_t)(i - 1)]);
+        s1 = s1 + diff;
+        s2 = s2 + diff * diff;
+        if (diff > linf) {
+            linf = diff;
+            im = i;
+        }
+    }
+    const double cells = (double)(i1 - i0 + 1);
+    norms[0] = s1 / cells;
+    norms[1] = std::sqrt(s2 / cells);
+    norms[2] = linf;
+    if (imax) *imax = im;
     return GRHD_OK;
 }
 

@@ -7,6 +7,8 @@
 // solver are uniform runtime branches.  This unit instantiates the FP64 stage
 // (compiled without FMA contraction) and the boundary, conversion and CFL
 // kernels.
+#include <algorithm>
+
 #include "grhd_stage.hpp"
 #include "grhd_bound.hpp"
 
@@ -63,6 +65,34 @@ __global__ __launch_bounds__(256) void k_boundaries(Params P, double *__restrict
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= ghost_cells(P)) return;
     bound_body<MK>(P, p, injector, t);
+}
+
+// --------------------------------------------------------- mass monitor --
+// Mass monitor (grhd2.f90:283-301): D dx dy dz summed over the interior, a
+// grid-stride sum per thread, wave64 shuffle, then the block's waves through
+// LDS; one partial per block, added on the host in block order.  D is the
+// first plane of the SoA conserved array, so the reads are coalesced.
+__global__ __launch_bounds__(256) void k_mass(Params P, const double *__restrict__ D, int64_t ncell,
+                                              double *__restrict__ partial)
+{
+    double acc = 0.0;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < ncell;
+         t += (int64_t)gridDim.x * blockDim.x)
+        acc = acc + D[t] * P.dx * P.dy * P.dz;
+    for (int off = 32; off > 0; off >>= 1) acc = acc + __shfl_xor(acc, off, 64);
+    __shared__ double red[4];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) red[wave] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) partial[blockIdx.x] = ((red[0] + red[1]) + red[2]) + red[3];
+}
+
+int launch_mass(const Params &P, hipStream_t s, const double *D, double *partial, int max_blocks)
+{
+    const int64_t ncell = (int64_t)P.nx * P.ny * P.nz;
+    const int nb = (int)std::max<int64_t>(1, std::min<int64_t>(max_blocks, (ncell + 255) / 256));
+    k_mass<<<nb, 256, 0, s>>>(P, D, ncell, partial);
+    return nb;
 }
 
 // ------------------------------------------------------------------ CFL --

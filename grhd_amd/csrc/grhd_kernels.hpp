@@ -30,5 +30,7 @@ void launch_p2c(const Params &P, hipStream_t s, const double *p, double *u);
 void launch_c2p(const Params &P, hipStream_t s, double *p, double *u, unsigned long long *counters);
 // Returns the number of partial records (4 doubles each) written.
 int launch_cfl(const Params &P, hipStream_t s, double *partial);
+// block partial sums of the mass monitor (at most max_blocks); returns the block count
+int launch_mass(const Params &P, hipStream_t s, const double *D, double *partial, int max_blocks);
 
 }  // namespace grhd

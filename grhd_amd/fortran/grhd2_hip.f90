@@ -10,8 +10,10 @@
 ! loop -- CFL dt, final-time clamp, SSP-RK3 stages with con2prim and boundaries
 ! -- runs on the device through grhd_advance.  The seeded perturbation is
 ! injected when t reaches perturbation_time (grhd2.f90:186-227) and the mass
-! monitor is printed every mass_monitor_stride steps (grhd2.f90:282-301).  At the
-! end the conserved state up(5,nx,ny,nz) is written in the reference's order to
+! monitor, reduced on the device, is printed every mass_monitor_stride steps
+! (grhd2.f90:282-301).  At the end the convergence norms of the advected wave,
+! Michel and dust problems are printed (grhd2.f90:358) and the conserved state
+! up(5,nx,ny,nz) is written in the reference's order to
 ! <output_folder>/<output_prefix>_final_u.bin (raw float64).
 ! =============================================================================
 program grhd2_hip
@@ -27,7 +29,8 @@ program grhd2_hip
   integer(c_int64_t) :: counters(4)
   character(len=1024) :: arg1, parfile
   logical :: check_only, perturbed
-  real(c_double) :: t, last_dt, dt, dx, dy, dz, mass
+  real(c_double) :: t, last_dt, dt, mass, norms(3)
+  integer(c_int32_t) :: imax
   real(c_double), allocatable :: u(:,:,:,:)
   integer :: nargs, nx, ny, nz, unit_out
   integer(c_int64_t) :: nsteps
@@ -80,16 +83,7 @@ program grhd2_hip
 
   call require(grhd_initial_conditions(ctx), 'grhd_initial_conditions')
   allocate(u(5, nx, ny, nz))
-  ! logical spacings (initialization.f90:88-100)
-  if (cfg%log_r /= 0) then
-    dx = (log(cfg%r_max) - log(cfg%r_min)) / dble(max(1, nx))
-  else
-    dx = (cfg%r_max - cfg%r_min) / dble(max(1, nx))
-  end if
-  dy = (cfg%y_max - cfg%y_min) / dble(max(1, ny))
-  dz = (cfg%z_max - cfg%z_min) / dble(max(1, nz))
-  call require(grhd_get_conserved(ctx, u), 'grhd_get_conserved')
-  mass = sum(u(1,:,:,:)) * dx * dy * dz
+  call require(grhd_total_mass(ctx, mass), 'grhd_total_mass')
   write(*,'(A,ES22.14)') 'Initial mass in domain: ', mass
   call require(grhd_compute_dt(ctx, dt), 'grhd_compute_dt')
   write(*,'(A,ES22.14)') 'dt (CFL, metric light cone): ', dt
@@ -116,8 +110,7 @@ program grhd2_hip
     nsteps = nsteps + done
     if (done == 0) exit
     if (rc%mass_monitor_stride > 0 .and. mod(nsteps, int(rc%mass_monitor_stride, 8)) == 0) then
-      call require(grhd_get_conserved(ctx, u), 'grhd_get_conserved')
-      mass = sum(u(1,:,:,:)) * dx * dy * dz
+      call require(grhd_total_mass(ctx, mass), 'grhd_total_mass')
       write(*,'(A,I0,A,ES14.6,A,ES22.14)') 'step ', nsteps, '  t = ', t, '  total mass: ', mass
     end if
   end do
@@ -135,6 +128,12 @@ program grhd2_hip
       dble(nx) * dble(ny) * dble(nz) * dble(nsteps) / wall1
   write(*,'(A,I0)') ' cells reset to atmosphere by con2prim: ', counters(1)
   write(*,'(A)') '========================================================='
+  ! calc_convergence_norms (grhd2.f90:358): advected wave, Michel and dust only
+  if (cfg%problem == 6 .or. cfg%problem == 7 .or. cfg%problem == 8) then
+    call require(grhd_convergence_norms(ctx, t, norms, imax), 'grhd_convergence_norms')
+    write(*,'(A,I0,A,ES20.12,A,ES20.12,A,ES20.12,A,I0)') ' N = ', nx, ' | L1 = ', norms(1), &
+      ' | L2 = ', norms(2), ' | Linf = ', norms(3), ' at cell ', imax
+  end if
 
   call execute_command_line('mkdir -p '//trim(rc%output_folder))
   open(newunit=unit_out, file=trim(rc%output_folder)//'/'//trim(rc%output_prefix)//'_final_u.bin', &

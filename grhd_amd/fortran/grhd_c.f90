@@ -14,6 +14,7 @@ module grhd_c
   public :: grhd_create, grhd_validate, grhd_destroy, grhd_last_error, grhd_dims, grhd_initial_conditions
   public :: grhd_set_primitives, grhd_get_primitives, grhd_set_conserved, grhd_get_conserved
   public :: grhd_compute_dt, grhd_step, grhd_advance, grhd_perturb, grhd_counters
+  public :: grhd_total_mass, grhd_convergence_norms
   public :: grhd_last_timing, grhd_synchronize, grhd_error_message
 
   integer, parameter, public :: GRHD_OK = 0
@@ -130,6 +131,18 @@ module grhd_c
       type(c_ptr), value :: ctx
       integer(c_int64_t), intent(out) :: counters(4)
     end function grhd_counters
+    integer(c_int) function grhd_total_mass(ctx, mass) bind(C, name='grhd_total_mass')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: mass
+    end function grhd_total_mass
+    integer(c_int) function grhd_convergence_norms(ctx, t, norms, imax) bind(C, name='grhd_convergence_norms')
+      import :: c_int, c_ptr, c_double, c_int32_t
+      type(c_ptr), value :: ctx
+      real(c_double), value :: t
+      real(c_double), intent(out) :: norms(3)
+      integer(c_int32_t), intent(out) :: imax
+    end function grhd_convergence_norms
     integer(c_int) function grhd_last_timing(ctx, ms, launches) bind(C, name='grhd_last_timing')
       import :: c_int, c_ptr, c_double, c_int64_t
       type(c_ptr), value :: ctx

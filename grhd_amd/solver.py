@@ -154,6 +154,20 @@ class Solver:
         self._chk(self._L.grhd_counters(self.h, c))
         return {"atmosphere_resets": c[0], "brent_fallbacks": c[1], "steps": c[2]}
 
+    def total_mass(self) -> float:
+        """Mass monitor (grhd2.f90:283-301), reduced on the device."""
+        m = ctypes.c_double()
+        self._chk(self._L.grhd_total_mass(self.h, ctypes.byref(m)))
+        return m.value
+
+    def convergence_norms(self, t: float = 0.0):
+        """calc_convergence_norms (output.f90:1144-1258): (L1, L2, Linf) of rho
+        against the analytic solution at time t, and the Linf cell."""
+        n = (ctypes.c_double * 3)()
+        im = ctypes.c_int32()
+        self._chk(self._L.grhd_convergence_norms(self.h, float(t), n, ctypes.byref(im)))
+        return (n[0], n[1], n[2]), im.value
+
     def synchronize(self):
         self._chk(self._L.grhd_synchronize(self.h))
 

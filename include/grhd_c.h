@@ -171,6 +171,20 @@ int grhd_step(grhd_ctx *ctx, double dt, int32_t nsteps);
  * dt from grhd_compute_dt, clamped at the end.  *t is read and updated. */
 int grhd_advance(grhd_ctx *ctx, double t_final, int32_t max_steps, double *t,
                  int32_t *steps_done, double *last_dt);
+/* Mass monitor (grhd2.f90:283-301): the sum over the interior of the current
+ * U(D) dx dy dz, reduced on the device (per-block partial sums added on the
+ * host in block order; the reference's serial order is not kept, so the sum
+ * agrees to rounding). */
+int grhd_total_mass(grhd_ctx *ctx, double *mass);
+/* calc_convergence_norms (output.f90:1144-1258) at integration time t: L1, L2
+ * and Linf of rho against the analytic solution on row j = max(1, ny/2),
+ * k = 1 -- the advected wave (problem 6: exact cell averages, cells 1..nx),
+ * Michel (7: the analytic initial data) and dust (8: the stationary free fall),
+ * both on cells int(0.1 nx)..int(0.95 nx).  norms[3] = L1, L2, Linf in the
+ * reference's summation order; *imax (may be NULL) = cell of the Linf.
+ * GRHD_ERR_UNSUPPORTED for the other problems (the reference prints nothing
+ * for them). */
+int grhd_convergence_norms(grhd_ctx *ctx, double t, double *norms, int32_t *imax);
 
 /* Unit entry points for parity checks. */
 /* set_boundary_conditions on the device primitives (conditions.f90:773-1066). */
