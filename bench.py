@@ -172,7 +172,7 @@ def _progress(c):
     print(f"served {c.label}: {c.steps} steps, latency {c.latency:.2f} s", file=sys.stderr, flush=True)
 
 
-def run_ensemble(n, rate, time_scale, chunk, seed=3435, mix="quick", progress=False):
+def run_ensemble(n, rate, time_scale, chunk, seed=3435, mix="quick", progress=False, pacing=8):
     """Served request mix: Poisson arrivals of independent problems (shock
     tubes, Michel accretion with spin 0 or in [0, 0.99), Kelvin-Helmholtz with
     seeded v^z noise), continuously batched on the device; p50/p99
@@ -182,12 +182,14 @@ def run_ensemble(n, rate, time_scale, chunk, seed=3435, mix="quick", progress=Fa
     m = MIXES[mix]
     reqs = request_mix(n, rate=rate, seed=seed, time_scale=time_scale, **m)
     with Batch(capacity=64) as B:
+        B.set_pacing(pacing)
         rep = Server(B, footprint, hbm_budget=64 << 30, chunk=chunk,
                      on_complete=_progress if progress else None).run(reqs)
         info = B.info()
     s = rep.summary()
     lo, hi = min(m["shock_nx"]), max(m["shock_nx"])
     s.update({"arrival_rate_per_s": rate, "final_time_scale": time_scale, "chunk_steps": chunk,
+              "pacing": pacing,
               "launch_form": info["variant"], "seed": seed,
               "mix": f"{mix}: sod/strong_shock/shu_osher nx {lo}-{hi}, michel nr "
                      f"{'/'.join(map(str, m['michel_nr']))} a=0 or U[0,0.99), kelvin_helmholtz "
@@ -280,6 +282,8 @@ def main():
                     help="Poisson arrival rates (requests/s) of the served ensemble, swept in order; "
                          "the line's 'ensemble' is the one at 16/s (or the first)")
     ap.add_argument("--serve-mix", choices=sorted(MIXES), default="quick")
+    ap.add_argument("--serve-pacing", type=int, default=8,
+                    help="steps of a cheaper batch group per step of the most expensive one (1..8)")
     ap.add_argument("--serve-progress", action="store_true",
                     help="print each served request to stderr as it completes")
     ap.add_argument("--serve-only", action="store_true",
@@ -308,7 +312,8 @@ def main():
     if a.serve_only:
         for r in (float(r) for r in a.serve_rates.split(",") if r):
             print(json.dumps(run_ensemble(a.serve_requests, r, a.serve_time_scale, a.serve_chunk,
-                                          mix=a.serve_mix, progress=a.serve_progress)), flush=True)
+                                          mix=a.serve_mix, progress=a.serve_progress,
+                                          pacing=a.serve_pacing)), flush=True)
         return
     nx, ny, nz = (int(v) for v in a.grid.split(","))
     cfg = c4_config(nx, ny, nz)
@@ -362,7 +367,7 @@ def main():
     if a.serve_requests > 0:
         rates = [float(r) for r in a.serve_rates.split(",") if r]
         sweep = [run_ensemble(a.serve_requests, r, a.serve_time_scale, a.serve_chunk, mix=a.serve_mix,
-                              progress=a.serve_progress)
+                              progress=a.serve_progress, pacing=a.serve_pacing)
                  for r in rates]
         line["ensemble"] = next((e for e in sweep if e["arrival_rate_per_s"] == 16.0), sweep[0])
         line["ensemble_sweep"] = [{k: e[k] for k in ("arrival_rate_per_s", "p50_s", "p99_s", "mean_wait_s",

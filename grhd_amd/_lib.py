@@ -108,6 +108,11 @@ _SIGS = {
     "grhd_con2prim": ([ctypes.c_void_p], ctypes.c_int),
     "grhd_counters": ([ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64)], ctypes.c_int),
     "grhd_total_mass": ([ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
+    "grhd_global_diagnostics": ([ctypes.c_void_p, ctypes.POINTER(ctypes.c_double),
+                                 ctypes.POINTER(ctypes.c_int64)], ctypes.c_int),
+    "grhd_ppi_modes": ([ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
+    "grhd_accretion_rate": ([ctypes.c_void_p, ctypes.POINTER(ctypes.c_double),
+                             ctypes.POINTER(ctypes.c_int32)], ctypes.c_int),
     "grhd_convergence_norms": ([ctypes.c_void_p, ctypes.c_double, ctypes.POINTER(ctypes.c_double),
                                 ctypes.POINTER(ctypes.c_int32)], ctypes.c_int),
     "grhd_set_stream": ([ctypes.c_void_p, ctypes.c_void_p], ctypes.c_int),
@@ -125,6 +130,7 @@ _SIGS = {
     "grhd_batch_status": ([ctypes.c_void_p, _I, ctypes.POINTER(_I), ctypes.POINTER(_D),
                            ctypes.POINTER(ctypes.c_int64)], ctypes.c_int),
     "grhd_batch_set_variant": ([ctypes.c_void_p, _I], ctypes.c_int),
+    "grhd_batch_set_pacing": ([ctypes.c_void_p, _I], ctypes.c_int),
     "grhd_batch_info": ([ctypes.c_void_p, ctypes.POINTER(_I), ctypes.POINTER(_I),
                          ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(_D),
                          ctypes.POINTER(ctypes.c_int64)], ctypes.c_int),

@@ -103,6 +103,11 @@ class Batch:
     def set_variant(self, variant: int):
         self._chk(self._L.grhd_batch_set_variant(self.h, int(variant)))
 
+    def set_pacing(self, max_ratio: int):
+        """Cheaper groups take up to max_ratio steps per step of the most
+        expensive group (1: every problem advances the same steps)."""
+        self._chk(self._L.grhd_batch_set_pacing(self.h, int(max_ratio)))
+
     def info(self) -> dict:
         v = ctypes.c_int32()
         a = ctypes.c_int32()

@@ -29,7 +29,8 @@ using namespace grhd;
 namespace {
 
 constexpr int kMaxGroups = 16;  // metric kinds x face arithmetics
-constexpr int kMaxChunk = 64;   // steps per captured graph
+constexpr int kMaxChunk = 64;   // steps per captured graph (of the most expensive group)
+constexpr int kMaxPace = 8;     // steps of a cheaper group per step of the most expensive one
 constexpr int kTuneSteps = 2;   // steps timed per candidate
 // Problems of at most this many cells run whole chunks of steps in one block
 // each (kb_small); larger ones use the launch form the autotuner picks.  One
@@ -62,7 +63,9 @@ struct BGroup {
     int mk = 0;
     bool mixed = false;
     bool small = false;
+    bool large = false;  // with pacing: problems above kFusedStageCells cells
     std::vector<int> slots;
+    int ratio = 1;      // steps per chunk step (pacing)
     int64_t total[NPH] = {0, 0, 0, 0, 0};
     int lds_groups = 1;
     unsigned grid = 0;  // persistent grid
@@ -83,6 +86,8 @@ struct grhd_batch {
     std::vector<BGroup> groups;
     bool dirty = true;
     int forced = 0;     // 0: autotune, else variant index + 1
+    int pace = 1;       // at most this many steps of a cheap group per chunk step
+    std::vector<int> slot_ratio;   // steps per chunk step of each slot's group
     int variant = 0;    // variant in use (index)
     std::map<std::string, int> tuned;
     struct GraphEntry {
@@ -141,29 +146,71 @@ void run_blocks(const Params &P, int gw, int64_t out[NPH], int &lds_groups)
     if (P.nz > 1) lds_groups = std::max(lds_groups, 4 / tz.gw);
 }
 
+// Pacing: with pace > 1 and the groups forked onto their own streams, a group
+// whose step is cheaper than the most expensive group's takes up to pace steps
+// per chunk step, so that small problems (a Michel run of 10^5-10^6 steps) do
+// not advance at the step rate of the largest ones.  The step cost is
+// estimated as a latency floor plus a per-cell term: about nine launches of
+// about 6 us and 0.5 ns per cell (the one-block form of the smallest problems
+// measured within 10 % of the launch forms per step, so it takes the same
+// floor).  The estimate only sets how many steps a group takes, never what a
+// step computes.
+void pace_groups(grhd_batch *b, int v)
+{
+    b->slot_ratio.assign(b->cap, 1);
+    double most = 0.0;
+    std::vector<double> cost(b->groups.size(), 0.0);
+    for (size_t g = 0; g < b->groups.size(); g++) {
+        const BGroup &G = b->groups[g];
+        int64_t cells = 0;
+        for (int sl : G.slots) {
+            const Params &P = b->runs[sl].ctx->P;
+            cells += (int64_t)P.nx * P.ny * P.nz;
+        }
+        cost[g] = 54.0e-6 + 0.5e-9 * (double)cells;
+        most = std::max(most, cost[g]);
+    }
+    const bool forked = !kVariants[v].persistent && b->groups.size() > 1;
+    for (size_t g = 0; g < b->groups.size(); g++) {
+        BGroup &G = b->groups[g];
+        G.ratio = 1;
+        if (forked && b->pace > 1) G.ratio = std::max(1, std::min(b->pace, (int)(most / cost[g])));
+        for (int sl : G.slots) b->slot_ratio[sl] = G.ratio;
+    }
+}
+
 // Groups and block tables of the unfinished problems for variant v.
 int rebuild(grhd_batch *b, int v)
 {
     BCHK(b, hipStreamSynchronize(b->stream));
     drop_graphs(b);
-    b->groups.clear();
-    for (int s = 0; s < b->cap; s++) {
-        const BRun &r = b->runs[s];
-        if (!r.ctx || r.done) continue;
-        const int mk = r.ctx->P.mk;
-        const bool mixed = r.ctx->mixed;
-        const bool small = (int64_t)r.ctx->P.nx * r.ctx->P.ny * r.ctx->P.nz <= kSmallCells;
-        auto it = std::find_if(b->groups.begin(), b->groups.end(), [&](const BGroup &g) {
-            return g.mk == mk && g.mixed == mixed && g.small == small;
-        });
-        if (it == b->groups.end()) {
-            b->groups.push_back(BGroup());
-            it = b->groups.end() - 1;
-            it->mk = mk;
-            it->mixed = mixed;
-            it->small = small;
+    // with pacing, large problems form groups of their own, so that the small
+    // problems of the same metric kind are not held to their rate (unless that
+    // would exceed the group limit)
+    for (int split = (b->pace > 1) ? 1 : 0; split >= 0; split--) {
+        b->groups.clear();
+        for (int s = 0; s < b->cap; s++) {
+            const BRun &r = b->runs[s];
+            if (!r.ctx || r.done) continue;
+            const int mk = r.ctx->P.mk;
+            const bool mixed = r.ctx->mixed;
+            const int64_t cells = (int64_t)r.ctx->P.nx * r.ctx->P.ny * r.ctx->P.nz;
+            const bool small = cells <= kSmallCells;
+            const bool large = split && cells > kFusedStageCells;
+            auto it = std::find_if(b->groups.begin(), b->groups.end(), [&](const BGroup &g) {
+                return g.mk == mk && g.mixed == mixed && g.small == small && g.large == large;
+            });
+            if (it == b->groups.end()) {
+                b->groups.push_back(BGroup());
+                it = b->groups.end() - 1;
+                it->mk = mk;
+                it->mixed = mixed;
+                it->small = small;
+                it->large = large;
+            }
+            it->slots.push_back(s);
         }
-        it->slots.push_back(s);
+        if ((int)b->groups.size() <= kMaxGroups) break;
     }
     if ((int)b->groups.size() > kMaxGroups) return bfail(b, GRHD_ERR_UNSUPPORTED, "too many batch groups");
     const int gw = kVariants[v].gw;
@@ -191,6 +238,7 @@ int rebuild(grhd_batch *b, int v)
             G.grid = (unsigned)std::min<int64_t>((int64_t)occ * b->cu, most);
         }
     }
+    pace_groups(b, v);
     BCHK(b, hipStreamSynchronize(b->stream));
     b->dirty = false;
     b->variant = v;
@@ -231,13 +279,13 @@ void enqueue_steps(grhd_batch *b, int v, int K, int64_t &launches)
             L.dts = b->d_dts;
             L.mk = b->groups[g].mk;
             L.small = true;
-            L.K = K;
+            L.K = K * b->groups[g].ratio;
             L.cap = b->cap;
             if (b->groups[g].mixed) batch_stage_float(L, sg);
             else batch_stage_double(L, sg);
             launches += 1;
         }
-        for (int s = 0; s < K && !b->groups[g].small; s++)
+        for (int s = 0; s < K * b->groups[g].ratio && !b->groups[g].small; s++)
             for (int stage = 1; stage <= 3; stage++) {
                 BatchLaunch L;
                 std::memset(&L, 0, sizeof(L));
@@ -291,19 +339,30 @@ int graph_for(grhd_batch *b, int v, int K, hipGraphExec_t &exec, int64_t &launch
     return GRHD_OK;
 }
 
-// dt of every slot for the next K steps (grhd2.f90:179-184, 229-230: dt from
-// the CFL condition, clamped to the final time, t advances first); 0 once a
-// problem has reached its final time.  Returns the steps each slot takes.
+// Largest steps-per-chunk-step ratio of the current groups.
+int chunk_ratio(const grhd_batch *b)
+{
+    int m = 1;
+    for (const BGroup &G : b->groups) m = std::max(m, G.ratio);
+    return m;
+}
+
+// dt of every slot for the next K steps of its group's pacing (K times its
+// ratio; grhd2.f90:179-184, 229-230: dt from the CFL condition, clamped to the
+// final time, t advances first); 0 once a problem has reached its final time
+// or taken its steps.  Returns the steps each slot takes.
 void fill_dts(grhd_batch *b, int K, std::vector<int> &active)
 {
     active.assign(b->cap, 0);
+    const int rows = K * chunk_ratio(b);
     for (int s = 0; s < b->cap; s++) {
         BRun &r = b->runs[s];
         double t = r.t;
         const double tf = r.ctx ? r.ctx->cfg.final_time : 0.0;
-        for (int k = 0; k < K; k++) {
+        const int ks = K * (s < (int)b->slot_ratio.size() ? b->slot_ratio[s] : 1);
+        for (int k = 0; k < rows; k++) {
             double dt = 0.0;
-            if (r.ctx && !r.done && t < tf) {
+            if (k < ks && r.ctx && !r.done && t < tf) {
                 dt = r.ctx->dt_cfl;
                 if (t + dt > tf) dt = tf - t;
                 t = t + dt;
@@ -344,7 +403,7 @@ int run_chunk(grhd_batch *b, int v, int K, double *ms)
     fill_runs(b);
     BCHK(b, hipMemcpyAsync(b->d_runs, b->h_runs, (size_t)b->cap * sizeof(RunDesc), hipMemcpyHostToDevice,
                            b->stream));
-    BCHK(b, hipMemcpyAsync(b->d_dts, b->h_dts, (size_t)K * b->cap * sizeof(double), hipMemcpyHostToDevice,
+    BCHK(b, hipMemcpyAsync(b->d_dts, b->h_dts, (size_t)K * chunk_ratio(b) * b->cap * sizeof(double), hipMemcpyHostToDevice,
                            b->stream));
     hipGraphExec_t exec = nullptr;
     int64_t launches = 0;
@@ -443,9 +502,9 @@ int grhd_batch_create(int32_t capacity, uint64_t hbm_budget, grhd_batch **out)
     if ((e = hipMalloc(&b->d_runs, capacity * sizeof(RunDesc))) != hipSuccess) return bail(e, "malloc");
     if ((e = hipHostMalloc(&b->h_runs, capacity * sizeof(RunDesc), hipHostMallocDefault)) != hipSuccess)
         return bail(e, "host malloc");
-    if ((e = hipMalloc(&b->d_dts, (size_t)kMaxChunk * capacity * sizeof(double))) != hipSuccess)
+    if ((e = hipMalloc(&b->d_dts, (size_t)kMaxChunk * kMaxPace * capacity * sizeof(double))) != hipSuccess)
         return bail(e, "malloc");
-    if ((e = hipHostMalloc(&b->h_dts, (size_t)kMaxChunk * capacity * sizeof(double), hipHostMallocDefault)) !=
+    if ((e = hipHostMalloc(&b->h_dts, (size_t)kMaxChunk * kMaxPace * capacity * sizeof(double), hipHostMallocDefault)) !=
         hipSuccess)
         return bail(e, "host malloc");
     if ((e = hipMalloc(&b->d_slot, (size_t)kMaxGroups * capacity * sizeof(int))) != hipSuccess)
@@ -563,6 +622,14 @@ int grhd_batch_set_variant(grhd_batch *b, int32_t variant)
 {
     if (!b || variant < 0 || variant > kNumVariants) return GRHD_ERR_INVALID_ARGUMENT;
     b->forced = variant;
+    b->dirty = true;
+    return GRHD_OK;
+}
+
+int grhd_batch_set_pacing(grhd_batch *b, int32_t max_ratio)
+{
+    if (!b || max_ratio < 1 || max_ratio > kMaxPace) return GRHD_ERR_INVALID_ARGUMENT;
+    b->pace = max_ratio;
     b->dirty = true;
     return GRHD_OK;
 }

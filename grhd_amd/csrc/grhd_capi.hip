@@ -340,7 +340,9 @@ uint64_t ctx_footprint(const grhd_config &cfg)
     uint64_t b = (uint64_t)P.vstride * NEQ * sizeof(double) + 2 * ncell * NEQ * sizeof(double);
     for (int d = 0; d < 3; d++) b += nf[d] * (d == 0 || nf[d] > 1 ? NEQ : 1) * rsz;
     b += (uint64_t)NEQ * P.ng * P.ny * P.nz * sizeof(double) + 8;
-    b += (((uint64_t)P.nx * P.ny + 255) / 256) * 4 * sizeof(double) + 4 * sizeof(unsigned long long);
+    b += (uint64_t)std::max<int64_t>((((int64_t)P.nx * P.ny + 255) / 256) * 4, std::max<int64_t>(4096, P.nz)) *
+             sizeof(double) +
+         4 * sizeof(unsigned long long);
     b += (uint64_t)coord_table_size(P.nx, P.ny, P.ng) * sizeof(double);
     return b;
 }
@@ -422,7 +424,8 @@ int grhd_create(const grhd_config *cfg, grhd_ctx **out)
     if ((e = hipMalloc(&c->injector, (size_t)NEQ * P.ng * P.ny * P.nz * sizeof(double) + 8)) != hipSuccess)
         return bail(e, "hipMalloc injector");
     c->partial_n = (int)(((int64_t)P.nx * P.ny + 255) / 256);
-    if ((e = hipMalloc(&c->partial, (size_t)c->partial_n * 4 * sizeof(double))) != hipSuccess)
+    c->partial_cap = std::max<int64_t>((int64_t)c->partial_n * 4, std::max<int64_t>(4096, P.nz));
+    if ((e = hipMalloc(&c->partial, (size_t)c->partial_cap * sizeof(double))) != hipSuccess)
         return bail(e, "hipMalloc partial");
     if ((e = hipMalloc(&c->counters, 4 * sizeof(unsigned long long))) != hipSuccess)
         return bail(e, "hipMalloc counters");
@@ -683,7 +686,7 @@ int grhd_con2prim(grhd_ctx *c)
 int grhd_total_mass(grhd_ctx *c, double *mass)
 {
     if (!c || !mass) return GRHD_ERR_INVALID_ARGUMENT;
-    const int nb = launch_mass(c->P, c->stream, c->u[c->ucur], c->partial, c->partial_n * 4);
+    const int nb = launch_mass(c->P, c->stream, c->u[c->ucur], c->partial, (int)std::min<int64_t>(c->partial_cap, 4096));
     HIPCHK(c, hipGetLastError());
     std::vector<double> h((size_t)nb);
     HIPCHK(c, hipMemcpyAsync(h.data(), c->partial, h.size() * sizeof(double), hipMemcpyDeviceToHost,
@@ -750,6 +753,113 @@ int grhd_convergence_norms(grhd_ctx *c, double t, double *norms, int32_t *imax)
     norms[1] = std::sqrt(s2 / cells);
     norms[2] = linf;
     if (imax) *imax = im;
+    return GRHD_OK;
+}
+
+int grhd_global_diagnostics(grhd_ctx *c, double *out, int64_t *counts)
+{
+    if (!c || !out || !counts) return GRHD_ERR_INVALID_ARGUMENT;
+    const int nb = launch_global(c->P, c->stream, c->p, c->u[c->ucur], c->partial,
+                                 (int)std::min<int64_t>(256, c->partial_cap / kGlobalPartials));
+    HIPCHK(c, hipGetLastError());
+    std::vector<double> h((size_t)nb * kGlobalPartials);
+    HIPCHK(c, hipMemcpyAsync(h.data(), c->partial, h.size() * sizeof(double), hipMemcpyDeviceToHost,
+                             c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    double r[kGlobalPartials];
+    for (int q = 0; q < kGlobalPartials; q++) r[q] = h[(size_t)q];
+    for (int b = 1; b < nb; b++) {
+        const double *x = &h[(size_t)b * kGlobalPartials];
+        r[0] = r[0] + x[0];
+        r[1] = r[1] + x[1];
+        r[2] = r[2] + x[2];
+        r[3] = std::fmin(r[3], x[3]);
+        r[4] = std::fmin(r[4], x[4]);
+        r[5] = std::fmax(r[5], x[5]);
+        r[6] = std::fmax(r[6], x[6]);
+        r[7] = std::fmax(r[7], x[7]);
+        r[8] = r[8] + x[8];
+        r[9] = r[9] + x[9];
+    }
+    // output.f90:1120-1122 order: mass, disk mass, angular momentum, rho min,
+    // p min, rho max, p max, max v^2; atmosphere and invalid cells
+    const double order[8] = {r[0], r[1], r[2], r[3], r[4], r[5], r[6], r[7]};
+    for (int q = 0; q < 8; q++) out[q] = order[q];
+    counts[0] = (int64_t)r[8];
+    counts[1] = (int64_t)r[9];
+    return GRHD_OK;
+}
+
+int grhd_ppi_modes(grhd_ctx *c, double *out)
+{
+    if (!c || !out) return GRHD_ERR_INVALID_ARGUMENT;
+    const Params &P = c->P;
+    if (P.nz < 2) return fail(c, GRHD_ERR_UNSUPPORTED, "azimuthal modes need nz >= 2");
+    launch_plane_mass(P, c->stream, c->p, c->u[c->ucur], c->partial);
+    HIPCHK(c, hipGetLastError());
+    std::vector<double> m((size_t)P.nz);
+    HIPCHK(c, hipMemcpyAsync(m.data(), c->partial, m.size() * sizeof(double), hipMemcpyDeviceToHost,
+                             c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    // the plane masses summed over k in the reference's order (output.f90:1021-1044)
+    constexpr int M = kPpiModes;
+    double c0 = 0.0, cr[M] = {0.0}, ci[M] = {0.0};
+    for (int k = 1; k <= P.nz; k++) {
+        const double mk = m[(size_t)(k - 1)];
+        c0 = c0 + mk;
+        for (int mode = 1; mode <= M; mode++) {
+            cr[mode - 1] = cr[mode - 1] + mk * std::cos((double)mode * zc(P, k));
+            ci[mode - 1] = ci[mode - 1] - mk * std::sin((double)mode * zc(P, k));
+        }
+    }
+    out[0] = c0;
+    for (int mode = 1; mode <= M; mode++) {
+        double *o = out + 1 + 4 * (mode - 1);
+        o[0] = cr[mode - 1];
+        o[1] = ci[mode - 1];
+        o[2] = 0.0;
+        o[3] = 0.0;
+        if (c0 > DBL_MIN) {
+            o[2] = std::sqrt(cr[mode - 1] * cr[mode - 1] + ci[mode - 1] * ci[mode - 1]) / c0;
+            o[3] = -std::atan2(ci[mode - 1], cr[mode - 1]) / (double)mode;
+        }
+    }
+    return GRHD_OK;
+}
+
+int grhd_accretion_rate(grhd_ctx *c, double *mdot, int32_t *probe)
+{
+    if (!c || !mdot) return GRHD_ERR_INVALID_ARGUMENT;
+    const Params &P = c->P;
+    // the probe cell (initialization.f90:117-190): the first centre at or
+    // outside the event horizon r+ = M + sqrt(M^2 - a^2) (a = 0 for EF)
+    if (c->cfg.metric == GRHD_METRIC_MINKOWSKI || P.mass < 0.1)
+        return fail(c, GRHD_ERR_UNSUPPORTED, "accretion rate needs a black-hole metric with M >= 0.1");
+    const double a = (c->cfg.metric == GRHD_METRIC_KERR_SCHILD) ? P.spin : 0.0;
+    const double rh = P.mass + std::sqrt(std::fmax(0.0, P.mass * P.mass - a * a));
+    const double xh = (c->cfg.log_r != 0) ? std::log(rh) : rh;
+    int ip = 0;
+    if (xh < xf(P, 0)) {
+        ip = 1;
+    } else if (xh <= xf(P, P.nx)) {
+        for (int i = 1; i <= P.nx; i++)
+            if (xc(P, i) >= xh) {
+                ip = i;
+                break;
+            }
+    }
+    if (ip == 0) return fail(c, GRHD_ERR_UNSUPPORTED, "no cell centre lies outside the event horizon");
+    const int nb = launch_mdot(P, c->stream, ip, c->p, c->u[c->ucur], c->partial,
+                               (int)std::min<int64_t>(256, c->partial_cap));
+    HIPCHK(c, hipGetLastError());
+    std::vector<double> h((size_t)nb);
+    HIPCHK(c, hipMemcpyAsync(h.data(), c->partial, h.size() * sizeof(double), hipMemcpyDeviceToHost,
+                             c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    double s = 0.0;
+    for (int b = 0; b < nb; b++) s = s + h[(size_t)b];
+    *mdot = -s;
+    if (probe) *probe = ip;
     return GRHD_OK;
 }
 

@@ -21,7 +21,8 @@ struct grhd_ctx {
     int64_t fstr[3] = {0, 0, 0};
     double *injector = nullptr;
     double *partial = nullptr;
-    int partial_n = 0;
+    int partial_n = 0;        // CFL partials (4 doubles each)
+    int64_t partial_cap = 0;  // doubles in partial (reductions and observers)
     unsigned long long *counters = nullptr;
     double *coord = nullptr;  // coordinate table (Params::coord)
     int64_t steps = 0;

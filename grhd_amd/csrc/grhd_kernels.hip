@@ -95,6 +95,116 @@ int launch_mass(const Params &P, hipStream_t s, const double *D, double *partial
     return nb;
 }
 
+// ------------------------------------------------------------ observers --
+// Block reduction of one value by op (sum, min or max) over the 256 threads:
+// wave64 shuffle, then the four waves through LDS.  Thread 0 gets the result.
+enum { RED_SUM = 0, RED_MIN = 1, RED_MAX = 2 };
+__device__ __forceinline__ double red_op(int op, double a, double b)
+{
+    return op == RED_SUM ? a + b : (op == RED_MIN ? fmin(a, b) : fmax(a, b));
+}
+__device__ double block_reduce(int op, double v, double *sh)
+{
+    for (int off = 32; off > 0; off >>= 1) v = red_op(op, v, __shfl_xor(v, off, 64));
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    __syncthreads();  // sh is reused by consecutive reductions
+    if (lane == 0) sh[wave] = v;
+    __syncthreads();
+    return red_op(op, red_op(op, sh[0], sh[1]), red_op(op, sh[2], sh[3]));
+}
+
+// calc_global_diagnostics (output.f90:1052-1125) over the interior: per block
+// NGLOB partials -- mass, disk mass (rho > 1e3 rho_floor), angular momentum
+// (S_3), rho min / max and p min / max over finite values, max v^2 over finite
+// values, atmosphere and invalid cell counts.  v^2 = gamma_ij v^i v^j with the
+// centre metric evaluated in registers.
+constexpr int NGLOB = 10;
+template <int MK>
+__global__ __launch_bounds__(256) void k_global(Params P, const double *__restrict__ p,
+                                                const double *__restrict__ u, double *__restrict__ partial)
+{
+    const int64_t ncell = (int64_t)P.nx * P.ny * P.nz;
+    const double vol = P.dx * P.dy * P.dz, cut = 1.0e3 * P.rho_floor;
+    double acc[NGLOB] = {0.0, 0.0, 0.0, DBL_MAX, DBL_MAX, -DBL_MAX, -DBL_MAX, 0.0, 0.0, 0.0};
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < ncell;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        const int i = (int)(t % P.nx) + 1;
+        const int64_t rem = t / P.nx;
+        const int j = (int)(rem % P.ny) + 1, k = (int)(rem / P.ny) + 1;
+        const int64_t pc = pindex(P, i, j, k);
+        double q[NEQ];
+        for (int v = 0; v < NEQ; v++) q[v] = p[v * P.vstride + pc];
+        double g[3][3], gi[3][3], sqg, alpha, beta[3];
+        metric_c2p<MK>(P, coord_cc(P, i, j), g, gi, sqg, alpha, beta);
+        double vsq = 0.0;
+        for (int a = 0; a < 3; a++)
+            for (int b = 0; b < 3; b++) vsq = vsq + g[a][b] * q[EQ_VX + a] * q[EQ_VX + b];
+        const double rho = q[EQ_DE], pr = q[EQ_PR];
+        if (!isfinite(rho) || !isfinite(pr) || !isfinite(vsq) || rho <= 0.0 || pr <= 0.0 || vsq < 0.0 ||
+            vsq >= 1.0)
+            acc[9] = acc[9] + 1.0;
+        if (isfinite(rho)) {
+            acc[3] = fmin(acc[3], rho);
+            acc[5] = fmax(acc[5], rho);
+        }
+        if (isfinite(pr)) {
+            acc[4] = fmin(acc[4], pr);
+            acc[6] = fmax(acc[6], pr);
+        }
+        if (isfinite(vsq)) acc[7] = fmax(acc[7], vsq);
+        const double D = u[EQ_DE * ncell + t];
+        acc[0] = acc[0] + D * vol;
+        acc[2] = acc[2] + u[EQ_VZ * ncell + t] * vol;
+        if (rho > cut) acc[1] = acc[1] + D * vol;
+        else acc[8] = acc[8] + 1.0;
+    }
+    __shared__ double sh[4];
+    const int ops[NGLOB] = {RED_SUM, RED_SUM, RED_SUM, RED_MIN, RED_MIN, RED_MAX, RED_MAX, RED_MAX, RED_SUM, RED_SUM};
+    for (int q = 0; q < NGLOB; q++) {
+        const double r = block_reduce(ops[q], acc[q], sh);
+        if (threadIdx.x == 0) partial[(size_t)blockIdx.x * NGLOB + q] = r;
+    }
+}
+
+// Mass of each phi plane above the density cut (calc_ppi_modes,
+// output.f90:1011-1020): one block per plane k, partial[k - 1].
+__global__ __launch_bounds__(256) void k_plane_mass(Params P, const double *__restrict__ p,
+                                                    const double *__restrict__ u, double *__restrict__ partial)
+{
+    const int k = blockIdx.x + 1;
+    const int64_t ncell = (int64_t)P.nx * P.ny * P.nz, plane = (int64_t)P.nx * P.ny;
+    const double vol = P.dx * P.dy * P.dz, cut = 1.0e3 * P.rho_floor;
+    double acc = 0.0;
+    for (int64_t c = threadIdx.x; c < plane; c += blockDim.x) {
+        const int i = (int)(c % P.nx) + 1, j = (int)(c / P.nx) + 1;
+        if (p[pindex(P, i, j, k)] > cut) acc = acc + u[EQ_DE * ncell + (int64_t)(k - 1) * plane + c] * vol;
+    }
+    __shared__ double sh[4];
+    const double r = block_reduce(RED_SUM, acc, sh);
+    if (threadIdx.x == 0) partial[k - 1] = r;
+}
+
+// calc_m_dot (output.f90:951-993) on cell i = probe: D (alpha v^x - beta^x) dy dz
+// summed over (j, k), one partial per block.
+template <int MK>
+__global__ __launch_bounds__(256) void k_mdot(Params P, int probe, const double *__restrict__ p,
+                                              const double *__restrict__ u, double *__restrict__ partial)
+{
+    const int64_t ncell = (int64_t)P.nx * P.ny * P.nz, njk = (int64_t)P.ny * P.nz;
+    double acc = 0.0;
+    for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < njk; c += (int64_t)gridDim.x * blockDim.x) {
+        const int j = (int)(c % P.ny) + 1, k = (int)(c / P.ny) + 1;
+        double g[3][3], gi[3][3], sqg, alpha, beta[3];
+        metric_c2p<MK>(P, coord_cc(P, probe, j), g, gi, sqg, alpha, beta);
+        const double veff = alpha * p[EQ_VX * P.vstride + pindex(P, probe, j, k)] - beta[0];
+        const int64_t t = ((int64_t)(k - 1) * P.ny + (j - 1)) * P.nx + (probe - 1);
+        acc = acc + u[EQ_DE * ncell + t] * veff * P.dy * P.dz;
+    }
+    __shared__ double sh[4];
+    const double r = block_reduce(RED_SUM, acc, sh);
+    if (threadIdx.x == 0) partial[blockIdx.x] = r;
+}
+
 // ------------------------------------------------------------------ CFL --
 // update_adaptive_dt (evolution.f90:218-255): per-(i, j) light-cone speeds
 // (the metric does not depend on the third coordinate), wave64 max-reduction,
@@ -212,6 +322,29 @@ int launch_cfl(const Params &P, hipStream_t s, double *partial)
     const unsigned nb = blocks_for(n, 256);
     GRHD_DISPATCH(P.mk, (k_cfl<MKC><<<nb, 256, 0, s>>>(P, partial)));
     return (int)nb;
+}
+
+
+int launch_global(const Params &P, hipStream_t s, const double *p, const double *u, double *partial, int max_blocks)
+{
+    const int64_t ncell = (int64_t)P.nx * P.ny * P.nz;
+    const int nb = (int)std::max<int64_t>(1, std::min<int64_t>(max_blocks, (ncell + 255) / 256));
+    GRHD_DISPATCH(P.mk, (k_global<MKC><<<nb, 256, 0, s>>>(P, p, u, partial)));
+    return nb;
+}
+
+void launch_plane_mass(const Params &P, hipStream_t s, const double *p, const double *u, double *partial)
+{
+    k_plane_mass<<<P.nz, 256, 0, s>>>(P, p, u, partial);
+}
+
+int launch_mdot(const Params &P, hipStream_t s, int probe, const double *p, const double *u, double *partial,
+                int max_blocks)
+{
+    const int64_t njk = (int64_t)P.ny * P.nz;
+    const int nb = (int)std::max<int64_t>(1, std::min<int64_t>(max_blocks, (njk + 255) / 256));
+    GRHD_DISPATCH(P.mk, (k_mdot<MKC><<<nb, 256, 0, s>>>(P, probe, p, u, partial)));
+    return nb;
 }
 
 }  // namespace grhd

@@ -32,5 +32,13 @@ void launch_c2p(const Params &P, hipStream_t s, double *p, double *u, unsigned l
 int launch_cfl(const Params &P, hipStream_t s, double *partial);
 // block partial sums of the mass monitor (at most max_blocks); returns the block count
 int launch_mass(const Params &P, hipStream_t s, const double *D, double *partial, int max_blocks);
+// observers: global diagnostics (10 partials per block), per-plane masses (one
+// per phi plane), accretion-rate partials on the probe cell
+constexpr int kGlobalPartials = 10;
+constexpr int kPpiModes = 4;  // PPI_MAX_MODE (variables.f90:141)
+int launch_global(const Params &P, hipStream_t s, const double *p, const double *u, double *partial, int max_blocks);
+void launch_plane_mass(const Params &P, hipStream_t s, const double *p, const double *u, double *partial);
+int launch_mdot(const Params &P, hipStream_t s, int probe, const double *p, const double *u, double *partial,
+                int max_blocks);
 
 }  // namespace grhd

@@ -19,6 +19,7 @@
 program grhd2_hip
   use grhd_c
   use grhd_par
+  use grhd_ckpt
   implicit none
 
   type(grhd_config) :: cfg
@@ -31,7 +32,9 @@ program grhd2_hip
   logical :: check_only, perturbed
   real(c_double) :: t, last_dt, dt, mass, norms(3)
   integer(c_int32_t) :: imax
-  real(c_double), allocatable :: u(:,:,:,:)
+  real(c_double), allocatable :: u(:,:,:,:), u_restart(:,:,:,:)
+  type(checkpoint_state) :: cks
+  real(c_double) :: next_checkpoint
   integer :: nargs, nx, ny, nz, unit_out
   integer(c_int64_t) :: nsteps
   real(c_double) :: wall1
@@ -56,8 +59,11 @@ program grhd2_hip
 
   call read_par(trim(parfile), cfg, rc)
   if (rc%restart) then
-    write(*,*) 'ERROR: restart from a checkpoint is not supported by this driver'
-    error stop 1
+    ! restart branch of grhd2.f90:63-117: the problem, its parameters and
+    ! state come from the checkpoint, the .par may change the run control only
+    write(*,'(A,A)') '>>> restarting from checkpoint ', trim(rc%restart_file)
+    call read_checkpoint(rc%restart_file, cfg, rc, cks, u_restart)
+    call read_restart_par(trim(parfile), cfg, rc)
   end if
 
   st = grhd_validate(cfg)
@@ -69,7 +75,14 @@ program grhd2_hip
   write(*,'(A,3(I0,1X))') 'grid nx ny nz: ', cfg%nx, cfg%ny, cfg%nz
   write(*,'(A,ES12.5,A,ES12.5)') 'final_time: ', cfg%final_time, '  cfl: ', cfg%cfl
   if (check_only) then
-    write(*,'(A)') 'parameter file valid; simulation not started.'
+    if (rc%restart) then
+      write(*,'(A,ES22.14,A,I0)') 'checkpoint at t = ', cks%t, '  step ', cks%n_steps
+      write(*,'(A,L1,A,L1,A,I0)') 'observers: extract_mdot ', rc%extract_mdot, '  ppi_diagnostics ', &
+        rc%ppi_diagnostics, '  diagnostic_stride ', rc%diagnostic_stride
+      write(*,'(A)') 'parameter file and checkpoint valid; simulation not started.'
+    else
+      write(*,'(A)') 'parameter file valid; simulation not started.'
+    end if
     stop
   end if
 
@@ -81,16 +94,30 @@ program grhd2_hip
   call require(grhd_dims(ctx, dims), 'grhd_dims')
   nx = dims(1); ny = dims(2); nz = dims(3)
 
+  ! the initial conditions also build the case's boundary data (the Michel
+  ! injector, prepare_case_boundary_data); a restart then replaces the state:
+  ! U from the checkpoint, cons_to_prim and the boundaries (grhd2.f90:95-115)
   call require(grhd_initial_conditions(ctx), 'grhd_initial_conditions')
+  if (rc%restart) call require(grhd_set_conserved(ctx, u_restart), 'grhd_set_conserved')
   allocate(u(5, nx, ny, nz))
   call require(grhd_total_mass(ctx, mass), 'grhd_total_mass')
   write(*,'(A,ES22.14)') 'Initial mass in domain: ', mass
   call require(grhd_compute_dt(ctx, dt), 'grhd_compute_dt')
   write(*,'(A,ES22.14)') 'dt (CFL, metric light cone): ', dt
 
-  t = 0.0d0
-  nsteps = 0
-  perturbed = (rc%perturbation == 0)
+  t = cks%t
+  nsteps = cks%n_steps
+  ! observers at the start (grhd2.f90:139-145)
+  call execute_command_line('mkdir -p '//trim(rc%output_folder)//'/'//trim(scheme_name_of(cfg)))
+  call observers(rc%extract_mdot, rc%ppi_diagnostics)
+  perturbed = (rc%perturbation == 0) .or. cks%perturbation_applied
+  ! checkpoints every checkpoint_interval of time (grhd2.f90:117,123,338-341)
+  if (rc%restart) then
+    next_checkpoint = dble(int(t / rc%checkpoint_interval) + 1) * rc%checkpoint_interval
+  else
+    next_checkpoint = rc%checkpoint_interval
+  end if
+  call execute_command_line('mkdir -p '//trim(rc%output_folder))
   chunk = int(max(1, rc%mass_monitor_stride), c_int32_t)
   if (rc%mass_monitor_stride <= 0) chunk = huge(chunk)
   call system_clock(c0, crate)
@@ -100,15 +127,33 @@ program grhd2_hip
                                 int(rc%perturbation_seed, c_int64_t)), 'grhd_perturb')
       perturbed = .true.
       write(*,'(A,ES14.6)') '>>> perturbation injected at t = ', t
+      call observers(.false., rc%ppi_diagnostics)   ! grhd2.f90:222-225
     end if
     ! up to the next multiple of the monitor stride (grhd2.f90:282-301)
     n_adv = chunk
     if (rc%mass_monitor_stride > 0) &
       n_adv = int(rc%mass_monitor_stride - mod(nsteps, int(rc%mass_monitor_stride, 8)), c_int32_t)
+    if (rc%extract_mdot) n_adv = min(n_adv, int(rc%extraction_stride - &
+                                     mod(nsteps, int(rc%extraction_stride, 8)), c_int32_t))
+    if (rc%ppi_diagnostics) n_adv = min(n_adv, int(rc%diagnostic_stride - &
+                                        mod(nsteps, int(rc%diagnostic_stride, 8)), c_int32_t))
     if (.not. perturbed) n_adv = min(n_adv, steps_until(rc%perturbation_time))
+    n_adv = max(1_c_int32_t, min(n_adv, steps_until(next_checkpoint)))
     call require(grhd_advance(ctx, cfg%final_time, n_adv, t, done, last_dt), 'grhd_advance')
     nsteps = nsteps + done
     if (done == 0) exit
+    ! observers every extraction_stride / diagnostic_stride steps (grhd2.f90:266-280)
+    call observers(rc%extract_mdot .and. mod(nsteps, int(rc%extraction_stride, 8)) == 0, &
+                   rc%ppi_diagnostics .and. mod(nsteps, int(rc%diagnostic_stride, 8)) == 0)
+    if (t >= next_checkpoint) then
+      call require(grhd_get_conserved(ctx, u), 'grhd_get_conserved')
+      cks%t = t
+      cks%n_steps = int(nsteps)
+      cks%perturbation_applied = perturbed .and. rc%perturbation /= 0
+      call write_checkpoint(checkpoint_file_name(cfg, rc, int(nsteps)), cfg, rc, cks, u)
+      write(*,'(A,A)') '>>> checkpoint ', trim(checkpoint_file_name(cfg, rc, int(nsteps)))
+      next_checkpoint = next_checkpoint + rc%checkpoint_interval
+    end if
     if (rc%mass_monitor_stride > 0 .and. mod(nsteps, int(rc%mass_monitor_stride, 8)) == 0) then
       call require(grhd_total_mass(ctx, mass), 'grhd_total_mass')
       write(*,'(A,I0,A,ES14.6,A,ES22.14)') 'step ', nsteps, '  t = ', t, '  total mass: ', mass
@@ -135,7 +180,6 @@ program grhd2_hip
       ' | L2 = ', norms(2), ' | Linf = ', norms(3), ' at cell ', imax
   end if
 
-  call execute_command_line('mkdir -p '//trim(rc%output_folder))
   open(newunit=unit_out, file=trim(rc%output_folder)//'/'//trim(rc%output_prefix)//'_final_u.bin', &
        access='stream', form='unformatted', status='replace')
   write(unit_out) u
@@ -158,6 +202,39 @@ contains
       steps_until = steps_until + 1
     end do
   end function steps_until
+
+  ! calc_m_dot, calc_ppi_modes and calc_global_diagnostics (output.f90:951-1125)
+  ! from the device reductions, appended to <output_folder>/<scheme>/ with the
+  ! reference's record formats
+  subroutine observers(do_mdot, do_ppi)
+    logical, intent(in) :: do_mdot, do_ppi
+    character(len=1024) :: dir
+    real(c_double) :: mdot, modes(17), glob(8)
+    integer(c_int64_t) :: cnt(2)
+    integer(c_int32_t) :: probe
+    integer :: uo
+    dir = trim(rc%output_folder)//'/'//trim(scheme_name_of(cfg))
+    ! calc_m_dot returns for flat metrics and M < 0.1 (output.f90:959)
+    if (do_mdot .and. cfg%metric /= 0 .and. cfg%bh_mass >= 0.1d0) then
+      call require(grhd_accretion_rate(ctx, mdot, probe), 'grhd_accretion_rate')
+      open(newunit=uo, file=trim(dir)//'/m_dot.dat', position='append', status='unknown')
+      write(uo, '(2(E15.7, 2X))') t, mdot
+      close(uo)
+    end if
+    if (do_ppi) then
+      if (nz >= 2) then
+        call require(grhd_ppi_modes(ctx, modes), 'grhd_ppi_modes')
+        open(newunit=uo, file=trim(dir)//'/ppi_modes.dat', position='append', status='unknown', action='write')
+        write(uo, '(*(ES24.16E3,1X))') t, modes
+        close(uo)
+      end if
+      call require(grhd_global_diagnostics(ctx, glob, cnt), 'grhd_global_diagnostics')
+      open(newunit=uo, file=trim(dir)//'/global_diagnostics.dat', position='append', status='unknown', &
+           action='write')
+      write(uo, '(9(ES24.16E3,1X),2(I0,1X))') t, glob, cnt
+      close(uo)
+    end if
+  end subroutine observers
 
   subroutine require(status, what)
     integer(c_int), intent(in) :: status

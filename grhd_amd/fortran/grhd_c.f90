@@ -14,7 +14,8 @@ module grhd_c
   public :: grhd_create, grhd_validate, grhd_destroy, grhd_last_error, grhd_dims, grhd_initial_conditions
   public :: grhd_set_primitives, grhd_get_primitives, grhd_set_conserved, grhd_get_conserved
   public :: grhd_compute_dt, grhd_step, grhd_advance, grhd_perturb, grhd_counters
-  public :: grhd_total_mass, grhd_convergence_norms
+  public :: grhd_total_mass, grhd_convergence_norms, grhd_global_diagnostics, grhd_ppi_modes
+  public :: grhd_accretion_rate
   public :: grhd_last_timing, grhd_synchronize, grhd_error_message
 
   integer, parameter, public :: GRHD_OK = 0
@@ -143,6 +144,23 @@ module grhd_c
       real(c_double), intent(out) :: norms(3)
       integer(c_int32_t), intent(out) :: imax
     end function grhd_convergence_norms
+    integer(c_int) function grhd_global_diagnostics(ctx, out, counts) bind(C, name='grhd_global_diagnostics')
+      import :: c_int, c_ptr, c_double, c_int64_t
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: out(8)
+      integer(c_int64_t), intent(out) :: counts(2)
+    end function grhd_global_diagnostics
+    integer(c_int) function grhd_ppi_modes(ctx, out) bind(C, name='grhd_ppi_modes')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: out(17)
+    end function grhd_ppi_modes
+    integer(c_int) function grhd_accretion_rate(ctx, mdot, probe) bind(C, name='grhd_accretion_rate')
+      import :: c_int, c_ptr, c_double, c_int32_t
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: mdot
+      integer(c_int32_t), intent(out) :: probe
+    end function grhd_accretion_rate
     integer(c_int) function grhd_last_timing(ctx, ms, launches) bind(C, name='grhd_last_timing')
       import :: c_int, c_ptr, c_double, c_int64_t
       type(c_ptr), value :: ctx

@@ -15,7 +15,7 @@ module grhd_par
   use grhd_c
   implicit none
   private
-  public :: run_control, read_par, problem_label
+  public :: run_control, read_par, read_restart_par, problem_label
 
   real(c_double), parameter :: PI = 3.141592653589793d0
 
@@ -28,7 +28,13 @@ module grhd_par
     real(c_double) :: perturbation_amplitude = 0.01d0
     integer :: mass_monitor_stride = 1000
     real(c_double) :: save_interval = 1.0d0
+    real(c_double) :: checkpoint_interval = 100.0d0   ! variables.f90:264
+    ! observers (variables.f90:132-148): accretion rate every extraction_stride
+    ! steps, azimuthal modes and global diagnostics every diagnostic_stride
+    logical :: extract_mdot = .false., ppi_diagnostics = .false.
+    integer :: extraction_stride = 10, diagnostic_stride = 100
     logical :: restart = .false.
+    character(len=1024) :: restart_file = ' '
   end type run_control
 
 contains
@@ -252,6 +258,11 @@ contains
       if (pass == 2) then
         call apply_preset(c, pid)
         prefix = ic_prefix(pid)
+        ! the presets' observers (initialization.f90:656-802): the accretion
+        ! rate for Michel, dust and the sagittal and 3D tori, the azimuthal
+        ! modes for the equatorial and 3D tori
+        rc%extract_mdot = any(pid == [7, 8, 11, 12])
+        rc%ppi_diagnostics = any(pid == [10, 12])
       end if
       ln = 0
       nseen = 0
@@ -278,6 +289,10 @@ contains
           if (trim(key) == 'problem') then
             pid = problem_id(val)
             if (pid < 0) call fail(file_name, ln, 'unknown problem: '//trim(val))
+          else if (trim(key) == 'restart') then
+            rc%restart = parse_bool(val, file_name, ln) == 1
+          else if (trim(key) == 'restart_file') then
+            rc%restart_file = unquote_long(val)
           end if
           cycle
         end if
@@ -286,11 +301,62 @@ contains
         call apply_key(c, rc, key, val, file_name, ln)
       end do
       close(u)
+      ! a restart takes the problem from the checkpoint (parameters.f90:84-91);
+      ! the caller reads it, then applies the restart-control keys
+      if (pass == 1 .and. rc%restart) then
+        if (len_trim(rc%restart_file) == 0) &
+          call fail(file_name, 0, 'restart = true requires the parameter restart_file')
+        return
+      end if
       if (pass == 1 .and. pid < 0) &
         call fail(file_name, 0, 'missing mandatory parameter problem for a cold start')
     end do
     if (c%metric == 2 .and. c%geometry == 2) c%geometry = 3   ! parameters.f90:428-431
   end subroutine read_par
+
+  ! Keys a restart may change (is_restart_control, parameters.f90:716-731), plus
+  ! this build's precision; every other key belongs to the checkpoint.
+  logical function is_restart_control(key)
+    character(len=*), intent(in) :: key
+    select case(trim(key))
+    case('restart', 'restart_file', 'final_time', 'cfl', 'save_interval', &
+         'checkpoint_interval', 'output_prefix', 'output_folder', 'vtk_mapping', &
+         'shock_sensor', 'extract_gw', 'extract_mdot', 'ppi_diagnostics', 'diagnostic_stride', &
+         'extraction_stride', 'mass_monitor_stride', 'perturbation', 'perturbation_seed', &
+         'perturbation_time', 'perturbation_amplitude', 'perturbation_mode', 'precision')
+      is_restart_control = .true.
+    case default
+      is_restart_control = .false.
+    end select
+  end function is_restart_control
+
+  ! The .par of a restart over the configuration read from the checkpoint
+  ! (read_parameter_file with restart_mode, parameters.f90:95-160).
+  subroutine read_restart_par(file_name, c, rc)
+    character(len=*), intent(in) :: file_name
+    type(grhd_config), intent(inout) :: c
+    type(run_control), intent(inout) :: rc
+    character(len=1024) :: line, key, val
+    integer :: u, ios, ln, eq, hash
+    open(newunit=u, file=trim(file_name), status='old', action='read', iostat=ios)
+    if (ios /= 0) call fail(file_name, 0, 'cannot open the parameter file')
+    ln = 0
+    do
+      read(u, '(A)', iostat=ios) line
+      if (ios < 0) exit
+      ln = ln + 1
+      hash = index(line, '#')
+      if (hash > 0) line(hash:) = ' '
+      if (len_trim(line) == 0) cycle
+      eq = index(line, '=')
+      key = lower(trim(adjustl(line(:eq-1))))
+      val = trim(adjustl(line(eq+1:)))
+      if (.not. is_restart_control(key)) call fail(file_name, ln, &
+        'this parameter belongs to the checkpoint and cannot change on restart: '//trim(key))
+      call apply_key(c, rc, key, val, file_name, ln)
+    end do
+    close(u)
+  end subroutine read_restart_par
 
   function ic_prefix(id) result(p)
     integer, intent(in) :: id
@@ -328,9 +394,23 @@ contains
     select case(trim(key))
     case('problem')
     case('restart'); rc%restart = parse_bool(val, f, ln) == 1
-    case('restart_file', 'vtk_mapping', 'extract_gw', 'extract_mdot', 'ppi_diagnostics', &
-         'diagnostic_stride', 'extraction_stride', 'checkpoint_interval')
-      ! observers and checkpoints: accepted, not part of the update path
+    case('restart_file'); rc%restart_file = unquote_long(val)
+    case('checkpoint_interval')
+      call parse_real(val, x, f, ln)
+      if (.not. (x > 0.0d0)) call fail(f, ln, 'checkpoint_interval must be > 0')
+      rc%checkpoint_interval = x
+    case('extract_mdot'); rc%extract_mdot = parse_bool(val, f, ln) == 1
+    case('ppi_diagnostics'); rc%ppi_diagnostics = parse_bool(val, f, ln) == 1
+    case('diagnostic_stride', 'extraction_stride')
+      call parse_int(val, n, f, ln)
+      if (n < 1) call fail(f, ln, trim(key)//' must be >= 1')
+      if (trim(key) == 'diagnostic_stride') then
+        rc%diagnostic_stride = n
+      else
+        rc%extraction_stride = n
+      end if
+    case('vtk_mapping', 'extract_gw')
+      ! the VTK writer and the gravitational-wave strain are not built
     case('metric')
       select case(trim(lower(adjustl(val))))
       case('minkowski'); c%metric = 0
@@ -469,6 +549,18 @@ contains
       end select
     end select
   end subroutine apply_key
+
+  function unquote_long(text) result(s)
+    character(len=*), intent(in) :: text
+    character(len=1024) :: s
+    integer :: n
+    s = adjustl(text)
+    n = len_trim(s)
+    if (n >= 2) then
+      if ((s(1:1) == '"' .and. s(n:n) == '"') .or. (s(1:1) == "'" .and. s(n:n) == "'")) &
+        s = s(2:n-1)
+    end if
+  end function unquote_long
 
   function unquote(text) result(s)
     character(len=*), intent(in) :: text

@@ -168,6 +168,30 @@ class Solver:
         self._chk(self._L.grhd_convergence_norms(self.h, float(t), n, ctypes.byref(im)))
         return (n[0], n[1], n[2]), im.value
 
+    def global_diagnostics(self) -> dict:
+        """calc_global_diagnostics (output.f90:1052-1125), reduced on the device."""
+        o = (ctypes.c_double * 8)()
+        n = (ctypes.c_int64 * 2)()
+        self._chk(self._L.grhd_global_diagnostics(self.h, o, n))
+        keys = ("mass", "disk_mass", "angular_momentum", "rho_min", "p_min", "rho_max", "p_max", "v2_max")
+        d = {k: o[i] for i, k in enumerate(keys)}
+        d.update(atmosphere_cells=n[0], invalid_cells=n[1])
+        return d
+
+    def ppi_modes(self) -> np.ndarray:
+        """calc_ppi_modes (output.f90:995-1049): c0, then per m = 1..4 Re, Im,
+        amplitude and phase."""
+        o = (ctypes.c_double * 17)()
+        self._chk(self._L.grhd_ppi_modes(self.h, o))
+        return np.array(o[:])
+
+    def accretion_rate(self):
+        """calc_m_dot (output.f90:951-993): (mdot, probe cell)."""
+        m = ctypes.c_double()
+        ip = ctypes.c_int32()
+        self._chk(self._L.grhd_accretion_rate(self.h, ctypes.byref(m), ctypes.byref(ip)))
+        return m.value, ip.value
+
     def synchronize(self):
         self._chk(self._L.grhd_synchronize(self.h))
 

@@ -185,6 +185,24 @@ int grhd_total_mass(grhd_ctx *ctx, double *mass);
  * GRHD_ERR_UNSUPPORTED for the other problems (the reference prints nothing
  * for them). */
 int grhd_convergence_norms(grhd_ctx *ctx, double t, double *norms, int32_t *imax);
+/* calc_global_diagnostics (output.f90:1052-1125) of the current state,
+ * reduced on the device: out[8] = total mass, disk mass (rho > 1e3 rho_floor),
+ * angular momentum (sum of S_3), rho min, p min, rho max, p max (finite values)
+ * and the largest finite v^2 = gamma_ij v^i v^j; counts[2] = atmosphere cells,
+ * invalid cells.  The sums are added in another order than the reference's
+ * loop; minima, maxima and counts are exact. */
+int grhd_global_diagnostics(grhd_ctx *ctx, double *out, int64_t *counts);
+/* calc_ppi_modes (output.f90:995-1049): out[1 + 4 M], M = 4 = c0, then for
+ * m = 1..M: Re and Im of sum_k m_k exp(-i m phi_k), amplitude |.|/c0 and phase
+ * -atan2(Im, Re)/m, where m_k is the mass of phi plane k above the density cut
+ * (planes reduced on the device, modes summed over k on the host in the
+ * reference's order).  GRHD_ERR_UNSUPPORTED when nz < 2. */
+int grhd_ppi_modes(grhd_ctx *ctx, double *out);
+/* calc_m_dot (output.f90:951-993): -sum over (j, k) of D (alpha v^x - beta^x)
+ * dy dz on the probe cell, the first centre at or outside the event horizon
+ * (initialization.f90:117-190); *probe (may be NULL) = its index.
+ * GRHD_ERR_UNSUPPORTED for Minkowski metrics, M < 0.1 or no such cell. */
+int grhd_accretion_rate(grhd_ctx *ctx, double *mdot, int32_t *probe);
 
 /* Unit entry points for parity checks. */
 /* set_boundary_conditions on the device primitives (conditions.f90:773-1066). */
@@ -237,7 +255,8 @@ int grhd_batch_problem(grhd_batch *b, int32_t slot, grhd_ctx **ctx);
 int grhd_batch_remove(grhd_batch *b, int32_t slot);
 /* Advances every unfinished problem by nsteps steps of the reference time
  * loop (grhd2.f90:179-265: dt, final-time clamp, three stages), each with
- * its own dt and time; a problem stops at its final_time. */
+ * its own dt and time; a problem stops at its final_time (with pacing,
+ * cheaper groups take more steps: grhd_batch_set_pacing). */
 int grhd_batch_step(grhd_batch *b, int32_t nsteps);
 /* done: final_time reached; t: its time; steps: steps taken in the batch. */
 int grhd_batch_status(grhd_batch *b, int32_t slot, int32_t *done, double *t, int64_t *steps);
@@ -246,6 +265,16 @@ int grhd_batch_status(grhd_batch *b, int32_t slot, int32_t *done, double *t, int
  * and ghost fill with 4/1 waves per face-geometry column; 4-5 one persistent
  * kernel per stage with 4/1.  All forms give identical results. */
 int grhd_batch_set_variant(grhd_batch *b, int32_t variant);
+/* Pacing (no reference counterpart; default 1).  With max_ratio > 1 and the
+ * groups of different metric kinds or arithmetic on their own streams (the
+ * phase launch forms), a group whose step is cheaper than the most expensive
+ * group's takes up to max_ratio (<= 8) steps per step of that group in each
+ * chunk, so small problems are not held to the step rate of the largest.
+ * grhd_batch_step(nsteps) then advances the most expensive group nsteps steps
+ * and cheaper ones up to max_ratio x nsteps; every problem still stops at its
+ * final time, grhd_batch_status reports its steps, and its state equals its
+ * state alone after that many steps, bit for bit. */
+int grhd_batch_set_pacing(grhd_batch *b, int32_t max_ratio);
 /* variant in use, unfinished problems, admitted bytes, device time (ms) and
  * kernel launches of the last grhd_batch_step. */
 int grhd_batch_info(grhd_batch *b, int32_t *variant, int32_t *active, uint64_t *hbm_used,

@@ -96,6 +96,10 @@ def ref_lib():
         lib = ctypes.CDLL(str(REF_SO))
         lib.ref_init.restype = ctypes.c_int
         lib.ref_init.argtypes = [ctypes.c_char_p, ctypes.c_int]
+        lib.ref_restart.restype = ctypes.c_int
+        lib.ref_restart.argtypes = [ctypes.c_char_p, ctypes.c_int]
+        lib.ref_save_checkpoint.argtypes = [ctypes.c_char_p, ctypes.c_int]
+        lib.ref_observers.argtypes = [ctypes.c_char_p, ctypes.c_int]
         lib.ref_compute_dt.restype = ctypes.c_double
         lib.ref_dims.argtypes = [ctypes.POINTER(ctypes.c_int)]
         for name in ("ref_scalars", "ref_get_prim", "ref_set_prim", "ref_put_prim",
@@ -254,11 +258,23 @@ class Reference:
     live per process; creating a second one releases the first.
     """
 
-    def __init__(self, cfg=None, par_text: str | None = None, workdir: str | None = None):
+    def __init__(self, cfg=None, par_text: str | None = None, workdir: str | None = None,
+                 restart: bool = False):
+        """Cold start from cfg or par_text; with restart=True, par_text is a
+        restart .par (restart = true, restart_file = ...) and the problem comes
+        from the checkpoint through the reference's restart branch."""
         from grhd_amd.params import config_to_par
 
         self.lib = ref_lib()
         self.cfg = cfg
+        if restart:
+            # restart_file is character(len=100) in the reference
+            # (variables.f90:267); a longer value makes its parser end the
+            # process with error stop (parameters.f90:879-881)
+            for line in (par_text or "").splitlines():
+                key, _, value = line.partition("=")
+                if key.strip().lower() == "restart_file" and len(value.strip().strip("'\"")) > 100:
+                    raise ValueError("restart_file longer than the reference's 100 characters")
         self._tmp = tempfile.TemporaryDirectory(prefix="grhd_ref_", dir=workdir)
         out = os.path.join(self._tmp.name, "out")
         text = par_text if par_text is not None else config_to_par(cfg, output_folder=out)
@@ -271,11 +287,11 @@ class Reference:
         cwd = os.getcwd()
         os.chdir(self._tmp.name)
         try:
-            rc = self.lib.ref_init(b, len(b))
+            rc = (self.lib.ref_restart if restart else self.lib.ref_init)(b, len(b))
         finally:
             os.chdir(cwd)
         if rc != 0:
-            raise RuntimeError("reference init failed")
+            raise RuntimeError(f"reference {'restart' if restart else 'init'} failed ({rc})")
         dims = (ctypes.c_int * 8)()
         self.lib.ref_dims(dims)
         self.nx, self.ny, self.nz, self.ng = dims[0], dims[1], dims[2], dims[3]
@@ -335,6 +351,34 @@ class Reference:
         st = np.zeros(3)
         self.lib.ref_step(nsteps, _ptr(st))
         return st[0], st[1], int(st[2])
+
+    def save_checkpoint(self, folder):
+        """The reference's save_checkpoint of the current state into folder;
+        returns the file's path."""
+        import glob
+        os.makedirs(folder, exist_ok=True)
+        b = str(folder).encode()
+        before = set(glob.glob(os.path.join(folder, "checkpoint_*.rst")))
+        self.lib.ref_save_checkpoint(b, len(b))
+        new = sorted(set(glob.glob(os.path.join(folder, "checkpoint_*.rst"))) - before)
+        if len(new) != 1:
+            raise RuntimeError("reference checkpoint not written")
+        return new[0]
+
+    def observers(self, folder, scheme_name):
+        """The reference's calc_global_diagnostics, calc_ppi_modes and
+        calc_m_dot of the current state: the last line of each .dat file they
+        append to under folder/scheme_name (missing files give None)."""
+        sub = os.path.join(folder, scheme_name)
+        os.makedirs(sub, exist_ok=True)
+        b = str(folder).encode()
+        self.lib.ref_observers(b, len(b))
+        out = {}
+        for name in ("global_diagnostics", "ppi_modes", "m_dot"):
+            path = os.path.join(sub, name + ".dat")
+            out[name] = (np.array([float(x) for x in open(path).read().split("\n")[-2].split()])
+                         if os.path.exists(path) else None)
+        return out
 
     def close(self):
         self.lib.ref_release()

@@ -21,6 +21,8 @@ module grhd_ref_harness
   use parameters
   use metrics
   use conditions
+  use output, only: save_checkpoint, read_checkpoint_metadata, read_checkpoint_state, &
+                    calc_global_diagnostics, calc_ppi_modes, calc_m_dot
   use equations
   use fluxes
   use initialization
@@ -130,6 +132,85 @@ contains
     problem_ready = .true.
     ref_init = 0
   end function ref_init
+
+  ! The restart branch of the reference main program (grhd2.f90:57,63-117):
+  ! the .par header (restart and restart_file), the checkpoint's metadata, the
+  ! .par's restart-control keys, validation, floors, metric binding, grid,
+  ! metric cache, wave speeds, the checkpoint's state, the case's boundary
+  ! data, u = up, cons_to_prim and the boundaries.  Returns 1 when the .par
+  ! does not ask for a restart and 2 when the checkpoint cannot be opened.
+  integer(c_int) function ref_restart(path, n) bind(C, name='ref_restart')
+    character(kind=c_char), intent(in) :: path(*)
+    integer(c_int), value :: n
+    character(len=1024) :: fname
+    integer :: i, j, k, io_status
+    call c_to_f_string(path, n, fname)
+    call release_problem()
+    call reset_globals()
+    call read_parameter_header(trim(fname))
+    if (.not. do_restart) then
+      ref_restart = 1
+      return
+    end if
+    open(20, file=trim(restart_file), status='old', form='unformatted', iostat=io_status)
+    if (io_status /= 0) then
+      ref_restart = 2
+      return
+    end if
+    call read_checkpoint_metadata(20)
+    call read_parameter_file(trim(fname), .true.)
+    call validate_parameters(trim(fname))
+    g1 = adb_idx / (adb_idx - 1.0d0)
+    rho_floor = 1.0d-10
+    p_floor   = rho_floor * 1.0d-3
+    D_floor   = rho_floor
+    tau_floor = p_floor * (g1 - 1.0d0)
+    call set_metric_type()
+    call allocate_and_grid()
+    call precalculate_metric_cache()
+    call init_wavespeed_solver()
+    call read_checkpoint_state(20, n_steps, integration_time)
+    close(20)
+    call prepare_case_boundary_data()
+    u = up
+    do k = 1, nz
+      do j = 1, ny
+        do i = 1, nx
+          call cons_to_prim(u(:,i,j,k), p(:,i,j,k), i, j, k)
+        end do
+      end do
+    end do
+    call set_boundary_conditions(p)
+    problem_ready = .true.
+    ref_restart = 0
+  end function ref_restart
+
+  ! save_checkpoint (output.f90:474-527) of the current state and time into
+  ! folder dir: <dir>/checkpoint_<scheme_name>_step_<n_steps, 9 digits>.rst
+  subroutine ref_save_checkpoint(dir, n) bind(C, name='ref_save_checkpoint')
+    character(kind=c_char), intent(in) :: dir(*)
+    integer(c_int), value :: n
+    character(len=256) :: d
+    call c_to_f_string(dir, n, d)
+    output_folder = d
+    call save_checkpoint(n_steps, integration_time)
+  end subroutine ref_save_checkpoint
+
+  ! The reference's observers of the current state (output.f90:951-1125),
+  ! appended to <dir>/<scheme_name>/{global_diagnostics,ppi_modes,m_dot}.dat;
+  ! the azimuthal modes and global diagnostics are switched on for the call.
+  subroutine ref_observers(dir, n) bind(C, name='ref_observers')
+    character(kind=c_char), intent(in) :: dir(*)
+    integer(c_int), value :: n
+    character(len=256) :: d
+    call c_to_f_string(dir, n, d)
+    output_folder = d
+    do_ppi_diagnostics = .true.
+    call calc_global_diagnostics(integration_time)
+    call calc_ppi_modes(integration_time)
+    call calc_m_dot(integration_time)
+    do_ppi_diagnostics = .false.
+  end subroutine ref_observers
 
   subroutine ref_release() bind(C, name='ref_release')
     call release_problem()

@@ -44,10 +44,11 @@ def _alone(name, nsteps, precision=None):
         return S.get_conserved(), S.get_primitives(), S.t, n
 
 
-def _batched(names, nsteps, variant=0, precision=None, chunks=1):
+def _batched(names, nsteps, variant=0, precision=None, chunks=1, pacing=1):
     out = {}
     with Batch(capacity=len(names) + 2) as B:
         B.set_variant(variant)
+        B.set_pacing(pacing)
         slots = {}
         for n in names:
             slots[n] = B.add(_cfg(n, precision))
@@ -93,6 +94,38 @@ def test_mixed_precision_problems_match_alone():
     for n in names:
         u, _, _, _ = _alone(n, NSTEPS, precision="mixed")
         assert np.array_equal(res[n][0], u), n
+
+
+@pytest.mark.parametrize("variant", [1, 3])
+def test_paced_batch_matches_alone(variant):
+    """With pacing, groups whose steps are cheaper take more steps per chunk
+    (up to 8 per step of the most expensive group): beside a 512^2
+    Kelvin-Helmholtz problem (a step of about 190 us by the batch's estimate)
+    the Michel and shock-tube groups (about 54 us) take three steps per chunk
+    step.  Each problem's state equals its state alone after the steps it
+    took, bit for bit."""
+    big = CASES["khi_weno3_sensor"].replace("nx = 32", "nx = 512").replace("nz = 32", "nz = 512")
+    texts = {"khi_512": big, "michel_ef_log": CASES["michel_ef_log"], "sod_c1": CASES["sod_c1"]}
+    print(f"paced batch variant {variant}", flush=True)
+    res = {}
+    with Batch(capacity=4) as B:
+        B.set_variant(variant)
+        B.set_pacing(8)
+        slots = {n: B.add(config_from_par(t)) for n, t in texts.items()}
+        B.step(4)
+        B.step(3)
+        for n in texts:
+            res[n] = (B.problem(slots[n]).get_conserved(), B.status(slots[n])[2])
+    taken = {n: res[n][1] for n in texts}
+    print("  steps taken", taken, flush=True)
+    assert taken["khi_512"] == NSTEPS
+    assert taken["michel_ef_log"] == 3 * NSTEPS and taken["sod_c1"] == 3 * NSTEPS
+    for n, text in texts.items():
+        with Solver(config_from_par(text)) as S:
+            S.initial_conditions()
+            na, _ = S.advance(max_steps=taken[n])
+            assert na == taken[n]
+            assert np.array_equal(res[n][0], S.get_conserved()), n
 
 
 def test_problems_stop_at_their_final_time():

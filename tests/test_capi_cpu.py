@@ -81,7 +81,8 @@ int main(void) {
     ok, fp, nbytes, bad, msg = int(out[0]), int(out[1]), int(out[2]), int(out[3]), out[4]
     assert ok == 0 and fp == 0 and bad == 1
     # p (407 x 5 doubles) + 2 U (400 x 5) + x faces (401 x 5) + coordinate
-    # table (3 x 407 + 3 x 401 + 2 x 7 + 2 x 2 doubles) + small buffers
-    base = 407 * 40 + 2 * 400 * 40 + 401 * 40 + (3 * 407 + 3 * 401 + 18) * 8
+    # table (3 x 407 + 3 x 401 + 2 x 7 + 2 x 2 doubles) + the reduction
+    # buffer of the CFL, mass and observer partials (4096 doubles) + small buffers
+    base = 407 * 40 + 2 * 400 * 40 + 401 * 40 + (3 * 407 + 3 * 401 + 18) * 8 + 4096 * 8
     assert base <= nbytes < base + 1024
     assert "reconstruction" in msg

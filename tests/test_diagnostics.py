@@ -115,3 +115,51 @@ def test_norms_of_other_problems_are_unsupported():
         with pytest.raises(GrhdError) as e:
             S.convergence_norms(0.0)
     assert e.value.code == 6
+
+
+@pytest.mark.parametrize("name", ["fm3d_ks_mp5_hllc_sensor", "fm3d_ef"])
+def test_observers_match_reference(name, tmp_path):
+    """The device's global diagnostics, azimuthal modes and accretion rate of
+    the reference's own state after three steps equal the reference's
+    calc_global_diagnostics, calc_ppi_modes and calc_m_dot (output.f90:951-1125):
+    counts, minima and maxima exactly, sums to 1e-12 relative (the reduction
+    order differs), the modes' phases to 1e-9 absolute, the accretion rate to
+    the 8 digits the reference writes."""
+    if not oracle.reference_available():
+        pytest.skip("reference build absent")
+    cfg = config_from_par(CASES[name])
+    R = oracle.Reference(cfg)
+    try:
+        R.set_time(0.0, 1.0e9)
+        R.step(3)
+        p, u = R.get_prim(), R.get_cons()
+        scheme = {4: "mp5", 5: "weno5"}[cfg.reconstruction] + ("_hllc" if cfg.riemann_solver == 2 else "_hlle")
+        ref = R.observers(str(tmp_path), scheme)
+    finally:
+        R.close()
+    print("observers", name, flush=True)
+    with Solver(cfg) as S:
+        S.initial_conditions()
+        S.put_primitives(p)
+        S.put_conserved(u)
+        g = S.global_diagnostics()
+        modes = S.ppi_modes()
+        mdot, probe = S.accretion_rate()
+    gr = ref["global_diagnostics"]
+    sums = [g["mass"], g["disk_mass"], g["angular_momentum"]]
+    assert np.allclose(sums, gr[1:4], rtol=1e-12, atol=1e-300), (sums, gr[1:4])
+    assert [g["rho_min"], g["p_min"], g["rho_max"], g["p_max"]] == list(gr[4:8])
+    assert g["v2_max"] == pytest.approx(gr[8], rel=1e-14)
+    assert (g["atmosphere_cells"], g["invalid_cells"]) == (int(gr[9]), int(gr[10]))
+    mr = ref["ppi_modes"]
+    assert np.allclose(modes[:1], mr[1:2], rtol=1e-12)
+    for m in range(4):
+        re, im, amp, ph = modes[1 + 4 * m: 5 + 4 * m]
+        rre, rim, ramp, rph = mr[2 + 4 * m: 6 + 4 * m]
+        scale = mr[1]
+        assert abs(re - rre) <= 1e-12 * scale and abs(im - rim) <= 1e-12 * scale
+        assert abs(amp - ramp) <= 1e-11
+        assert abs(ph - rph) <= 1e-9 or amp < 1e-12
+    # m_dot.dat holds 8 significant digits (E15.7, output.f90:989)
+    assert mdot == pytest.approx(ref["m_dot"][1], rel=1e-6)
+    assert probe >= 1

@@ -109,3 +109,43 @@ def test_driver_perturbation_and_monitor_match_python(tmp_path):
         ref = S.get_conserved()
     u = np.fromfile(tmp_path / "out" / "m_final_u.bin", dtype=np.float64).reshape(ref.shape)
     assert np.array_equal(u, ref)
+
+
+@pytest.mark.gpu
+def test_driver_observers_match_reference(tmp_path):
+    """ppi_diagnostics and extract_mdot in the driver: the records it appends
+    every diagnostic_stride / extraction_stride steps (grhd2.f90:139-145,
+    266-280) follow the reference's formats, and the first ones (the initial
+    state) equal the reference's own calc_global_diagnostics, calc_ppi_modes
+    and calc_m_dot of its initial state."""
+    import oracle
+    from grhd_amd.params import config_from_par
+    from tests.cases import CASES
+    if not oracle.reference_available():
+        pytest.skip("reference build absent")
+    text = CASES["fm3d_ef"]
+    par = tmp_path / "obs.par"
+    par.write_text(text + "final_time = 0.05\nppi_diagnostics = true\nextract_mdot = true\n"
+                   "diagnostic_stride = 2\nextraction_stride = 2\n"
+                   f"output_folder = {tmp_path}/out\noutput_prefix = o\n")
+    print("driver: observers on fm3d_ef", flush=True)
+    r = run([str(par)], tmp_path, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    steps = int(r.stdout.split("time steps: ")[1].split()[0])
+    d = tmp_path / "out" / "weno5_hllc"
+    glob = [np.array(l.split(), float) for l in (d / "global_diagnostics.dat").read_text().splitlines()]
+    modes = [np.array(l.split(), float) for l in (d / "ppi_modes.dat").read_text().splitlines()]
+    mdot = [np.array(l.split(), float) for l in (d / "m_dot.dat").read_text().splitlines()]
+    assert len(glob) == len(modes) == 1 + steps // 2 and len(mdot) == 1 + steps // 2
+    assert all(len(g) == 11 for g in glob) and all(len(m) == 18 for m in modes)
+    R = oracle.Reference(config_from_par(text))
+    try:
+        ref = R.observers(str(tmp_path / "ref"), "weno5_hllc")
+    finally:
+        R.close()
+    assert glob[0][0] == 0.0 and ref["global_diagnostics"][0] == 0.0
+    # the two initial states agree to 1e-13 (initial-data parity), not bit for bit
+    assert np.allclose(glob[0][1:9], ref["global_diagnostics"][1:9], rtol=1e-12)
+    assert np.array_equal(glob[0][9:], ref["global_diagnostics"][9:])
+    assert np.allclose(modes[0][1:2], ref["ppi_modes"][1:2], rtol=1e-12)
+    assert mdot[0][1] == pytest.approx(ref["m_dot"][1], rel=1e-6)
