@@ -94,6 +94,7 @@ _SIGS = {
     "grhd_set_primitives": ([ctypes.c_void_p, ctypes.POINTER(_D)], ctypes.c_int),
     "grhd_perturb": ([ctypes.c_void_p, _I, _D, ctypes.c_uint64], ctypes.c_int),
     "grhd_fishbone_unit_density_k": ([ctypes.POINTER(Config), ctypes.POINTER(_D)], ctypes.c_int),
+    "grhd_lapse_shift": ([ctypes.POINTER(Config), ctypes.POINTER(_D), ctypes.POINTER(_D)], ctypes.c_int),
     "grhd_put_primitives": ([ctypes.c_void_p, ctypes.POINTER(_D)], ctypes.c_int),
     "grhd_set_conserved": ([ctypes.c_void_p, ctypes.POINTER(_D)], ctypes.c_int),
     "grhd_put_conserved": ([ctypes.c_void_p, ctypes.POINTER(_D)], ctypes.c_int),

@@ -86,7 +86,7 @@ def build_driver(verbose: bool = True) -> Path:
     obj = out / "_obj"
     obj.mkdir(parents=True, exist_ok=True)
     objs = []
-    for src in ["grhd_c.f90", "grhd_par.f90", "grhd_ckpt.f90", "grhd2_hip.f90"]:
+    for src in ["grhd_c.f90", "grhd_par.f90", "grhd_ckpt.f90", "grhd_vtk.f90", "grhd2_hip.f90"]:
         o = obj / (Path(src).stem + ".o")
         cmd = [FC, "-O2", f"-J{obj}", "-c", str(FORTRAN / src), "-o", str(o)]
         if verbose:

@@ -533,6 +533,53 @@ int grhd_fishbone_unit_density_k(const grhd_config *cfg, double *K)
     return GRHD_OK;
 }
 
+extern "C++" {
+namespace {
+// alpha and beta^i at the interior cell centres, from the same metric
+// functions and coordinate evaluation the coordinate table uses
+// (precalculate_metric_cache, initialization.f90:283-299).  The metrics are
+// independent of the third coordinate, so one (i, j) column fills every k.
+template <int MK>
+void lapse_shift_mk(const Params &P, double *alpha, double *beta)
+{
+    for (int j = 1; j <= P.ny; j++) {
+        const double y = yc(P, j);
+        for (int i = 1; i <= P.nx; i++) {
+            FaceMetric m;
+            metric_face<MK>(P, coord_of(P.mk, xc(P, i), y), m);
+            for (int k = 0; k < P.nz; k++) {
+                const int64_t t = ((int64_t)k * P.ny + (j - 1)) * P.nx + (i - 1);
+                if (alpha) alpha[t] = m.alpha;
+                if (beta)
+                    for (int d = 0; d < 3; d++) beta[3 * t + d] = m.beta[d];
+            }
+        }
+    }
+}
+}  // namespace
+}  // extern "C++"
+
+int grhd_lapse_shift(const grhd_config *cfg, double *alpha, double *beta)
+{
+    if (!cfg || (!alpha && !beta)) return fail(nullptr, GRHD_ERR_INVALID_ARGUMENT, "null argument");
+    std::string msg;
+    if (validate(*cfg, msg)) return fail(nullptr, GRHD_ERR_INVALID_CONFIG, msg);
+    grhd_config c = *cfg;
+    if (c.metric == GRHD_METRIC_KERR_SCHILD) c.geometry = GRHD_GEOM_SPHEROIDAL;
+    const Params P = make_params(c);
+    switch (P.mk) {
+    case MK_MINK_CART: lapse_shift_mk<MK_MINK_CART>(P, alpha, beta); break;
+    case MK_MINK_CYL: lapse_shift_mk<MK_MINK_CYL>(P, alpha, beta); break;
+    case MK_MINK_SPH: lapse_shift_mk<MK_MINK_SPH>(P, alpha, beta); break;
+    case MK_MINK_SPH_LOG: lapse_shift_mk<MK_MINK_SPH_LOG>(P, alpha, beta); break;
+    case MK_EF: lapse_shift_mk<MK_EF>(P, alpha, beta); break;
+    case MK_EF_LOG: lapse_shift_mk<MK_EF_LOG>(P, alpha, beta); break;
+    case MK_KS: lapse_shift_mk<MK_KS>(P, alpha, beta); break;
+    default: lapse_shift_mk<MK_KS_LOG>(P, alpha, beta); break;
+    }
+    return GRHD_OK;
+}
+
 int grhd_put_primitives(grhd_ctx *c, const double *p)
 {
     if (!c || !p) return GRHD_ERR_INVALID_ARGUMENT;

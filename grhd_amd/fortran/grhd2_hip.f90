@@ -20,6 +20,7 @@ program grhd2_hip
   use grhd_c
   use grhd_par
   use grhd_ckpt
+  use grhd_vtk
   implicit none
 
   type(grhd_config) :: cfg
@@ -34,7 +35,7 @@ program grhd2_hip
   integer(c_int32_t) :: imax
   real(c_double), allocatable :: u(:,:,:,:), u_restart(:,:,:,:)
   type(checkpoint_state) :: cks
-  real(c_double) :: next_checkpoint
+  real(c_double) :: next_checkpoint, next_save
   integer :: nargs, nx, ny, nz, unit_out
   integer(c_int64_t) :: nsteps
   real(c_double) :: wall1
@@ -109,7 +110,12 @@ program grhd2_hip
   nsteps = cks%n_steps
   ! observers at the start (grhd2.f90:139-145)
   call execute_command_line('mkdir -p '//trim(rc%output_folder)//'/'//trim(scheme_name_of(cfg)))
+  ! the cold start's frame of the initial state (grhd2.f90:136)
+  if (.not. rc%restart) call write_vtk_frame(ctx, cfg, rc, t)
   call observers(rc%extract_mdot, rc%ppi_diagnostics)
+  ! frames on the absolute calendar of save_interval, also after a restart
+  ! (grhd2.f90:150)
+  next_save = dble(int(t / rc%save_interval) + 1) * rc%save_interval
   perturbed = (rc%perturbation == 0) .or. cks%perturbation_applied
   ! checkpoints every checkpoint_interval of time (grhd2.f90:117,123,338-341)
   if (rc%restart) then
@@ -138,6 +144,7 @@ program grhd2_hip
     if (rc%ppi_diagnostics) n_adv = min(n_adv, int(rc%diagnostic_stride - &
                                         mod(nsteps, int(rc%diagnostic_stride, 8)), c_int32_t))
     if (.not. perturbed) n_adv = min(n_adv, steps_until(rc%perturbation_time))
+    n_adv = min(n_adv, steps_until(next_save))
     n_adv = max(1_c_int32_t, min(n_adv, steps_until(next_checkpoint)))
     call require(grhd_advance(ctx, cfg%final_time, n_adv, t, done, last_dt), 'grhd_advance')
     nsteps = nsteps + done
@@ -145,6 +152,15 @@ program grhd2_hip
     ! observers every extraction_stride / diagnostic_stride steps (grhd2.f90:266-280)
     call observers(rc%extract_mdot .and. mod(nsteps, int(rc%extraction_stride, 8)) == 0, &
                    rc%ppi_diagnostics .and. mod(nsteps, int(rc%diagnostic_stride, 8)) == 0)
+    if (rc%mass_monitor_stride > 0 .and. mod(nsteps, int(rc%mass_monitor_stride, 8)) == 0) then
+      call require(grhd_total_mass(ctx, mass), 'grhd_total_mass')
+      write(*,'(A,I0,A,ES14.6,A,ES22.14)') 'step ', nsteps, '  t = ', t, '  total mass: ', mass
+    end if
+    ! then the frame and the checkpoint (grhd2.f90:333-341)
+    if (t >= next_save) then
+      call write_vtk_frame(ctx, cfg, rc, next_save)
+      next_save = next_save + rc%save_interval
+    end if
     if (t >= next_checkpoint) then
       call require(grhd_get_conserved(ctx, u), 'grhd_get_conserved')
       cks%t = t
@@ -154,11 +170,11 @@ program grhd2_hip
       write(*,'(A,A)') '>>> checkpoint ', trim(checkpoint_file_name(cfg, rc, int(nsteps)))
       next_checkpoint = next_checkpoint + rc%checkpoint_interval
     end if
-    if (rc%mass_monitor_stride > 0 .and. mod(nsteps, int(rc%mass_monitor_stride, 8)) == 0) then
-      call require(grhd_total_mass(ctx, mass), 'grhd_total_mass')
-      write(*,'(A,I0,A,ES14.6,A,ES22.14)') 'step ', nsteps, '  t = ', t, '  total mass: ', mass
-    end if
   end do
+  ! a last observer sample at the final time when the strides missed it
+  ! (grhd2.f90:344-354)
+  call observers(rc%extract_mdot .and. mod(nsteps, int(rc%extraction_stride, 8)) /= 0, &
+                 rc%ppi_diagnostics .and. mod(nsteps, int(rc%diagnostic_stride, 8)) /= 0)
   call require(grhd_synchronize(ctx), 'grhd_synchronize')
   call system_clock(c1)
   wall1 = dble(c1 - c0) / dble(crate)

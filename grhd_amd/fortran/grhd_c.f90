@@ -15,7 +15,7 @@ module grhd_c
   public :: grhd_set_primitives, grhd_get_primitives, grhd_set_conserved, grhd_get_conserved
   public :: grhd_compute_dt, grhd_step, grhd_advance, grhd_perturb, grhd_counters
   public :: grhd_total_mass, grhd_convergence_norms, grhd_global_diagnostics, grhd_ppi_modes
-  public :: grhd_accretion_rate
+  public :: grhd_accretion_rate, grhd_lapse_shift
   public :: grhd_last_timing, grhd_synchronize, grhd_error_message
 
   integer, parameter, public :: GRHD_OK = 0
@@ -84,6 +84,11 @@ module grhd_c
       type(c_ptr), value :: ctx
       real(c_double), intent(in) :: p(*)
     end function grhd_set_primitives
+    integer(c_int) function grhd_lapse_shift(cfg, alpha, beta) bind(C, name='grhd_lapse_shift')
+      import :: c_int, c_double, grhd_config
+      type(grhd_config), intent(in) :: cfg
+      real(c_double), intent(out) :: alpha(*), beta(*)
+    end function grhd_lapse_shift
     integer(c_int) function grhd_get_primitives(ctx, p) bind(C, name='grhd_get_primitives')
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: ctx

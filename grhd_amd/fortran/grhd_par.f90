@@ -33,6 +33,8 @@ module grhd_par
     ! steps, azimuthal modes and global diagnostics every diagnostic_stride
     logical :: extract_mdot = .false., ppi_diagnostics = .false.
     integer :: extraction_stride = 10, diagnostic_stride = 100
+    ! VTK frames every save_interval (output.f90:154-471); vtk_mapping
+    logical :: vtk_untwisted = .false.
     logical :: restart = .false.
     character(len=1024) :: restart_file = ' '
   end type run_control
@@ -263,6 +265,7 @@ contains
         ! modes for the equatorial and 3D tori
         rc%extract_mdot = any(pid == [7, 8, 11, 12])
         rc%ppi_diagnostics = any(pid == [10, 12])
+        call preset_output(rc, pid)
       end if
       ln = 0
       nseen = 0
@@ -313,6 +316,32 @@ contains
     end do
     if (c%metric == 2 .and. c%geometry == 2) c%geometry = 3   ! parameters.f90:428-431
   end subroutine read_par
+
+  ! The presets' frame interval, file prefix and folder (initialization.f90:
+  ! 504-820)
+  subroutine preset_output(rc, id)
+    type(run_control), intent(inout) :: rc
+    integer, intent(in) :: id
+    select case(id)
+    case(1); rc%save_interval = 0.1d0; rc%output_prefix = 'Sod'; rc%output_folder = 'ShockTubes'
+    case(2); rc%save_interval = 0.1d0; rc%output_prefix = 'Strong'; rc%output_folder = 'ShockTubes'
+    case(3); rc%save_interval = 0.1d0; rc%output_prefix = 'Shu'; rc%output_folder = 'ShockTubes'
+    case(4); rc%save_interval = 0.1d0; rc%output_prefix = 'KHI'; rc%output_folder = 'KelvinHelmholtz'
+    case(5)
+      rc%save_interval = 1.0d0; rc%output_prefix = 'JET'; rc%output_folder = 'AxisymmetricJet_extreme'
+    case(6)
+      rc%save_interval = 0.1d0; rc%output_prefix = 'Conv'; rc%output_folder = 'ConvergenceTest_N2048'
+    case(7); rc%save_interval = 1.0d0; rc%output_prefix = 'MA'; rc%output_folder = 'Michel_test_2D'
+    case(8); rc%save_interval = 50.0d0; rc%output_prefix = 'D'; rc%output_folder = 'Dust2D'
+    case(9); rc%save_interval = 25.0d0; rc%output_prefix = 'OR'; rc%output_folder = 'OffAxis'
+    case(10)
+      rc%save_interval = 10.0d0; rc%output_prefix = 'FM_KS_a0.9d0'; rc%output_folder = 'FM_KS_a0.9d0_data'
+    case(11)
+      rc%save_interval = 1.0d0; rc%output_prefix = 'FM_Sag'; rc%output_folder = 'FishboneMoncrief_Sagital'
+    case(12)
+      rc%save_interval = 1.0d0; rc%output_prefix = 'FM_3D'; rc%output_folder = 'FishboneMoncrief_3D'
+    end select
+  end subroutine preset_output
 
   ! Keys a restart may change (is_restart_control, parameters.f90:716-731), plus
   ! this build's precision; every other key belongs to the checkpoint.
@@ -409,8 +438,15 @@ contains
       else
         rc%extraction_stride = n
       end if
-    case('vtk_mapping', 'extract_gw')
-      ! the VTK writer and the gravitational-wave strain are not built
+    case('vtk_mapping')
+      ! parse_vtk_mapping (parameters.f90:966-980)
+      select case(trim(lower(adjustl(val))))
+      case('physical', 'ks_cartesian', 'complete', 'completo'); rc%vtk_untwisted = .false.
+      case('untwisted', 'unrolled', 'desenrollado'); rc%vtk_untwisted = .true.
+      case default; call fail(f, ln, 'vtk_mapping must be physical or untwisted')
+      end select
+    case('extract_gw')
+      ! the gravitational-wave strain is not built
     case('metric')
       select case(trim(lower(adjustl(val))))
       case('minkowski'); c%metric = 0

@@ -31,6 +31,16 @@ def fishbone_unit_density_k(cfg: RunConfig) -> float:
     return k.value
 
 
+def lapse_shift(cfg: RunConfig):
+    """alpha (nz, ny, nx) and beta (nz, ny, nx, 3) at the interior cell
+    centres, evaluated on the host (no device needed)."""
+    c = _lib.Config.from_run(cfg)
+    alpha = np.zeros((cfg.nz, cfg.ny, cfg.nx))
+    beta = np.zeros((cfg.nz, cfg.ny, cfg.nx, 3))
+    _lib.check(_lib.lib().grhd_lapse_shift(ctypes.byref(c), _ptr(alpha), _ptr(beta)))
+    return alpha, beta
+
+
 class Solver:
     def __init__(self, cfg: RunConfig, handle=None):
         """A new device context for cfg, or (handle given) a view of a context

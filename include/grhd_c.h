@@ -144,6 +144,13 @@ int grhd_perturb(grhd_ctx *ctx, int32_t type, double amplitude, uint64_t seed);
  * the pressure maximum: K = (h_c - 1)(Gamma - 1)/Gamma with
  * h_c = exp(W_in - W(r_c, pi/2)) (conditions.f90:470-546). */
 int grhd_fishbone_unit_density_k(const grhd_config *cfg, double *K);
+/* Lapse alpha and shift beta^i at the interior cell centres of cfg's grid,
+ * computed on the host (no device, no context): alpha(nx, ny, nz) and
+ * beta(3, nx, ny, nz) in Fortran order, either may be NULL.  Replaces the
+ * reference's alpha_c / beta_c cache (initialization.f90:283-299) that
+ * write_vtk_cell_data (output.f90:402-455) writes as Lapse_alpha and
+ * Shift_beta_r. */
+int grhd_lapse_shift(const grhd_config *cfg, double *alpha, double *beta);
 /* Uploads a full ghosted primitive array and rebuilds U = P2C(p) on the
  * device (grhd2.f90:206-218 after a perturbation). */
 int grhd_set_primitives(grhd_ctx *ctx, const double *p);

@@ -100,6 +100,8 @@ def ref_lib():
         lib.ref_restart.argtypes = [ctypes.c_char_p, ctypes.c_int]
         lib.ref_save_checkpoint.argtypes = [ctypes.c_char_p, ctypes.c_int]
         lib.ref_observers.argtypes = [ctypes.c_char_p, ctypes.c_int]
+        lib.ref_save_vtk.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_double]
+        lib.ref_lapse_shift.argtypes = [_dp, _dp]
         lib.ref_compute_dt.restype = ctypes.c_double
         lib.ref_dims.argtypes = [ctypes.POINTER(ctypes.c_int)]
         for name in ("ref_scalars", "ref_get_prim", "ref_set_prim", "ref_put_prim",
@@ -379,6 +381,27 @@ class Reference:
             out[name] = (np.array([float(x) for x in open(path).read().split("\n")[-2].split()])
                          if os.path.exists(path) else None)
         return out
+
+    def save_vtk(self, folder, scheme_name, t):
+        """The reference's save_vtk_at_time(t) of the current state under
+        folder/scheme_name/; returns the new file's path."""
+        import glob
+        sub = os.path.join(folder, scheme_name)
+        os.makedirs(sub, exist_ok=True)
+        b = str(folder).encode()
+        before = set(glob.glob(os.path.join(sub, "*.vtk")))
+        self.lib.ref_save_vtk(b, len(b), float(t))
+        new = sorted(set(glob.glob(os.path.join(sub, "*.vtk"))) - before)
+        if len(new) != 1:
+            raise RuntimeError("reference VTK frame not written")
+        return new[0]
+
+    def lapse_shift(self):
+        """alpha_c as (nz, ny, nx) and beta_c as (nz, ny, nx, 3)."""
+        a = np.zeros((self.nz, self.ny, self.nx))
+        b = np.zeros((self.nz, self.ny, self.nx, 3))
+        self.lib.ref_lapse_shift(_ptr(a), _ptr(b))
+        return a, b
 
     def close(self):
         self.lib.ref_release()

@@ -21,7 +21,7 @@ module grhd_ref_harness
   use parameters
   use metrics
   use conditions
-  use output, only: save_checkpoint, read_checkpoint_metadata, read_checkpoint_state, &
+  use output, only: save_vtk_at_time, save_checkpoint, read_checkpoint_metadata, read_checkpoint_state, &
                     calc_global_diagnostics, calc_ppi_modes, calc_m_dot
   use equations
   use fluxes
@@ -211,6 +211,26 @@ contains
     call calc_m_dot(integration_time)
     do_ppi_diagnostics = .false.
   end subroutine ref_observers
+
+  ! save_vtk_at_time (output.f90:458-471) of the current state: the frame of
+  ! time t under <dir>/<scheme_name>/
+  subroutine ref_save_vtk(dir, n, t) bind(C, name='ref_save_vtk')
+    character(kind=c_char), intent(in) :: dir(*)
+    integer(c_int), value :: n
+    real(c_double), value :: t
+    character(len=256) :: d
+    call c_to_f_string(dir, n, d)
+    output_folder = d
+    call save_vtk_at_time(t)
+  end subroutine ref_save_vtk
+
+  ! the centre metric cache: alpha_c(nx, ny, nz), beta_c(3, nx, ny, nz)
+  ! (initialization.f90:229-230, 283-299)
+  subroutine ref_lapse_shift(alpha, beta) bind(C, name='ref_lapse_shift')
+    real(c_double), intent(out) :: alpha(nx, ny, nz), beta(3, nx, ny, nz)
+    alpha = alpha_c(1:nx, 1:ny, 1:nz)
+    beta = beta_c(:, 1:nx, 1:ny, 1:nz)
+  end subroutine ref_lapse_shift
 
   subroutine ref_release() bind(C, name='ref_release')
     call release_problem()

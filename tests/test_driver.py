@@ -114,10 +114,11 @@ def test_driver_perturbation_and_monitor_match_python(tmp_path):
 @pytest.mark.gpu
 def test_driver_observers_match_reference(tmp_path):
     """ppi_diagnostics and extract_mdot in the driver: the records it appends
-    every diagnostic_stride / extraction_stride steps (grhd2.f90:139-145,
-    266-280) follow the reference's formats, and the first ones (the initial
-    state) equal the reference's own calc_global_diagnostics, calc_ppi_modes
-    and calc_m_dot of its initial state."""
+    every diagnostic_stride / extraction_stride steps and at the final time
+    (grhd2.f90:139-145, 266-280, 344-354) follow the reference's formats, and
+    the first ones (the initial state) equal the reference's own
+    calc_global_diagnostics, calc_ppi_modes and calc_m_dot of its initial
+    state."""
     import oracle
     from grhd_amd.params import config_from_par
     from tests.cases import CASES
@@ -136,7 +137,10 @@ def test_driver_observers_match_reference(tmp_path):
     glob = [np.array(l.split(), float) for l in (d / "global_diagnostics.dat").read_text().splitlines()]
     modes = [np.array(l.split(), float) for l in (d / "ppi_modes.dat").read_text().splitlines()]
     mdot = [np.array(l.split(), float) for l in (d / "m_dot.dat").read_text().splitlines()]
-    assert len(glob) == len(modes) == 1 + steps // 2 and len(mdot) == 1 + steps // 2
+    # the initial sample, one every 2 steps, and one at the final time when the
+    # stride missed it (grhd2.f90:344-354)
+    n_expected = 1 + steps // 2 + (steps % 2 != 0)
+    assert len(glob) == len(modes) == n_expected and len(mdot) == n_expected
     assert all(len(g) == 11 for g in glob) and all(len(m) == 18 for m in modes)
     R = oracle.Reference(config_from_par(text))
     try:
