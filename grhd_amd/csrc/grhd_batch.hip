@@ -31,6 +31,11 @@ namespace {
 constexpr int kMaxGroups = 16;  // metric kinds x face arithmetics
 constexpr int kMaxChunk = 64;   // steps per captured graph (of the most expensive group)
 constexpr int kMaxPace = 8;     // steps of a cheaper group per step of the most expensive one
+// A group is paced only when its estimated step is at least this many times
+// cheaper than the most expensive group's: near the launch-latency floor the
+// estimate cannot separate steps that differ by less, and a cheap group given
+// too many steps becomes the chunk's critical path.
+constexpr int kMinPace = 3;
 constexpr int kTuneSteps = 2;   // steps timed per candidate
 // Problems of at most this many cells run whole chunks of steps in one block
 // each (kb_small); larger ones use the launch form the autotuner picks.  One
@@ -174,7 +179,8 @@ void pace_groups(grhd_batch *b, int v)
     for (size_t g = 0; g < b->groups.size(); g++) {
         BGroup &G = b->groups[g];
         G.ratio = 1;
-        if (forked && b->pace > 1) G.ratio = std::max(1, std::min(b->pace, (int)(most / cost[g])));
+        const int r = (int)(most / cost[g]);
+        if (forked && b->pace > 1 && r >= kMinPace) G.ratio = std::min(b->pace, r);
         for (int sl : G.slots) b->slot_ratio[sl] = G.ratio;
     }
 }

@@ -274,9 +274,10 @@ int grhd_batch_status(grhd_batch *b, int32_t slot, int32_t *done, double *t, int
 int grhd_batch_set_variant(grhd_batch *b, int32_t variant);
 /* Pacing (no reference counterpart; default 1).  With max_ratio > 1 and the
  * groups of different metric kinds or arithmetic on their own streams (the
- * phase launch forms), a group whose step is cheaper than the most expensive
- * group's takes up to max_ratio (<= 8) steps per step of that group in each
- * chunk, so small problems are not held to the step rate of the largest.
+ * phase launch forms), a group whose estimated step is at least 3 times
+ * cheaper than the most expensive group's takes up to max_ratio (<= 8) steps
+ * per step of that group in each chunk, so small problems are not held to the
+ * step rate of the largest.
  * grhd_batch_step(nsteps) then advances the most expensive group nsteps steps
  * and cheaper ones up to max_ratio x nsteps; every problem still stops at its
  * final time, grhd_batch_status reports its steps, and its state equals its
