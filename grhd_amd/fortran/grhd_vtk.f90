@@ -27,7 +27,8 @@ module grhd_vtk
   character(len=10), parameter :: VAR_NAMES(5) = &
     [character(len=10) :: 'Density', 'Pressure', 'Velocity_X', 'Velocity_Y', 'Velocity_Z']
 
-  ! the grid of the configuration last written (faces and centres)
+  ! the grid of the configuration last written (faces, centres, lapse and
+  ! shift), kept between frames: the driver runs one configuration per process
   type :: vtk_grid
     integer :: nx = 0, ny = 0, nz = 0
     real(c_double), allocatable :: xf(:), yf(:), zf(:), xc(:), yc(:), zc(:)

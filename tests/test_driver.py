@@ -45,6 +45,7 @@ def test_check_valid(tmp_path):
     ("metric = kerr_schild\ngeometry = cartesian\n", "metric"),
     ("spin = 1.5\nmetric = kerr_schild\ngeometry = spheroidal\nr_min = 2\nr_max = 10\n", "spin"),
     ("unknown_key = 3\n", "unknown_key"),
+    ("vtk_mapping = fancy\n", "vtk_mapping"),
 ])
 def test_check_rejects(tmp_path, bad, message):
     par = tmp_path / "bad.par"
