@@ -128,6 +128,30 @@ def test_paced_batch_matches_alone(variant):
             assert np.array_equal(res[n][0], S.get_conserved()), n
 
 
+def test_pacing_needs_a_threefold_gap():
+    """Beside a 400^2 Kelvin-Helmholtz problem (about 134 us by the estimate,
+    2.5 times the small groups' 54 us) no group is paced: near the latency
+    floor the estimate cannot separate such steps.  Every problem takes the
+    chunk's steps and matches its run alone."""
+    mid = CASES["khi_weno3_sensor"].replace("nx = 32", "nx = 400").replace("nz = 32", "nz = 400")
+    texts = {"khi_400": mid, "michel_ef_log": CASES["michel_ef_log"], "sod_c1": CASES["sod_c1"]}
+    print("pacing threshold", flush=True)
+    res = {}
+    with Batch(capacity=4) as B:
+        B.set_variant(1)
+        B.set_pacing(8)
+        slots = {n: B.add(config_from_par(t)) for n, t in texts.items()}
+        B.step(NSTEPS)
+        for n in texts:
+            res[n] = (B.problem(slots[n]).get_conserved(), B.status(slots[n])[2])
+    assert {n: res[n][1] for n in texts} == {n: NSTEPS for n in texts}
+    for n, text in texts.items():
+        with Solver(config_from_par(text)) as S:
+            S.initial_conditions()
+            S.advance(max_steps=NSTEPS)
+            assert np.array_equal(res[n][0], S.get_conserved()), n
+
+
 def test_problems_stop_at_their_final_time():
     """A short problem finishes inside a chunk while a long one continues;
     the short one matches its own full run, final-step clamp included."""
