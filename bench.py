@@ -282,14 +282,18 @@ def main():
                     help="Poisson arrival rates (requests/s) of the served ensemble, swept in order; "
                          "the line's 'ensemble' is the one at 16/s (or the first)")
     ap.add_argument("--serve-mix", choices=sorted(MIXES), default="quick")
-    ap.add_argument("--serve-pacing", type=int, default=8,
-                    help="steps of a cheaper batch group per step of the most expensive one (1..8)")
+    ap.add_argument("--serve-pacing", type=int, default=None,
+                    help="steps of a cheaper batch group per step of the most expensive one (1..8); "
+                         "default 8 for the full mix, whose problem sizes span 200 to 1M cells, "
+                         "and 1 for the quick mix, where pacing raised the median time-to-solution")
     ap.add_argument("--serve-progress", action="store_true",
                     help="print each served request to stderr as it completes")
     ap.add_argument("--serve-only", action="store_true",
                     help="only the served ensemble (no torus throughput, no CPU baseline)")
     ap.add_argument("--serve-time-scale", type=float, default=1.0)
     ap.add_argument("--serve-chunk", type=int, default=32)
+    ap.add_argument("--serve-seed", type=int, default=3435,
+                    help="seed of the request stream (arrivals and request draws)")
     ap.add_argument("--michel", action="store_true",
                     help="only the Michel C2 convergence study (nr 256/1024/4096, FP64 and mixed)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -309,11 +313,14 @@ def main():
         print(f"bench.py: this build measures one GPU only (asked for --gpus {a.gpus}); "
               "multi-GPU sharding is designed (radial slabs) but not built", file=sys.stderr)
         sys.exit(2)
+    if a.serve_pacing is None:
+        a.serve_pacing = 8 if a.serve_mix == "full" else 1
     if a.serve_only:
         for r in (float(r) for r in a.serve_rates.split(",") if r):
             print(json.dumps(run_ensemble(a.serve_requests, r, a.serve_time_scale, a.serve_chunk,
-                                          mix=a.serve_mix, progress=a.serve_progress,
-                                          pacing=a.serve_pacing)), flush=True)
+                                          seed=a.serve_seed, mix=a.serve_mix,
+                                          progress=a.serve_progress, pacing=a.serve_pacing)),
+                  flush=True)
         return
     nx, ny, nz = (int(v) for v in a.grid.split(","))
     cfg = c4_config(nx, ny, nz)
@@ -366,8 +373,8 @@ def main():
         }
     if a.serve_requests > 0:
         rates = [float(r) for r in a.serve_rates.split(",") if r]
-        sweep = [run_ensemble(a.serve_requests, r, a.serve_time_scale, a.serve_chunk, mix=a.serve_mix,
-                              progress=a.serve_progress, pacing=a.serve_pacing)
+        sweep = [run_ensemble(a.serve_requests, r, a.serve_time_scale, a.serve_chunk, seed=a.serve_seed,
+                              mix=a.serve_mix, progress=a.serve_progress, pacing=a.serve_pacing)
                  for r in rates]
         line["ensemble"] = next((e for e in sweep if e["arrival_rate_per_s"] == 16.0), sweep[0])
         line["ensemble_sweep"] = [{k: e[k] for k in ("arrival_rate_per_s", "p50_s", "p99_s", "mean_wait_s",
