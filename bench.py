@@ -252,6 +252,56 @@ def michel_convergence(resolutions=(256, 1024, 4096)):
     return out
 
 
+C3_PAR = """problem = fishbone_equatorial
+fm_inner_radius = 6
+fm_pressure_max_radius = 12
+reconstruction = weno5
+riemann_solver = hlle
+"""
+
+
+def equatorial_torus(steps=1000):
+    """C3: Fishbone-Moncrief equatorial torus, Kerr-Schild a = 0.9, log-r
+    spheroidal grid 400 x 1 x 200 (the reference's preset grid), r_in = 6,
+    pressure maximum at r = 12, K for rho_max = 1, 1e-2 seeded pressure noise
+    inside the torus, WENO5 + HLLE, `steps` RK3 steps in FP64 and in mixed
+    precision: throughput, and the mixed run's conserved density D against the
+    FP64 run's (relative L1).  The floors are the reference's constants (rho 1e-10,
+    p 1e-13); r-dependent floors are not a feature of the reference."""
+    import numpy as np
+    from grhd_amd.params import config_from_par
+    from grhd_amd.solver import Solver, fishbone_unit_density_k
+    out, rho = {}, {}
+    for prec in ("fp64", "mixed"):
+        cfg = config_from_par(C3_PAR + f"precision = {prec}\n")
+        cfg.fm_polytropic_constant = fishbone_unit_density_k(cfg)
+        with Solver(cfg) as W:          # loads the kernels outside the timed run
+            W.initial_conditions()
+            W.step(W.compute_dt(), 3)
+            W.synchronize()
+        with Solver(cfg) as S:
+            S.initial_conditions()
+            S.perturb(1, 1.0e-2, 3435)
+            dt = S.compute_dt()
+            S.synchronize()
+            t0 = time.perf_counter()
+            S.step(dt, steps)
+            S.synchronize()
+            wall = time.perf_counter() - t0
+            ev_ms, _ = S.last_timing()
+            rho[prec] = S.get_conserved()[..., 0]
+            out[prec] = {"cell_updates_per_s": S.n_cells * steps / wall, "ms_per_step": 1e3 * wall / steps,
+                         "event_ms_per_step": ev_ms / steps, "steps": steps, "t": steps * dt,
+                         **S.counters()}
+    a, b = rho["mixed"], rho["fp64"]
+    rec = {"workload": "fishbone_equatorial Kerr-Schild a=0.9 log-r 400x1x200, r_in=6, r_pmax=12, "
+                       "K for rho_max=1, 1e-2 pressure noise, WENO5+HLLE",
+           **out, "mixed_speedup": out["mixed"]["cell_updates_per_s"] / out["fp64"]["cell_updates_per_s"],
+           "mixed_vs_fp64_D_L1": float(np.abs(a - b).sum() / np.abs(b).sum())}
+    print(json.dumps(rec), flush=True)
+    return rec
+
+
 def copy_calibration(nbytes=1 << 30, reps=5):
     """A plain device-to-device copy of nbytes, reps times: its FETCH_SIZE and
     WRITE_SIZE per dispatch give the counters' scale against known traffic
@@ -294,6 +344,8 @@ def main():
     ap.add_argument("--serve-chunk", type=int, default=32)
     ap.add_argument("--serve-seed", type=int, default=3435,
                     help="seed of the request stream (arrivals and request draws)")
+    ap.add_argument("--c3", action="store_true",
+                    help="only the equatorial torus: 1000 RK3 steps in FP64 and mixed precision")
     ap.add_argument("--michel", action="store_true",
                     help="only the Michel C2 convergence study (nr 256/1024/4096, FP64 and mixed)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -304,6 +356,9 @@ def main():
         return
     if a.michel:
         michel_convergence()
+        return
+    if a.c3:
+        equatorial_torus()
         return
     cnx, cny, cnz = (int(v) for v in a.cpu_grid.split(","))
     if a.cpu_child:
