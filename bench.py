@@ -269,6 +269,7 @@ def equatorial_torus(steps=1000):
     FP64 run's (relative L1).  The floors are the reference's constants (rho 1e-10,
     p 1e-13); r-dependent floors are not a feature of the reference."""
     import numpy as np
+    from grhd_amd.batch import Batch
     from grhd_amd.params import config_from_par
     from grhd_amd.solver import Solver, fishbone_unit_density_k
     out, rho = {}, {}
@@ -289,10 +290,33 @@ def equatorial_torus(steps=1000):
             S.synchronize()
             wall = time.perf_counter() - t0
             ev_ms, _ = S.last_timing()
-            rho[prec] = S.get_conserved()[..., 0]
+            u_direct = S.get_conserved()
+            rho[prec] = u_direct[..., 0]
             out[prec] = {"cell_updates_per_s": S.n_cells * steps / wall, "ms_per_step": 1e3 * wall / steps,
                          "event_ms_per_step": ev_ms / steps, "steps": steps, "t": steps * dt,
                          **S.counters()}
+            ncell = S.n_cells
+        # the same run as a batch of one: graph-replayed chunks of 64 steps in
+        # each launch form (capture included in the time); the state must equal
+        # the direct run's bit for bit
+        forms = {}
+        for v in range(1, 6):
+            with Batch(capacity=1) as B:
+                B.set_variant(v)
+                slot = B.add(cfg)
+                view = B.problem(slot)
+                view.perturb(1, 1.0e-2, 3435)
+                view.synchronize()
+                t0 = time.perf_counter()
+                B.step(steps)
+                taken = B.status(slot)[2]
+                wall_b = time.perf_counter() - t0
+                assert taken == steps and np.array_equal(view.get_conserved(), u_direct), (prec, v)
+            forms[v] = ncell * steps / wall_b
+        best = max(forms, key=forms.get)
+        out[prec]["batched_cell_updates_per_s"] = forms[best]
+        out[prec]["batched_launch_form"] = best
+        out[prec]["batched_by_form"] = {str(k): round(x / 1e6, 1) for k, x in forms.items()}
     a, b = rho["mixed"], rho["fp64"]
     rec = {"workload": "fishbone_equatorial Kerr-Schild a=0.9 log-r 400x1x200, r_in=6, r_pmax=12, "
                        "K for rho_max=1, 1e-2 pressure noise, WENO5+HLLE",
