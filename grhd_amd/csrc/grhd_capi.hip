@@ -305,7 +305,7 @@ int run_steps(grhd_ctx *c, const double *dts, int n)
     int64_t launches = 0;
     launch_boundaries(P, c->stream, c->p, c->injector);  // grhd2.f90:233
     launches++;
-    const int per_stage = ((int64_t)P.nx * P.ny * P.nz <= kFusedStageCells)
+    const int per_stage = single_stage_fused(P)
                               ? 3  // faces, cells, ghosts
                               : 1 + (P.ny > 1) + (P.nz > 1) + (c->mixed ? 2 : 1) + 1;
     for (int s = 0; s < n; s++) {

@@ -292,8 +292,8 @@ void launch_stage(const Params &P, hipStream_t s, double *p, void *const F[3], c
                   const double *u0, double *u, int stage, double dt, unsigned long long *counters,
                   bool mixed, const double *injector, bool with_boundaries)
 {
-    // small grids are launch-bound: fused stage form (3 launches)
-    const bool fused = (int64_t)P.nx * P.ny * P.nz <= kFusedStageCells;
+    // small and 2D grids: fused stage form (3 launches)
+    const bool fused = single_stage_fused(P);
     if (mixed) launch_stage_f32(P, s, p, F, fstr, u0, u, stage, dt, counters, fused);
     else launch_stage_f64(P, s, p, F, fstr, u0, u, stage, dt, counters, fused);
     if (with_boundaries) launch_boundaries(P, s, p, injector);

@@ -17,8 +17,20 @@ void launch_stage_f32(const Params &P, hipStream_t s, double *p, void *const F[3
 
 // Grids of at most this many cells use the fused stage form (one face launch,
 // one cell launch); larger ones one launch per face direction and separate
-// update and con2prim kernels.
+// update and con2prim kernels.  The batch also splits its groups there.
 constexpr int64_t kFusedStageCells = 1 << 16;
+// A single problem with an inactive y or z axis keeps the fused form up to
+// this many cells: measured faster on 2D equatorial tori of 80k, 320k and
+// 1.28M cells (+26, +19 and +15 % in FP64), while 3D tori above 2^16 cells
+// run faster with one launch per direction (the fused form lost 1 % at 524k
+// cells, 8 % at 1.77M and 12 % at 4.2M).
+constexpr int64_t kFusedStage2DCells = 1 << 21;
+
+inline bool single_stage_fused(const Params &P)
+{
+    const int64_t n = (int64_t)P.nx * P.ny * P.nz;
+    return n <= kFusedStageCells || ((P.ny == 1 || P.nz == 1) && n <= kFusedStage2DCells);
+}
 
 // One RK stage: face fluxes, cell update (stage 1..3; 0 = rhs only into u),
 // then the ghost fill when with_boundaries.

@@ -401,9 +401,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
 }
 
 // fused: one face launch for every direction (for small grids, where launch
-// latency dominates); otherwise one face launch per direction -- measured
-// faster on large grids, where the three face code paths share a CU's
-// instruction cache.  Either way the flux divergence, RK combination and
+// latency dominates, and for grids with an inactive axis, which have two face
+// code paths); otherwise one face launch per direction -- measured faster on
+// large 3D grids, where the three face code paths share a CU's instruction
+// cache.  Either way the flux divergence, RK combination and
 // con2prim of a cell run in one thread of one cell kernel.
 template <int MK, typename R>
 void launch_stage_mk(const Params &P, hipStream_t s, double *p, void *const F[3], const int64_t fstr[3],
