@@ -389,8 +389,8 @@ def main():
         cpu_child(a.cpu_child, cnx, cny, cnz, a.cpu_seconds)
         return
     if a.gpus != 1:
-        print(f"bench.py: this build measures one GPU only (asked for --gpus {a.gpus}); "
-              "multi-GPU sharding is designed (radial slabs) but not built", file=sys.stderr)
+        print(f"bench.py: measures one GPU only (asked for --gpus {a.gpus}); "
+              "sharding by radial slabs across GPUs is not implemented", file=sys.stderr)
         sys.exit(2)
     if a.serve_pacing is None:
         a.serve_pacing = 8 if a.serve_mix == "full" else 1

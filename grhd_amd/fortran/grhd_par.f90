@@ -344,7 +344,8 @@ contains
   end subroutine preset_output
 
   ! Keys a restart may change (is_restart_control, parameters.f90:716-731), plus
-  ! this build's precision; every other key belongs to the checkpoint.
+  ! precision (the FP64 or mixed face arithmetic, which the reference does not
+  ! have); every other key belongs to the checkpoint.
   logical function is_restart_control(key)
     character(len=*), intent(in) :: key
     select case(trim(key))
