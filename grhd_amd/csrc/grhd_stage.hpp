@@ -363,7 +363,7 @@ struct Flux3 {
     int64_t nb[3];
     int kt[3], gw[3];
 };
-template <int MK, typename R>
+template <int MK, typename R, int SCH>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_flux3(
     Params P, const double *__restrict__ p, R *__restrict__ Fx, R *__restrict__ Fy, R *__restrict__ Fz,
     int64_t fsx, int64_t fsy, int64_t fsz, Flux3 f)
@@ -372,15 +372,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
     R *lds = reinterpret_cast<R *>(lds_geo);
     int64_t L = xcd_block(blockIdx.x, gridDim.x);
     if (L < f.nb[0]) {
-        flux_body<MK, 1, R>(P, p, Fx, fsx, L, f.kt[0], f.gw[0], lds);
+        flux_body<MK, 1, R, SCH>(P, p, Fx, fsx, L, f.kt[0], f.gw[0], lds);
         return;
     }
     L -= f.nb[0];
     if (L < f.nb[1]) {
-        flux_body<MK, 2, R>(P, p, Fy, fsy, L, f.kt[1], f.gw[1], lds);
+        flux_body<MK, 2, R, SCH>(P, p, Fy, fsy, L, f.kt[1], f.gw[1], lds);
         return;
     }
-    flux_body<MK, 3, R>(P, p, Fz, fsz, L - f.nb[1], f.kt[2], f.gw[2], lds);
+    flux_body<MK, 3, R, SCH>(P, p, Fz, fsz, L - f.nb[1], f.kt[2], f.gw[2], lds);
 }
 
 // Cell update and con2prim of the same cell in one thread (stages 1..3).
@@ -424,12 +424,21 @@ void launch_stage_mk(const Params &P, hipStream_t s, double *p, void *const F[3]
         f.nb[d] = !on ? 0 : (d == 0) ? flux_blocks<1>(P, t) : (d == 1) ? flux_blocks<2>(P, t) : flux_blocks<3>(P, t);
         if (on) groups = std::max(groups, 4 / t.gw);
     }
+    const int sch = face_scheme<MK>(P);
     if (fused) {
         const size_t lds = (size_t)groups * NFG * 64 * sizeof(R);
-        k_flux3<MK, R><<<(unsigned)(f.nb[0] + f.nb[1] + f.nb[2]), bs, lds, s>>>(P, p, Fx, Fy, Fz, fstr[0],
-                                                                                fstr[1], fstr[2], f);
+        const unsigned nb = (unsigned)(f.nb[0] + f.nb[1] + f.nb[2]);
+        // compiled schemes for FP32 faces only: with FP64 faces the fused
+        // kernel spills far more with them (260 B per lane for WENO5 + HLLE
+        // against 56 for the run-time scheme) and measured 4 % slower
+        if constexpr ((MK == MK_KS || MK == MK_KS_LOG) && sizeof(R) == sizeof(float)) {
+            if (sch == 1) k_flux3<MK, R, 1><<<nb, bs, lds, s>>>(P, p, Fx, Fy, Fz, fstr[0], fstr[1], fstr[2], f);
+            else if (sch == 2) k_flux3<MK, R, 2><<<nb, bs, lds, s>>>(P, p, Fx, Fy, Fz, fstr[0], fstr[1], fstr[2], f);
+            else k_flux3<MK, R, 0><<<nb, bs, lds, s>>>(P, p, Fx, Fy, Fz, fstr[0], fstr[1], fstr[2], f);
+        } else {
+            k_flux3<MK, R, 0><<<nb, bs, lds, s>>>(P, p, Fx, Fy, Fz, fstr[0], fstr[1], fstr[2], f);
+        }
     } else {
-        const int sch = face_scheme<MK>(P);
         if constexpr (MK == MK_KS || MK == MK_KS_LOG) {
             if (sch == 1) launch_faces<MK, R, 1>(P, s, p, Fx, Fy, Fz, fstr);
             else if (sch == 2) launch_faces<MK, R, 2>(P, s, p, Fx, Fy, Fz, fstr);
