@@ -12,6 +12,7 @@ from pathlib import Path
 LIB_PATH = Path(__file__).resolve().parent / "libgrhd_hip.so"
 
 GRHD_OK = 0
+GRHD_ERR_INVALID_METRIC = 2
 ERRORS = {
     1: "invalid configuration",
     2: "invalid metric / CFL",

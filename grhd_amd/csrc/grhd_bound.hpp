@@ -129,6 +129,17 @@ __device__ __forceinline__ void bound_body(const Params &P, double *__restrict__
         } else if (prob == 11) {  // reflective theta (conditions.f90:977-987)
             load(gi, hi ? ny + 1 - g : g, gk);
             q[EQ_VY] = -q[EQ_VY];
+        } else if (prob == 12 && (hi ? P.open_hi : P.open_lo)) {
+            // theta boundary away from the pole (no reference counterpart: the
+            // reference requires theta = [0, pi] here, parameters.f90:577-583):
+            // outflow copy of the edge cell, rho and p floored, v^theta
+            // limited to outflow.  beta^theta = 0 for every metric of this
+            // geometry, so the shift-aware limit of the radial rule
+            // (conditions.f90:1007-1026) is v^theta = 0.
+            load(gi, hi ? ny : 1, gk);
+            q[EQ_DE] = rmax(q[EQ_DE], P.rho_floor);
+            q[EQ_PR] = rmax(q[EQ_PR], P.p_floor);
+            q[EQ_VY] = hi ? rmax(0.0, q[EQ_VY]) : rmin(0.0, q[EQ_VY]);
         } else if (prob == 12) {  // trans-polar with phi + pi (conditions.f90:1032-1050)
             int ks = gk + nz / 2;
             if (ks > nz) ks -= nz;

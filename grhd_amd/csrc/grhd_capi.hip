@@ -106,10 +106,10 @@ int validate(const grhd_config &c, std::string &msg)
         const double tol = 128.0 * eps * kPi;
         if (c.ny <= 1 || c.nz <= 1 || c.nz % 2 != 0)
             return msg = "fishbone_3d needs ny, nz > 1 and even nz", 1;
-        if (!c.allow_open_theta &&
-            (std::fabs(c.y_min) > tol || std::fabs(c.y_max - kPi) > tol ||
-             std::fabs((c.z_max - c.z_min) - 2.0 * kPi) > tol))
-            return msg = "fishbone_3d needs theta = [0, pi] and phi length 2 pi", 1;
+        if (std::fabs((c.z_max - c.z_min) - 2.0 * kPi) > tol)
+            return msg = "fishbone_3d needs phi length 2 pi", 1;
+        if (!c.allow_open_theta && (std::fabs(c.y_min) > tol || std::fabs(c.y_max - kPi) > tol))
+            return msg = "fishbone_3d needs theta = [0, pi]", 1;
     }
     return 0;
 }
@@ -142,6 +142,11 @@ Params make_params(const grhd_config &c)
     P.sensor = c.shock_sensor ? 1 : 0;
     P.polar = (c.geometry == GRHD_GEOM_SPHERICAL || c.geometry == GRHD_GEOM_SPHEROIDAL);
     P.cyl = (c.geometry == GRHD_GEOM_CYLINDRICAL);
+    if (c.problem == 12) {
+        const double tol = 128.0 * DBL_EPSILON * kPi;
+        P.open_lo = c.y_min > tol;
+        P.open_hi = c.y_max < kPi - tol;
+    }
     P.gam = c.adiabatic_index;
     P.g1 = c.adiabatic_index / (c.adiabatic_index - 1.0);
     P.mass = c.bh_mass;

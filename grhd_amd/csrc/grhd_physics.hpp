@@ -43,6 +43,9 @@ struct Params {
     int rec, rs, lim, sensor;
     int polar;  // spherical / spheroidal geometry
     int cyl;    // cylindrical geometry
+    // fishbone_3d with a theta boundary away from the pole (low / high side):
+    // outflow ghost fill there instead of the trans-polar one
+    int open_lo, open_hi;
     double gam, g1, mass, spin;
     double rho_floor, p_floor;
     double x_min, dx, y_min, dy, z_min, dz;

@@ -533,9 +533,13 @@ def _validate_initial(cfg: RunConfig) -> None:
         if not bad and p == 11:
             if cfg.ny <= 1 or cfg.nz != 1 or abs(cfg.y_min) > tol or abs(cfg.y_max - PI) > tol:
                 raise ParameterError("fishbone_sagittal needs ny>1, nz=1 and theta=[0,1]")
-        if not bad and p == 12 and not cfg.extra.get("_allow_open_theta"):
-            if (cfg.ny <= 1 or cfg.nz <= 1 or cfg.nz % 2 != 0 or abs(cfg.y_min) > tol or
-                    abs(cfg.y_max - PI) > tol or abs((cfg.z_max - cfg.z_min) - 2.0 * PI) > tol):
+        if not bad and p == 12:
+            # with _allow_open_theta (a build option) theta may stop short of
+            # the poles; such a boundary gets an outflow ghost fill
+            open_ok = cfg.extra.get("_allow_open_theta")
+            if (cfg.ny <= 1 or cfg.nz <= 1 or cfg.nz % 2 != 0 or
+                    (not open_ok and (abs(cfg.y_min) > tol or abs(cfg.y_max - PI) > tol)) or
+                    abs((cfg.z_max - cfg.z_min) - 2.0 * PI) > tol):
                 raise ParameterError(
                     "fishbone_3d needs ny,nz>1, even nz, theta=[0,1] and phi length 2 [pi]")
     else:

@@ -106,7 +106,10 @@ typedef struct grhd_config {
     double offaxis_density_amplitude, offaxis_pressure_amplitude;
     double fm_inner_radius, fm_pressure_max_radius, fm_polytropic_constant;
     /* 1 accepts fishbone_3d with theta away from the poles (a build option,
-     * the reference requires theta = [0, pi]); the trans-polar fill is kept. */
+     * the reference requires theta = [0, pi], parameters.f90:577-583).  A
+     * theta boundary that is not a pole gets an outflow ghost fill (copy of
+     * the edge cell, rho and p floored, v^theta limited to outflow); a pole
+     * keeps the trans-polar fill (conditions.f90:1032-1050). */
     int32_t allow_open_theta;
     int32_t reserved[7];
 } grhd_config;

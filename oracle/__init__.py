@@ -80,8 +80,17 @@ def oracle_lib():
         for name in ("orc_cons_to_prim_cell", "orc_prim_to_cons_cell", "orc_sources_cell"):
             getattr(lib, name).argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_int, ctypes.c_int,
                                            ctypes.c_int]
+        lib.orc_brent_stats.argtypes = [ctypes.POINTER(ctypes.c_ulonglong), ctypes.c_int]
         _oracle_lib = lib
     return _oracle_lib
+
+
+def brent_stats(reset: bool = True) -> tuple[int, int]:
+    """(calls, loop iterations) of the oracle's Brent fallback of the
+    con2prim root solve since the last reset (equations.f90:337-339, 343-424)."""
+    out = (ctypes.c_ulonglong * 2)()
+    oracle_lib().orc_brent_stats(out, 1 if reset else 0)
+    return int(out[0]), int(out[1])
 
 
 def reference_available() -> bool:
@@ -316,6 +325,14 @@ class Reference:
         p = np.zeros(self.pshape)
         self.lib.ref_get_prim(_ptr(p))
         return p
+
+    def cons_to_prim_cell(self, cons, i, j, k):
+        """The reference's cons_to_prim on one conserved state with the
+        centre metric of interior cell (i, j, k) (1-based): (repaired cons, prim)."""
+        c = np.array(cons, float)
+        q = np.zeros(5)
+        self.lib.ref_cons_to_prim_cell(_ptr(c), _ptr(q), i, j, k)
+        return c, q
 
     def set_prim(self, p):
         """Load primitives and rebuild the conserved state from them."""

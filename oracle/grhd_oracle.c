@@ -1063,10 +1063,22 @@ static double w_upper(const orc_problem *P, const double *c, const double gi[3][
     const double wg = (sqrt(s) + c[EQ_DE]) / dmax(c[EQ_DE], P->rho_floor);
     return dmax(lo, wg * 2.0);
 }
+/* Calls of root_brent and iterations of its loop since the last
+ * orc_brent_stats(reset = 1): instrumentation for the tests that pin the
+ * fallback, with no effect on any result. */
+static unsigned long long brent_calls, brent_iters;
+void orc_brent_stats(unsigned long long *out, int reset)
+{
+    out[0] = brent_calls;
+    out[1] = brent_iters;
+    if (reset) brent_calls = brent_iters = 0;
+}
+
 /* root_brent (equations.f90:343-424). */
 static double root_brent(const orc_problem *P, const double *c, const double gi[3][3])
 {
     const double x1 = 1.0, xacc = 1.0e-8, EPS = 3.0e-8;
+    brent_calls++;
     const double x2 = w_upper(P, c, gi, 1.0e4);
     double a, b, cc, d = 0.0, e = 0.0, fa, fb, fc, df;
     eval_res(P, x1, c, gi, &fa, &df);
@@ -1074,6 +1086,7 @@ static double root_brent(const orc_problem *P, const double *c, const double gi[
     if (sgn1(fa) * sgn1(fb) > 0.0) return -1.0;
     a = x1; b = x2; cc = b; fc = fb;
     for (int it = 1; it <= 100; it++) {
+        brent_iters++;
         if (sgn1(fb) * sgn1(fc) > 0.0) { cc = a; fc = fa; d = b - a; e = d; }
         if (fabs(fc) < fabs(fb)) { a = b; b = cc; cc = a; fa = fb; fb = fc; fc = fa; }
         const double tol1 = 2.0 * EPS * fabs(b) + 0.5 * xacc;
@@ -1622,17 +1635,39 @@ void orc_boundaries(const orc_problem *P, double *p)
                     }
             PERIODIC_Z();
         } else { /* trans-polar theta with phi -> phi + pi, then periodic phi (conditions.f90:1032-1059) */
+            /* A theta boundary away from the pole (no reference counterpart:
+             * the reference requires theta = [0, pi] for this problem,
+             * parameters.f90:577-583) gets an outflow fill instead: a copy of
+             * the edge cell with rho and p floored and v^theta limited to
+             * outflow.  beta^theta = 0 for every metric of this geometry, so
+             * the shift-aware limit of the radial rule is v^theta = 0. */
+            const double pi = 3.14159265358979323846, tol = 128.0 * DBL_EPSILON * pi;
+            const int open_lo = P->c.y_min > tol, open_hi = P->c.y_max < pi - tol;
             for (int k = 1; k <= nz; k++) {
                 int ks = k + nz / 2;
                 if (ks > nz) ks -= nz;
                 for (int i = -ng; i <= nx + ng; i++)
                     for (int g = 1; g <= ng; g++) {
-                        ALLV Q(v, i, 1 - g, k) = Q(v, i, g, ks);
-                        Q(EQ_VY, i, 1 - g, k) = -Q(EQ_VY, i, g, ks);
-                        Q(EQ_VZ, i, 1 - g, k) = Q(EQ_VZ, i, g, ks);
-                        ALLV Q(v, i, ny + g, k) = Q(v, i, ny + 1 - g, ks);
-                        Q(EQ_VY, i, ny + g, k) = -Q(EQ_VY, i, ny + 1 - g, ks);
-                        Q(EQ_VZ, i, ny + g, k) = Q(EQ_VZ, i, ny + 1 - g, ks);
+                        if (open_lo) {
+                            ALLV Q(v, i, 1 - g, k) = Q(v, i, 1, k);
+                            Q(EQ_DE, i, 1 - g, k) = dmax(Q(EQ_DE, i, 1, k), P->rho_floor);
+                            Q(EQ_PR, i, 1 - g, k) = dmax(Q(EQ_PR, i, 1, k), P->p_floor);
+                            Q(EQ_VY, i, 1 - g, k) = dmin(0.0, Q(EQ_VY, i, 1, k));
+                        } else {
+                            ALLV Q(v, i, 1 - g, k) = Q(v, i, g, ks);
+                            Q(EQ_VY, i, 1 - g, k) = -Q(EQ_VY, i, g, ks);
+                            Q(EQ_VZ, i, 1 - g, k) = Q(EQ_VZ, i, g, ks);
+                        }
+                        if (open_hi) {
+                            ALLV Q(v, i, ny + g, k) = Q(v, i, ny, k);
+                            Q(EQ_DE, i, ny + g, k) = dmax(Q(EQ_DE, i, ny, k), P->rho_floor);
+                            Q(EQ_PR, i, ny + g, k) = dmax(Q(EQ_PR, i, ny, k), P->p_floor);
+                            Q(EQ_VY, i, ny + g, k) = dmax(0.0, Q(EQ_VY, i, ny, k));
+                        } else {
+                            ALLV Q(v, i, ny + g, k) = Q(v, i, ny + 1 - g, ks);
+                            Q(EQ_VY, i, ny + g, k) = -Q(EQ_VY, i, ny + 1 - g, ks);
+                            Q(EQ_VZ, i, ny + g, k) = Q(EQ_VZ, i, ny + 1 - g, ks);
+                        }
                     }
             }
             PERIODIC_Z();

@@ -54,6 +54,10 @@ void orc_cache_injector(orc_problem *P, const double *p);
 /* prim_to_cons over interior cells (grhd2.f90:126-134). */
 void orc_prim_to_cons_all(const orc_problem *P, const double *p, double *u);
 /* cons_to_prim over interior cells; u repaired in place (grhd2.f90:252-260). */
+/* out[0] = calls of the Brent fallback of the con2prim root solve
+ * (equations.f90:337-339, 343-424) and out[1] = iterations of its loop since
+ * the last reset (process-wide; reset = 1 clears them after reading). */
+void orc_brent_stats(unsigned long long *out, int reset);
 void orc_cons_to_prim_all(const orc_problem *P, double *u, double *p);
 /* set_boundary_conditions (conditions.f90:773-1066). */
 void orc_boundaries(const orc_problem *P, double *p);

@@ -79,6 +79,32 @@ r_max = 20
 michel_critical_radius = 8
 riemann_solver = hllc
 """,
+    # Michel accretion on Kerr-Schild spheroidal with spin, as the served
+    # request mix draws it (initial data conditions.f90:284-410 on the KS
+    # metric, outer injector conditions.f90:872-891)
+    "michel_ks_a090": """problem = michel
+metric = kerr_schild
+geometry = spheroidal
+logarithmic_r = true
+spin = 0.9
+nx = 96
+nz = 1
+r_min = 1.8
+r_max = 20
+michel_critical_radius = 8
+""",
+    "michel_ks_a099_hllc": """problem = michel
+metric = kerr_schild
+geometry = spheroidal
+logarithmic_r = true
+spin = 0.99
+nx = 80
+nz = 1
+r_min = 1.8
+r_max = 20
+michel_critical_radius = 8
+riemann_solver = hllc
+""",
     "dust_ef_sensor": """problem = dust
 nx = 64
 nz = 4
@@ -176,7 +202,7 @@ riemann_solver = hllc
 # cases whose initial state is perturbed (seeded) before stepping, so that
 # smooth equilibria also exercise the non-trivial branches
 NOISY = {"fm3d_ks_mp5_hllc_sensor": (2, 1.0e-2, 3435), "fm_equatorial_ks_log": (1, 1.0e-2, 3435),
-         "michel_ef_log": (2, 1.0e-3, 3435)}
+         "michel_ef_log": (2, 1.0e-3, 3435), "michel_ks_a090": (2, 1.0e-3, 3435)}
 
 
 # Michel accretion C2 (the reference's convergence case for the EF log-r

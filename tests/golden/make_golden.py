@@ -10,6 +10,14 @@ and outputs as compressed .npz fixtures:
   michel_c2_<nr>.npz: C2 at nr = 256 (and 1024) to t = 500 with seeded 1e-3
                density noise: p0, final u, the reference's L1/L2/Linf density
                error against the analytic solution (output.f90:1189-1213)
+  con2prim_<case>.npz: cons_to_prim over every interior cell of a case's
+               initial conserved state with extreme and invalid states planted
+               in some cells: ultra-hot states (tau/D >= 1e24, S up to 1.5x
+               its bound) whose rtsafe solve does not converge in 100
+               iterations and falls back to Brent (equations.f90:337-339,
+               343-424), and NaN, infinite, negative-D and negative-tau states
+               that reset to the atmosphere (equations.f90:116-124, 186-204):
+               u_in, u_out (repaired in place), p_out (interior), cells
   dust_<nr>.npz: stationary dust accretion (problem 8, EF, r in [1, 51]) at
                nr = 128, 256, 512 to t = 500: final u, p row and the dust
                norms against the analytic solution (output.f90:1214-1235)
@@ -101,6 +109,56 @@ def michel_c2(nr=256):
     print(f"michel nr={nr}: {n} steps, L1/L2/Linf = {norms}")
 
 
+CON2PRIM_CASES = ["fm3d_ks_mp5_hllc_sensor", "michel_ks_a090", "sod_mp5_hllc_sensor"]
+
+
+def con2prim_states():
+    """Conserved states planted by con2prim_fixture: (D, tau, S) with tau/D
+    from 1e24 to 1e40 and |S| a fraction of its bound sqrt(E^2 - D^2)
+    (beyond 1 exercises the momentum rescale), in three directions; then
+    states the root solve cannot take (NaN, inf, D < 0, tau < 0)."""
+    nan, inf = np.nan, np.inf
+    out = [[nan, 1.0, 0.0, 0.0, 0.0], [1.0, nan, 0.1, 0.0, 0.0], [1.0, 1.0, nan, 0.0, 0.0],
+           [-1.0, 1.0, 0.0, 0.0, 0.0], [1.0, -5.0, 0.1, 0.0, 0.0], [1.0, inf, 0.0, 0.0, 0.0],
+           [inf, 1.0, 0.0, 0.0, 0.0], [1.0, 1.0, 0.0, inf, 0.0], [0.0, 0.0, 0.0, 0.0, 0.0]]
+    dirs = [(1.0, 0.0, 0.0), (0.6, 0.8, 0.0), (0.3, 0.2, 0.93)]
+    for D in (1.0e-6, 1.0e-3, 1.0, 1.0e3):
+        for ratio in (1.0e24, 1.0e30, 1.0e40):
+            tau = D * ratio
+            smax = np.sqrt((tau + D) ** 2 - D * D)
+            for frac in (0.5, 0.999, 1.0, 1.5):
+                for d in dirs:
+                    out.append([D, tau, *(frac * smax * np.array(d))])
+    return np.array(out)
+
+
+def con2prim_fixture(name):
+    """The reference's cons_to_prim over the interior (ref_con2prim_all) on the
+    case's initial conserved state with the first min(153, ncell / 2) of
+    con2prim_states() planted at spread cells (flat interior index
+    (37 m + 5) mod ncell)."""
+    cfg = config_from_par(CASES[name])
+    R = oracle.Reference(cfg)
+    try:
+        u = R.get_cons()
+        flat = u.reshape(-1, 5)
+        ncell = flat.shape[0]
+        states = con2prim_states()[:ncell // 2]
+        cells = (37 * np.arange(len(states)) + 5) % ncell
+        assert len(set(cells.tolist())) == len(states)
+        flat[cells] = states
+        u_in = u.copy()
+        R.put_cons(u)
+        R.con2prim_all()
+        u_out, p = R.get_cons(), R.get_prim()
+        g = R.ng
+        sl = (slice(g + 1, g + 1 + R.nz), slice(g + 1, g + 1 + R.ny), slice(g + 1, g + 1 + R.nx))
+        p_out = p[sl]
+    finally:
+        R.close()
+    np.savez_compressed(OUT / f"con2prim_{name}.npz", u_in=u_in, u_out=u_out, p_out=p_out, cells=cells)
+
+
 def dust(nr):
     cfg = config_from_par(DUST.format(nr=nr))
     R = oracle.Reference(cfg)
@@ -129,6 +187,9 @@ if __name__ == "__main__":
             michel_c2(256)
         elif w.startswith("michel_c2_"):
             michel_c2(int(w.split("_")[-1]))
+        elif w == "con2prim":
+            for c in CON2PRIM_CASES:
+                con2prim_fixture(c)
         elif w.startswith("dust_"):
             dust(int(w.split("_")[-1]))
         else:

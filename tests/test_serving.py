@@ -145,3 +145,43 @@ def test_served_results_equal_alone():
             n, _ = S.advance()
             assert n == c.steps, c.label
             assert np.array_equal(S.get_conserved(), c.state), c.label
+
+
+@pytest.mark.gpu
+def test_served_results_equal_alone_at_workload_sizes():
+    """The served workload at its stated sizes in one mix: shock tubes at
+    nx = 3200 and 6400, Michel accretion at nr = 4096 on Kerr-Schild with
+    a = 0.97 and at nr = 1024 with a = 0, Kelvin-Helmholtz at 256^2 and
+    1024^2 (Gamma = 4/3, v = +-0.5, seeded v^z noise), arriving while others
+    run.  Final times are cut to 1 % of the presets' (64 steps of the nx = 6400
+    tube, 10,656 of the nr = 4096 Michel run, 246 of the 1024^2
+    Kelvin-Helmholtz run) so that it runs in well under a minute; every result
+    equals the same problem run alone through grhd_advance, bit for bit."""
+    from grhd_amd.batch import Batch, footprint
+    from grhd_amd.serving import Request, _kelvin_helmholtz, _michel, _shock_tube
+    from grhd_amd.solver import Solver
+    scale = 0.01
+    specs = [("shu_osher nx=6400", _shock_tube(3, 6400), None),
+             ("michel nr=4096 a=0.97", _michel(4096, 0.97), None),
+             ("kelvin_helmholtz 1024x1024", _kelvin_helmholtz(1024), (3, 1.0e-2, 17)),
+             ("strong_shock nx=3200", _shock_tube(2, 3200), None),
+             ("michel nr=1024 a=0", _michel(1024, 0.0), None),
+             ("kelvin_helmholtz 256x256", _kelvin_helmholtz(256), (3, 1.0e-2, 18))]
+    reqs = [Request(i, cfg.replace(final_time=cfg.final_time * scale), 0.05 * i, label, perturb=pert,
+                    want_state=True) for i, (label, cfg, pert) in enumerate(specs)]
+    print("serving at workload sizes:", [r.label for r in reqs], flush=True)
+    with Batch(capacity=8) as B:
+        rep = Server(B, footprint, hbm_budget=1 << 34, chunk=32).run(reqs)
+    s = rep.summary()
+    print("serving summary", s, flush=True)
+    assert s["requests"] == len(reqs) and rep.peak_active > 1
+    for c in rep.completed:
+        r = reqs[c.rid]
+        with Solver(r.cfg) as S:
+            S.initial_conditions()
+            if r.perturb is not None:
+                S.perturb(*r.perturb)
+            n, _ = S.advance()
+            assert n == c.steps, c.label
+            assert np.array_equal(S.get_conserved(), c.state), c.label
+        print(f"served {c.label}: {c.steps} steps, equal to the run alone", flush=True)
