@@ -122,6 +122,8 @@ _SIGS = {
     "grhd_last_timing": ([ctypes.c_void_p, ctypes.POINTER(_D), ctypes.POINTER(ctypes.c_int64)],
                          ctypes.c_int),
     "grhd_footprint": ([ctypes.POINTER(Config), ctypes.POINTER(ctypes.c_uint64)], ctypes.c_int),
+    "grhd_step_noalloc": ([ctypes.POINTER(Config), ctypes.c_void_p, ctypes.c_void_p, _D, _I, ctypes.c_void_p],
+                          ctypes.c_int),
     "grhd_batch_create": ([_I, ctypes.c_uint64, ctypes.POINTER(ctypes.c_void_p)], ctypes.c_int),
     "grhd_batch_destroy": ([ctypes.c_void_p], ctypes.c_int),
     "grhd_batch_last_error": ([ctypes.c_void_p], ctypes.c_char_p),

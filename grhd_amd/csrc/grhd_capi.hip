@@ -303,6 +303,27 @@ int upload_p(grhd_ctx *c, const double *href)
     return GRHD_OK;
 }
 
+// The coordinate table of P in order (layout at Params::coord): put(v) for
+// each entry.  Evaluated on the host with the reference's libm.
+template <class Put>
+void coord_entries(const Params &P, Put put)
+{
+    auto put_x = [&](double x) {
+        const Coord q = coord_of(P.mk, x, 0.0);
+        put(q.x);
+        put(q.r);
+        put(q.e2);
+    };
+    auto put_y = [&](double y) {
+        put(std::sin(y));
+        put(std::cos(y));
+    };
+    for (int i = -P.ng; i <= P.nx + P.ng; i++) put_x(xc(P, i));
+    for (int i = 0; i <= P.nx; i++) put_x(xf(P, i));
+    for (int j = -P.ng; j <= P.ny + P.ng; j++) put_y(yc(P, j));
+    for (int j = 0; j <= P.ny; j++) put_y(yf(P, j));
+}
+
 int run_steps(grhd_ctx *c, const double *dts, int n)
 {
     const Params &P = c->P;
@@ -439,20 +460,7 @@ int grhd_create(const grhd_config *cfg, grhd_ctx **out)
         // on the host (initialization.f90:84-263)
         std::vector<double> t((size_t)coord_table_size(P.nx, P.ny, P.ng));
         size_t o = 0;
-        auto put_x = [&](double x) {
-            const Coord q = coord_of(P.mk, x, 0.0);
-            t[o++] = q.x;
-            t[o++] = q.r;
-            t[o++] = q.e2;
-        };
-        auto put_y = [&](double y) {
-            t[o++] = std::sin(y);
-            t[o++] = std::cos(y);
-        };
-        for (int i = -P.ng; i <= P.nx + P.ng; i++) put_x(xc(P, i));
-        for (int i = 0; i <= P.nx; i++) put_x(xf(P, i));
-        for (int j = -P.ng; j <= P.ny + P.ng; j++) put_y(yc(P, j));
-        for (int j = 0; j <= P.ny; j++) put_y(yf(P, j));
+        coord_entries(P, [&](double v) { t[o++] = v; });
         if ((e = hipMalloc(&c->coord, t.size() * sizeof(double))) != hipSuccess) return bail(e, "hipMalloc coord");
         if ((e = hipMemcpy(c->coord, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice)) != hipSuccess)
             return bail(e, "hipMemcpy coord");
@@ -956,3 +964,155 @@ int grhd_last_timing(grhd_ctx *c, double *ms, int64_t *launches)
 }
 
 }  // extern "C"
+
+// ----------------------------------------------------------- no allocation --
+// grhd_step_noalloc keeps every intermediate of the step -- face fluxes, the
+// RK stage buffer, the Michel injector, the coordinate table and the
+// counters -- in one file-scope device array.  Each device that loads the
+// library holds its own copy for the life of the process, at its own
+// address: in host code `scratch` names only a host-side stand-in, so the
+// address of the current device's copy comes from hipGetSymbolAddress, once
+// per device, and only that address reaches kernels and copies.
+namespace {
+
+constexpr size_t kScratchBytes = (size_t)GRHD_NOALLOC_SCRATCH_BYTES;
+__device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char *g_scratch_dev[kMaxDevices] = {};
+
+// Address of the current device's scratch array (looked up on the first call
+// on that device).
+int scratch_address(unsigned char **out)
+{
+    int dev = -1;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return fail(nullptr, GRHD_ERR_HIP, std::string("hipGetDevice: ") + hipGetErrorString(e));
+    if (dev < 0 || dev >= kMaxDevices) return fail(nullptr, GRHD_ERR_UNSUPPORTED, "device index beyond 63");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void *ptr = nullptr;
+        e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return fail(nullptr, GRHD_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_dev[dev] = static_cast<unsigned char *>(ptr);
+    }
+    *out = g_scratch_dev[dev];
+    return GRHD_OK;
+}
+
+__global__ void k_clear_counters(unsigned long long *c)
+{
+    if (threadIdx.x < 4) c[threadIdx.x] = 0ull;
+}
+
+// The Michel outer injector (conditions.f90:229-246) from the outer radial
+// ghost layers of p, which hold it: the initial data puts it there and every
+// ghost fill writes it back.  injector[(((k-1) ny + j-1) ng + g-1) 5 + v].
+__global__ void k_injector_from_ghosts(Params P, const double *__restrict__ p, double *__restrict__ inj)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)P.ng * P.ny * P.nz) return;
+    const int g = (int)(t % P.ng) + 1;
+    const int64_t r = t / P.ng;
+    const int j = (int)(r % P.ny) + 1, k = (int)(r / P.ny) + 1;
+    const int64_t o = pindex(P, P.nx + g, j, k);
+    for (int v = 0; v < NEQ; v++) inj[t * NEQ + v] = p[v * P.vstride + o];
+}
+
+size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
+
+// Uploads P's coordinate table to dst through a fixed host staging buffer
+// (no allocation); synchronous on s.
+std::mutex g_stage_mu;
+double g_stage[4096];
+
+int upload_coord(const Params &P, double *dst, hipStream_t s)
+{
+    std::lock_guard<std::mutex> lk(g_stage_mu);
+    size_t n = 0, done = 0;
+    hipError_t e = hipSuccess;
+    auto flush = [&]() {
+        if (n == 0 || e != hipSuccess) return;
+        e = hipMemcpyAsync(dst + done, g_stage, n * sizeof(double), hipMemcpyHostToDevice, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        done += n;
+        n = 0;
+    };
+    coord_entries(P, [&](double v) {
+        g_stage[n++] = v;
+        if (n == sizeof(g_stage) / sizeof(g_stage[0])) flush();
+    });
+    flush();
+    if (e != hipSuccess) return fail(nullptr, GRHD_ERR_HIP, std::string("coordinate table upload: ") + hipGetErrorString(e));
+    return GRHD_OK;
+}
+
+}  // namespace
+
+extern "C" int grhd_step_noalloc(const grhd_config *cfg, double *p, double *u, double dt, int32_t nsteps,
+                                 void *stream)
+{
+    if (!cfg || !p || !u || nsteps < 0) return fail(nullptr, GRHD_ERR_INVALID_ARGUMENT, "null argument or nsteps < 0");
+    std::string msg;
+    if (validate(*cfg, msg)) return fail(nullptr, GRHD_ERR_INVALID_CONFIG, msg);
+    grhd_config c = *cfg;
+    if (c.metric == GRHD_METRIC_KERR_SCHILD) c.geometry = GRHD_GEOM_SPHEROIDAL;
+    Params P = make_params(c);
+    if ((P.problem == 5 && (P.nx < P.ng || (P.nz > 1 && P.nz < P.ng))) || (P.problem == 11 && P.ny < P.ng) ||
+        (P.problem == 12 && P.ny < P.ng))
+        return fail(nullptr, GRHD_ERR_INVALID_CONFIG, "grid smaller than the ghost depth");
+    const bool mixed = (cfg->precision == GRHD_PRECISION_MIXED);
+    const size_t rsz = mixed ? sizeof(float) : sizeof(double);
+    const int64_t ncell = (int64_t)P.nx * P.ny * P.nz;
+    const int64_t fstr[3] = {(int64_t)(P.nx + 1) * P.ny * P.nz, (int64_t)P.nx * (P.ny + 1) * P.nz,
+                             (int64_t)P.nx * P.ny * (P.nz + 1)};
+    // carve the array: faces of the active directions, the stage buffer, the
+    // injector, the coordinate table, the counters
+    size_t off[7], need = 0;
+    for (int d = 0; d < 3; d++) {
+        const bool active = (d == 0) || (d == 1 && P.ny > 1) || (d == 2 && P.nz > 1);
+        off[d] = need;
+        need += align256((active ? (size_t)fstr[d] * NEQ : 1) * rsz);
+    }
+    off[3] = need;
+    need += align256((size_t)ncell * NEQ * sizeof(double));
+    off[4] = need;
+    need += align256((size_t)NEQ * P.ng * P.ny * P.nz * sizeof(double) + 8);
+    off[5] = need;
+    need += align256((size_t)coord_table_size(P.nx, P.ny, P.ng) * sizeof(double));
+    off[6] = need;
+    need += align256(4 * sizeof(unsigned long long));
+    if (need > kScratchBytes)
+        return fail(nullptr, GRHD_ERR_CAPACITY,
+                    "grhd_step_noalloc: the problem needs " + std::to_string(need) +
+                        " bytes of intermediates, the scratch array holds " + std::to_string(kScratchBytes));
+    unsigned char *base = nullptr;
+    int rc = scratch_address(&base);
+    if (rc != GRHD_OK) return rc;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    void *F[3] = {base + off[0], base + off[1], base + off[2]};
+    double *ustage = reinterpret_cast<double *>(base + off[3]);
+    double *inj = reinterpret_cast<double *>(base + off[4]);
+    double *coord = reinterpret_cast<double *>(base + off[5]);
+    unsigned long long *counters = reinterpret_cast<unsigned long long *>(base + off[6]);
+    rc = upload_coord(P, coord, s);
+    if (rc != GRHD_OK) return rc;
+    P.coord = coord;
+    // the array keeps the last call's contents: the counters start at zero
+    k_clear_counters<<<1, 64, 0, s>>>(counters);
+    if (P.problem == 7)
+        k_injector_from_ghosts<<<(unsigned)(((int64_t)P.ng * P.ny * P.nz + 255) / 256), 256, 0, s>>>(P, p, inj);
+    launch_boundaries(P, s, p, inj);  // grhd2.f90:233
+    double *ua = u, *ub = ustage;
+    for (int st = 0; st < nsteps; st++) {
+        for (int rk = 1; rk <= 3; rk++) launch_stage(P, s, p, F, fstr, ua, ub, rk, dt, counters, mixed, inj, true);
+        std::swap(ua, ub);
+    }
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess && ua != u)
+        e = hipMemcpyAsync(u, ua, (size_t)ncell * NEQ * sizeof(double), hipMemcpyDeviceToDevice, s);
+    if (e != hipSuccess) return fail(nullptr, GRHD_ERR_HIP, std::string("grhd_step_noalloc: ") + hipGetErrorString(e));
+    return GRHD_OK;
+}

@@ -240,6 +240,31 @@ int grhd_last_timing(grhd_ctx *ctx, double *ms, int64_t *launches);
  * problems into a batch by HBM footprint. */
 int grhd_footprint(const grhd_config *cfg, uint64_t *bytes);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): grhd_step on a problem whose state the caller keeps in
+ * its own device buffers, with no context.  nsteps SSP-RK3 steps at a fixed
+ * dt (grhd2.f90:232-265: ghost fill, then three stages of calc_rhs, RK
+ * combination, cons_to_prim and ghost fill), results bit-identical to
+ * grhd_step.  Every intermediate (face fluxes, the RK stage buffer, the
+ * coordinate table, the Michel injector, the counters) lives in one
+ * file-scope device array of GRHD_NOALLOC_SCRATCH_BYTES per device, held until
+ * the process ends; the call allocates nothing.
+ *   p: device ghosted primitives, read and written, SoA [5][NZ][NY][NX] with
+ *      NX = nx + 2 ng + 1 and NY (NZ) = ny + 2 ng + 1 (nz + 2 ng + 1) when
+ *      ny (nz) > 1, else 1; ng = 1 Godunov, 2 TVD and WENO3, 3 MP5 and WENO5.
+ *      For Michel (problem 7) the outer radial ghost layers must hold the
+ *      injector state, as the initial data leaves them.
+ *   u: device interior conserved variables U^n, read and written, [5][nz][ny][nx].
+ *   stream: a hipStream_t (NULL: the default stream); the call returns after
+ *      queueing the steps (the coordinate-table upload synchronises it).
+ * GRHD_ERR_CAPACITY when the intermediates exceed the array (about 400k cells
+ * in FP64 3D).  Calls that overlap on one device race on the array.  The
+ * counters are cleared at each call and not reported. */
+#define GRHD_NOALLOC_SCRATCH_BYTES (64u << 20)
+int grhd_step_noalloc(const grhd_config *cfg, double *p, double *u, double dt, int32_t nsteps,
+                      void *stream);
+
 /* ------------------------------------------------------------------ batch --
  * Many independent problems resident on one device and advanced together.
  * The reference runs one problem per process (grhd2.f90); these entry points
