@@ -11,6 +11,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <string>
@@ -36,7 +38,7 @@ constexpr int kMaxPace = 8;     // steps of a cheaper group per step of the most
 // estimate cannot separate steps that differ by less, and a cheap group given
 // too many steps becomes the chunk's critical path.
 constexpr int kMinPace = 3;
-constexpr int kTuneSteps = 2;   // steps timed per candidate
+constexpr int kTuneSteps = 3;   // steps timed per candidate
 // Problems of at most this many cells run whole chunks of steps in one block
 // each (kb_small); larger ones use the launch form the autotuner picks.  One
 // block is latency-bound (one or two waves per SIMD): measured 10 % faster
@@ -55,6 +57,13 @@ struct Variant {
 const Variant kVariants[] = {{false, false, 4}, {false, true, 4}, {false, true, 1}, {true, false, 4},
                              {true, false, 1}};
 constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
+// The autotuner times the phase forms only.  The persistent stage kernel
+// holds the union of the face and cell code at 178 VGPRs (two waves per SIMD
+// against four for the phase kernels), runs the run-time face scheme and waits
+// at two grid barriers per stage: it measured 3.6-4x slower than the best
+// phase form on the 2D and 3D tori, so it stays selectable
+// (grhd_batch_set_variant) but is never chosen by timing.
+constexpr int kNumTuned = 3;
 
 struct BRun {
     grhd_ctx *ctx = nullptr;
@@ -126,6 +135,26 @@ int bfail(grhd_batch *b, int code, const std::string &m)
         if (e_ != hipSuccess)                                                                 \
             return bfail(b, e_ == hipErrorOutOfMemory ? GRHD_ERR_OUT_OF_MEMORY : GRHD_ERR_HIP, \
                          std::string(#call) + ": " + hipGetErrorString(e_));                  \
+    } while (0)
+
+// GRHD_TRACE_BATCH=1 prints each HIP call of a chunk (name, sizes, the
+// graph's key) to stderr, flushed, before it is made: a crash inside the
+// runtime then names the call it happened in.
+bool trace_batch()
+{
+    static const bool on = [] {
+        const char *e = std::getenv("GRHD_TRACE_BATCH");
+        return e && e[0] == '1';
+    }();
+    return on;
+}
+#define BTRACE(...)                                        \
+    do {                                                   \
+        if (trace_batch()) {                               \
+            std::fprintf(stderr, "grhd_batch: " __VA_ARGS__); \
+            std::fputc('\n', stderr);                     \
+            std::fflush(stderr);                           \
+        }                                                  \
     } while (0)
 
 void drop_graphs(grhd_batch *b)
@@ -334,10 +363,13 @@ int graph_for(grhd_batch *b, int v, int K, hipGraphExec_t &exec, int64_t &launch
     }
     launches = 0;
     hipGraph_t g = nullptr;
+    BTRACE("hipStreamBeginCapture (variant %d, %d steps)", v, K);
     BCHK(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
     enqueue_steps(b, v, K, launches);
+    BTRACE("hipStreamEndCapture");
     hipError_t e = hipStreamEndCapture(b->stream, &g);
     if (e != hipSuccess) return bfail(b, GRHD_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    BTRACE("hipGraphInstantiate");
     e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
     if (e != hipSuccess) return bfail(b, GRHD_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
@@ -407,20 +439,32 @@ int run_chunk(grhd_batch *b, int v, int K, double *ms)
     std::vector<int> active;
     fill_dts(b, K, active);
     fill_runs(b);
-    BCHK(b, hipMemcpyAsync(b->d_runs, b->h_runs, (size_t)b->cap * sizeof(RunDesc), hipMemcpyHostToDevice,
-                           b->stream));
-    BCHK(b, hipMemcpyAsync(b->d_dts, b->h_dts, (size_t)K * chunk_ratio(b) * b->cap * sizeof(double), hipMemcpyHostToDevice,
-                           b->stream));
+    const size_t runs_bytes = (size_t)b->cap * sizeof(RunDesc);
+    const size_t dts_bytes = (size_t)K * chunk_ratio(b) * b->cap * sizeof(double);
+    // both copies stay inside their pinned and device buffers (allocated for
+    // kMaxChunk x kMaxPace rows of cap slots)
+    if (dts_bytes > (size_t)kMaxChunk * kMaxPace * b->cap * sizeof(double))
+        return bfail(b, GRHD_ERR_INVALID_ARGUMENT, "chunk larger than the dt table");
+    BTRACE("hipMemcpyAsync runs %zu bytes from %p to %p", runs_bytes, (void *)b->h_runs, (void *)b->d_runs);
+    BCHK(b, hipMemcpyAsync(b->d_runs, b->h_runs, runs_bytes, hipMemcpyHostToDevice, b->stream));
+    BTRACE("hipMemcpyAsync dts %zu bytes from %p to %p", dts_bytes, (void *)b->h_dts, (void *)b->d_dts);
+    BCHK(b, hipMemcpyAsync(b->d_dts, b->h_dts, dts_bytes, hipMemcpyHostToDevice, b->stream));
     hipGraphExec_t exec = nullptr;
     int64_t launches = 0;
+    BTRACE("graph for variant %d, %d steps", v, K);
     int rc = graph_for(b, v, K, exec, launches);
     if (rc != GRHD_OK) return rc;
+    BTRACE("hipEventRecord ev0");
     BCHK(b, hipEventRecord(b->ev0, b->stream));
+    BTRACE("hipGraphLaunch %p (%lld launches)", (void *)exec, (long long)launches);
     BCHK(b, hipGraphLaunch(exec, b->stream));
+    BTRACE("hipEventRecord ev1");
     BCHK(b, hipEventRecord(b->ev1, b->stream));
+    BTRACE("hipStreamSynchronize");
     BCHK(b, hipStreamSynchronize(b->stream));
     float f = 0.0f;
     BCHK(b, hipEventElapsedTime(&f, b->ev0, b->ev1));
+    BTRACE("chunk done, %.3f ms", (double)f);
     if (ms) *ms = f;
     b->last_launches += launches;
     if (kVariants[v].persistent) {
@@ -683,10 +727,10 @@ int grhd_batch_step(grhd_batch *b, int32_t nsteps)
                     // small problems only: no launch form to choose
                 } else if (it != b->tuned.end()) {
                     if (it->second != b->variant && (rc = rebuild(b, it->second)) != GRHD_OK) return rc;
-                } else if (left >= 2 * kTuneSteps * kNumVariants) {
+                } else if (left >= 2 * kTuneSteps * kNumTuned) {
                     double best = 1e300;
                     int bv = 0;
-                    for (int c = 0; c < kNumVariants; c++) {
+                    for (int c = 0; c < kNumTuned; c++) {
                         if ((rc = rebuild(b, c)) != GRHD_OK) return rc;
                         double ms = 0.0;
                         // first chunk captures and instantiates, second is timed

@@ -170,6 +170,8 @@ __global__ __launch_bounds__(256) void kb_bound(BatchTables T, const double *__r
 // resets it and advances the generation bar[1].  The wait is bounded by the
 // 100 MHz constant clock: past the limit the block records the failure in
 // *err and continues, so the kernel always ends and the host reports it.
+// Once *err is set (by this or an earlier barrier of the captured chunk) no
+// block waits any more, so the rest of the chunk drains at once.
 __device__ __forceinline__ void grid_barrier(unsigned *bar, unsigned nblocks, unsigned *err,
                                              uint64_t limit_ticks)
 {
@@ -182,10 +184,11 @@ __device__ __forceinline__ void grid_barrier(unsigned *bar, unsigned nblocks, un
         if (arrived == nblocks - 1) {
             __hip_atomic_store(&bar[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             __hip_atomic_fetch_add(&bar[1], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        } else {
+        } else if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u) {
             const uint64_t t0 = wall_clock64();
             while (__hip_atomic_load(&bar[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == gen) {
                 __builtin_amdgcn_s_sleep(1);
+                if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) break;
                 if (wall_clock64() - t0 > limit_ticks) {
                     atomicOr(err, 1u);
                     break;

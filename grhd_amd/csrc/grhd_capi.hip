@@ -142,6 +142,7 @@ Params make_params(const grhd_config &c)
     P.sensor = c.shock_sensor ? 1 : 0;
     P.polar = (c.geometry == GRHD_GEOM_SPHERICAL || c.geometry == GRHD_GEOM_SPHEROIDAL);
     P.cyl = (c.geometry == GRHD_GEOM_CYLINDRICAL);
+    P.face_gw[0] = P.face_gw[1] = P.face_gw[2] = 4;
     if (c.problem == 12) {
         const double tol = 128.0 * DBL_EPSILON * kPi;
         P.open_lo = c.y_min > tol;
@@ -324,8 +325,52 @@ void coord_entries(const Params &P, Put put)
     for (int j = 0; j <= P.ny; j++) put_y(yf(P, j));
 }
 
+// Face-kernel tiling of the problem, chosen once per context by timing: for
+// each active direction, the face kernel alone with 4, 2, 8 and 1 waves sharing
+// one LDS copy of the face geometry (one warm-up launch, then three timed),
+// on the problem's own primitives.  The face kernels write only the face
+// arrays, which the next stage rewrites, and every tiling gives the same
+// fluxes, so tuning changes no result.  Grids in the fused stage form keep
+// groups of four.
+int tune_faces(grhd_ctx *c)
+{
+    c->faces_tuned = true;
+    if (single_stage_fused(c->P)) return GRHD_OK;
+    static const int cand[] = {4, 2, 8, 1};
+    for (int d = 0; d < 3; d++) {
+        if (!(d == 0 || (d == 1 && c->P.ny > 1) || (d == 2 && c->P.nz > 1))) continue;
+        float best = 0.0f;
+        int best_gw = 4;
+        for (int gw : cand) {
+            Params P = c->P;
+            P.face_gw[d] = gw;
+            for (int rep = 0; rep < 4; rep++) {
+                if (rep == 1) HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+                if (c->mixed) launch_face_dir_f32(P, c->stream, c->p, c->F, c->fstr, d + 1);
+                else launch_face_dir_f64(P, c->stream, c->p, c->F, c->fstr, d + 1);
+            }
+            HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+            HIPCHK(c, hipEventSynchronize(c->ev1));
+            float ms = 0.0f;
+            HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+            if (best == 0.0f || ms < best) {
+                best = ms;
+                best_gw = gw;
+            }
+        }
+        c->P.face_gw[d] = best_gw;
+    }
+    HIPCHK(c, hipGetLastError());
+    return GRHD_OK;
+}
+
 int run_steps(grhd_ctx *c, const double *dts, int n)
 {
+    if (!c->faces_tuned && n > 0) {
+        launch_boundaries(c->P, c->stream, c->p, c->injector);
+        const int rc = tune_faces(c);
+        if (rc != GRHD_OK) return rc;
+    }
     const Params &P = c->P;
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));
     int64_t launches = 0;

@@ -14,6 +14,7 @@ struct grhd_ctx {
     hipStream_t stream = nullptr;
     bool own_stream = false;
     bool mixed = false;
+    bool faces_tuned = false;  // face-kernel tiling chosen (tune_faces)
     double *p = nullptr;      // ghosted primitives, SoA [5][NZs][NYs][NXs]
     double *u[2] = {nullptr, nullptr};  // conserved, SoA [5][nz][ny][nx]
     int ucur = 0;             // u[ucur] holds U^n

@@ -288,6 +288,12 @@ void launch_stage_f64(const Params &P, hipStream_t s, double *p, void *const F[3
     GRHD_STAGE_DISPATCH(double)
 }
 
+void launch_face_dir_f64(const Params &P, hipStream_t s, const double *p, void *const F[3], const int64_t fstr[3],
+                         int dir)
+{
+    GRHD_FACE_DIR_DISPATCH(double)
+}
+
 void launch_stage(const Params &P, hipStream_t s, double *p, void *const F[3], const int64_t fstr[3],
                   const double *u0, double *u, int stage, double dt, unsigned long long *counters,
                   bool mixed, const double *injector, bool with_boundaries)

@@ -32,6 +32,12 @@ inline bool single_stage_fused(const Params &P)
     return n <= kFusedStageCells || ((P.ny == 1 || P.nz == 1) && n <= kFusedStage2DCells);
 }
 
+// The face kernel of direction dir (1..3) alone (face-tiling tuner).
+void launch_face_dir_f64(const Params &P, hipStream_t s, const double *p, void *const F[3], const int64_t fstr[3],
+                         int dir);
+void launch_face_dir_f32(const Params &P, hipStream_t s, const double *p, void *const F[3], const int64_t fstr[3],
+                         int dir);
+
 // One RK stage: face fluxes, cell update (stage 1..3; 0 = rhs only into u),
 // then the ghost fill when with_boundaries.
 void launch_stage(const Params &P, hipStream_t s, double *p, void *const F[3], const int64_t fstr[3],

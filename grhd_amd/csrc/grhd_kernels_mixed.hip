@@ -13,4 +13,10 @@ void launch_stage_f32(const Params &P, hipStream_t s, double *p, void *const F[3
     GRHD_STAGE_DISPATCH(float)
 }
 
+void launch_face_dir_f32(const Params &P, hipStream_t s, const double *p, void *const F[3], const int64_t fstr[3],
+                         int dir)
+{
+    GRHD_FACE_DIR_DISPATCH(float)
+}
+
 }  // namespace grhd

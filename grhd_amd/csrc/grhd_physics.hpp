@@ -46,6 +46,10 @@ struct Params {
     // fishbone_3d with a theta boundary away from the pole (low / high side):
     // outflow ghost fill there instead of the trans-polar one
     int open_lo, open_hi;
+    // waves of a face kernel sharing one LDS copy of the face geometry, per
+    // direction (1, 2, 4 or 8): the single-problem launchers' tiling, chosen
+    // by timing the face kernels on the problem (tune_faces)
+    int face_gw[3];
     double gam, g1, mass, spin;
     double rho_floor, p_floor;
     double x_min, dx, y_min, dy, z_min, dz;

@@ -184,7 +184,7 @@ __host__ __device__ inline int64_t flux_blocks(const Params &P, FluxTiling t, in
 // Four waves per SIMD: the compiler keeps the face kernels at 128 VGPRs with a
 // few doubles of scratch spill, measured faster than 151 VGPRs at three waves.
 template <int MK, int DIR, typename R, int SCH>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_flux(
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_flux(
     Params P, const double *__restrict__ p, R *__restrict__ F, int64_t fstride, int kt, int gw)
 {
     extern __shared__ double lds_geo[];
@@ -196,9 +196,11 @@ template <int MK, int DIR, typename R, int SCH>
 void launch_flux(const Params &P, hipStream_t s, const double *p, R *F, int64_t fstr)
 {
     const int nfz = P.nz + (DIR == 3);
-    const FluxTiling t = flux_tiling(nfz);
-    const size_t lds = (size_t)(4 / t.gw) * NFG * 64 * sizeof(R);
-    k_flux<MK, DIR, R, SCH><<<(unsigned)flux_blocks<DIR>(P, t), 256, lds, s>>>(P, p, F, fstr, t.kt, t.gw);
+    const FluxTiling t = flux_tiling(nfz, P.face_gw[DIR - 1]);
+    const int waves = t.gw > 4 ? t.gw : 4;
+    const size_t lds = (size_t)(waves / t.gw) * NFG * 64 * sizeof(R);
+    k_flux<MK, DIR, R, SCH><<<(unsigned)flux_blocks<DIR>(P, t, waves), 64 * waves, lds, s>>>(P, p, F, fstr, t.kt,
+                                                                                             t.gw);
 }
 
 // The compiled face scheme for a problem: the Kerr-Schild torus
@@ -222,6 +224,40 @@ void launch_faces(const Params &P, hipStream_t s, const double *p, R *Fx, R *Fy,
     if (P.ny > 1) launch_flux<MK, 2, R, SCH>(P, s, p, Fy, fstr[1]);
     if (P.nz > 1) launch_flux<MK, 3, R, SCH>(P, s, p, Fz, fstr[2]);
 }
+
+// The face kernel of one direction (1..3) alone, with the problem's compiled
+// face scheme (the face-tiling tuner times it; it writes only the face array).
+template <int MK, typename R>
+void launch_face_dir_mk(const Params &P, hipStream_t s, const double *p, void *const F[3], const int64_t fstr[3],
+                        int dir)
+{
+    R *Fd = (R *)F[dir - 1];
+    const int sch = face_scheme<MK>(P);
+#define GRHD_FACE_DIR(SCHV)                                                        \
+    do {                                                                           \
+        if (dir == 1) launch_flux<MK, 1, R, SCHV>(P, s, p, Fd, fstr[0]);           \
+        else if (dir == 2) launch_flux<MK, 2, R, SCHV>(P, s, p, Fd, fstr[1]);      \
+        else launch_flux<MK, 3, R, SCHV>(P, s, p, Fd, fstr[2]);                    \
+    } while (0)
+    if constexpr (MK == MK_KS || MK == MK_KS_LOG) {
+        if (sch == 1) { GRHD_FACE_DIR(1); return; }
+        if (sch == 2) { GRHD_FACE_DIR(2); return; }
+    }
+    GRHD_FACE_DIR(0);
+#undef GRHD_FACE_DIR
+}
+
+#define GRHD_FACE_DIR_DISPATCH(R)                                                                   \
+    switch (P.mk) {                                                                                 \
+    case MK_MINK_CART: launch_face_dir_mk<MK_MINK_CART, R>(P, s, p, F, fstr, dir); break;          \
+    case MK_MINK_CYL: launch_face_dir_mk<MK_MINK_CYL, R>(P, s, p, F, fstr, dir); break;            \
+    case MK_MINK_SPH: launch_face_dir_mk<MK_MINK_SPH, R>(P, s, p, F, fstr, dir); break;            \
+    case MK_MINK_SPH_LOG: launch_face_dir_mk<MK_MINK_SPH_LOG, R>(P, s, p, F, fstr, dir); break;    \
+    case MK_EF: launch_face_dir_mk<MK_EF, R>(P, s, p, F, fstr, dir); break;                        \
+    case MK_EF_LOG: launch_face_dir_mk<MK_EF_LOG, R>(P, s, p, F, fstr, dir); break;                \
+    case MK_KS: launch_face_dir_mk<MK_KS, R>(P, s, p, F, fstr, dir); break;                        \
+    default: launch_face_dir_mk<MK_KS_LOG, R>(P, s, p, F, fstr, dir); break;                       \
+    }
 
 // --------------------------------------------------------- cell update --
 // Cell t (0 <= t < nx ny nz, x fastest) of one problem.  stage 1..3: rhs
@@ -418,7 +454,7 @@ void launch_stage_mk(const Params &P, hipStream_t s, double *p, void *const F[3]
     int groups = 1;
     for (int d = 0; d < 3; d++) {
         const bool on = (d == 0) || (d == 1 && P.ny > 1) || (d == 2 && P.nz > 1);
-        const FluxTiling t = flux_tiling(P.nz + (d == 2));
+        const FluxTiling t = flux_tiling(P.nz + (d == 2), P.face_gw[d] > 4 ? 4 : P.face_gw[d]);  // 256-thread blocks
         f.kt[d] = t.kt;
         f.gw[d] = t.gw;
         f.nb[d] = !on ? 0 : (d == 0) ? flux_blocks<1>(P, t) : (d == 1) ? flux_blocks<2>(P, t) : flux_blocks<3>(P, t);

@@ -295,10 +295,11 @@ int grhd_batch_remove(grhd_batch *b, int32_t slot);
 int grhd_batch_step(grhd_batch *b, int32_t nsteps);
 /* done: final_time reached; t: its time; steps: steps taken in the batch. */
 int grhd_batch_status(grhd_batch *b, int32_t slot, int32_t *done, double *t, int64_t *steps);
-/* Launch form of the batched stage: 0 autotuned (default); 1 one kernel per
- * face direction, update, con2prim and ghost fill; 2-3 fused faces, cells
- * and ghost fill with 4/1 waves per face-geometry column; 4-5 one persistent
- * kernel per stage with 4/1.  All forms give identical results. */
+/* Launch form of the batched stage: 0 autotuned over forms 1-3 (default);
+ * 1 one kernel per face direction, update, con2prim and ghost fill; 2-3 fused
+ * faces, cells and ghost fill with 4/1 waves per face-geometry column; 4-5 one
+ * persistent kernel per stage with 4/1 (selectable only: slower than the
+ * phase forms).  All forms give identical results. */
 int grhd_batch_set_variant(grhd_batch *b, int32_t variant);
 /* Pacing (no reference counterpart; default 1).  With max_ratio > 1 and the
  * groups of different metric kinds or arithmetic on their own streams (the

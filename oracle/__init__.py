@@ -81,8 +81,17 @@ def oracle_lib():
             getattr(lib, name).argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_int, ctypes.c_int,
                                            ctypes.c_int]
         lib.orc_brent_stats.argtypes = [ctypes.POINTER(ctypes.c_ulonglong), ctypes.c_int]
+        lib.orc_hllc_stats.argtypes = [ctypes.POINTER(ctypes.c_ulonglong), ctypes.c_int]
         _oracle_lib = lib
     return _oracle_lib
+
+
+def hllc_stats(reset: bool = True) -> tuple[int, int]:
+    """(HLLC calls, HLLE fallbacks) of the oracle's Riemann solver since the
+    last reset (fluxes.f90:174-397)."""
+    out = (ctypes.c_ulonglong * 2)()
+    oracle_lib().orc_hllc_stats(out, 1 if reset else 0)
+    return int(out[0]), int(out[1])
 
 
 def brent_stats(reset: bool = True) -> tuple[int, int]:

@@ -741,6 +741,8 @@ static int local_to_coord(const double *Ul, const double *Fl, double w, double a
     return 1;
 }
 
+static unsigned long long hllc_calls, hllc_fallbacks;
+
 /* calc_hllc_fluxes (fluxes.f90:174-397): Mignone-Bodo HLLC in a local
  * orthonormal face frame, HLLE fallback on every degeneracy. */
 static void hllc(const orc_problem *P, const double *qL, const double *qR, int dir, double alpha,
@@ -857,13 +859,24 @@ static void hllc(const orc_problem *P, const double *qL, const double *qR, int d
         return;
     }
 fallback:
+    hllc_fallbacks++;
     hlle(P, qL, qR, dir, alpha, beta, g, sqg, F);
+}
+
+/* HLLC calls and HLLE fallbacks since the last orc_hllc_stats(reset = 1):
+ * instrumentation, with no effect on any result. */
+void orc_hllc_stats(unsigned long long *out, int reset)
+{
+    out[0] = hllc_calls;
+    out[1] = hllc_fallbacks;
+    if (reset) hllc_calls = hllc_fallbacks = 0;
 }
 
 /* resolve_riemann_problem (fluxes.f90:56-72). */
 static void riemann(const orc_problem *P, const double *qL, const double *qR, int dir,
                     double alpha, const double beta[3], const double g[3][3], double sqg, double *F)
 {
+    if (P->c.rs == ORC_RS_HLLC) hllc_calls++;
     if (P->c.rs == ORC_RS_HLLC) hllc(P, qL, qR, dir, alpha, beta, g, sqg, F);
     else hlle(P, qL, qR, dir, alpha, beta, g, sqg, F);
 }

@@ -58,6 +58,9 @@ void orc_prim_to_cons_all(const orc_problem *P, const double *p, double *u);
  * (equations.f90:337-339, 343-424) and out[1] = iterations of its loop since
  * the last reset (process-wide; reset = 1 clears them after reading). */
 void orc_brent_stats(unsigned long long *out, int reset);
+/* out[0] = HLLC calls, out[1] = their HLLE fallbacks (fluxes.f90:174-397)
+ * since the last reset (process-wide). */
+void orc_hllc_stats(unsigned long long *out, int reset);
 void orc_cons_to_prim_all(const orc_problem *P, double *u, double *p);
 /* set_boundary_conditions (conditions.f90:773-1066). */
 void orc_boundaries(const orc_problem *P, double *p);
