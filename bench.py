@@ -7,15 +7,18 @@ r in [1.2, 40], theta in [0.05 pi, 0.95 pi], phi in [0, 2 pi); Gamma = 4/3,
 r_in = 6, r_max = 12, polytropic K with rho_max = 1; MP5 reconstruction with the
 shock-sensor cascade, HLLC with HLLE fallback; seeded 1e-2 density noise.
 State resident in HBM before timing.  Reports FP64 (headline) and mixed
-FP32-face / FP64-state throughput, the HBM roofline fraction of the stage
-kernels from the algorithmic 560 B per cell-update, and the reference CPU path
-(OpenMP Fortran built from its sources) on a bounded sample on the host cores.
-Then a served ensemble: a seeded Poisson stream of independent problems of
-mixed kinds and sizes, continuously batched on the device, with p50/p99
-time-to-solution and aggregate cell-updates/s ("ensemble").
+FP32-face / FP64-state throughput with the mixed run's density error against
+the FP64 run, the HBM roofline fraction of the stage kernels from the
+algorithmic 560 B per cell-update, and the reference CPU path (OpenMP Fortran
+built from its sources) on the same grid for a bounded number of steps on the
+host cores.  Then a served ensemble: a seeded Poisson stream of independent
+problems over the serving workload's full ranges (shock tubes nx 200-6400,
+Michel nr 256-4096 with spin in [0, 0.99), Kelvin-Helmholtz 256^2-1024^2),
+continuously batched on the device, with p50/p99 time-to-solution and
+aggregate cell-updates/s ("ensemble").
 
 python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline] [--serve-requests N]
-                [--serve-rates 4,16,64] [--serve-mix quick|full]
+                [--serve-rates 16] [--serve-mix full|quick]
 """
 from __future__ import annotations
 
@@ -91,8 +94,14 @@ def run_gpu(cfg, precision, steps, warmup):
         ev_ms, launches = S.last_timing()
         counters = S.counters()
         ncell = S.n_cells
+        # final state (after the timed steps) for the mixed-vs-FP64 comparison
+        D = S.get_conserved()[..., 0].copy()
+        g = S.ng
+        rho = S.get_primitives()[g + 1:g + 1 + S.nz, g + 1:g + 1 + S.ny, g + 1:g + 1 + S.nx, 0].copy()
     wall = t1 - t0
     return {
+        "D": D,
+        "rho": rho,
         "cell_updates_per_s": ncell * steps / wall,
         "wall_s": wall,
         "event_ms": ev_ms,
@@ -111,7 +120,9 @@ def cpu_child(out_path, nx, ny, nz, seconds):
     os.dup2(devnull, 1)
     import oracle
     cfg = c4_config(nx, ny, nz, open_theta=False)  # the reference requires theta = [0, pi]
+    t_setup = time.perf_counter()
     R = oracle.Reference(cfg)
+    t_setup = time.perf_counter() - t_setup
     p = R.get_prim()
     rng = np.random.default_rng(3435)
     g = R.ng
@@ -130,10 +141,32 @@ def cpu_child(out_path, nx, ny, nz, seconds):
     R.step(n)
     el = time.perf_counter() - t0
     with open(out_path, "w") as fh:
-        json.dump({"steps": n, "seconds": el, "cells": nx * ny * nz}, fh)
+        json.dump({"steps": n, "seconds": el, "cells": nx * ny * nz, "setup_s": t_setup}, fh)
 
 
-def cpu_baseline(seconds=15.0, nx=64, ny=32, nz=32):
+def _host_description():
+    """CPU model and the reference build's compiler (for the baseline record)."""
+    model = "unknown CPU"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            model = next(ln.split(":", 1)[1].strip() for ln in fh if ln.startswith("model name"))
+    except (OSError, StopIteration):
+        pass
+    compiler = "amdflang"
+    try:
+        out = subprocess.run(["amdflang", "--version"], capture_output=True, text=True, timeout=30).stdout
+        compiler = out.splitlines()[0].strip() if out else compiler
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return model, compiler
+
+
+def cpu_baseline(seconds=15.0, nx=256, ny=128, nz=128):
+    """The reference (oracle/_ref: the unmodified Fortran sources compiled by
+    oracle/Makefile with amdflang -O2 -ffp-contract=off -fopenmp) on the
+    torus grid the GPU line measures, theta = [0, pi] (the reference refuses
+    the open theta range), for as many RK3 steps as fit in `seconds` after one
+    timed step, with OpenMP on the host cores this process may use."""
     import oracle
     if not oracle.reference_available():
         return None
@@ -143,17 +176,21 @@ def cpu_baseline(seconds=15.0, nx=64, ny=32, nz=32):
         out = os.path.join(td, "cpu.json")
         cmd = [sys.executable, str(Path(__file__).resolve()), "--cpu-child", out,
                "--cpu-seconds", str(seconds), "--cpu-grid", f"{nx},{ny},{nz}"]
-        subprocess.run(cmd, env=env, check=True, timeout=600, cwd=str(ROOT))
+        subprocess.run(cmd, env=env, check=True, timeout=900, cwd=str(ROOT))
         with open(out) as fh:
             r = json.load(fh)
+    model, compiler = _host_description()
     return {
         "value": r["cells"] * r["steps"] / r["seconds"],
         "unit": "cell-updates/s",
         "cores": cores,
         "kind": "reference",
-        "sample": (f"reference Fortran (amdflang -O2 -fopenmp, OpenMP {cores} threads) on "
-                   f"fishbone_3d KS a=0.9 {nx}x{ny}x{nz}, theta=[0,pi], MP5+HLLC+sensor, "
-                   f"{r['steps']} RK3 steps in {r['seconds']:.1f} s"),
+        "cpu": model,
+        "compiler": compiler,
+        "flags": "-O2 -ffp-contract=off -fopenmp (OMP_PROC_BIND=close)",
+        "sample": (f"reference Fortran, OpenMP {cores} threads, fishbone_3d KS a=0.9 {nx}x{ny}x{nz} "
+                   f"theta=[0,pi], MP5+HLLC+sensor, 1e-2 density noise: {r['steps']} RK3 steps in "
+                   f"{r['seconds']:.1f} s after one untimed step (set-up {r.get('setup_s', 0.0):.1f} s)"),
     }
 
 
@@ -348,14 +385,14 @@ def main():
     ap.add_argument("--precision", choices=["fp64", "mixed", "both"], default="both")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--cpu-grid", default="64,32,32")
+    ap.add_argument("--cpu-grid", default="256,128,128")
     ap.add_argument("--cpu-child", default=None)
     ap.add_argument("--serve-requests", type=int, default=32,
                     help="requests of the served ensemble (0: skip it)")
-    ap.add_argument("--serve-rates", default="4,16,64",
+    ap.add_argument("--serve-rates", default="16",
                     help="Poisson arrival rates (requests/s) of the served ensemble, swept in order; "
                          "the line's 'ensemble' is the one at 16/s (or the first)")
-    ap.add_argument("--serve-mix", choices=sorted(MIXES), default="quick")
+    ap.add_argument("--serve-mix", choices=sorted(MIXES), default="full")
     ap.add_argument("--serve-pacing", type=int, default=None,
                     help="steps of a cheaper batch group per step of the most expensive one (1..8); "
                          "default 8 for the full mix, whose problem sizes span 200 to 1M cells, "
@@ -450,6 +487,14 @@ def main():
             "speedup_vs_fp64": (res["mixed"]["cell_updates_per_s"] / res["fp64"]["cell_updates_per_s"]
                                 if "fp64" in res else None),
         }
+        if "fp64" in res:
+            import numpy as np
+            # both runs start from the same seeded state and take the same
+            # warmup + timed steps at the same dt
+            for key, name in (("D", "D_L1_vs_fp64"), ("rho", "rho_L1_vs_fp64")):
+                a64, amx = res["fp64"][key], res["mixed"][key]
+                line["mixed"][name] = float(np.abs(amx - a64).sum() / np.abs(a64).sum())
+            line["mixed"]["compared_after_steps"] = a.warmup + a.steps
     if a.serve_requests > 0:
         rates = [float(r) for r in a.serve_rates.split(",") if r]
         sweep = [run_ensemble(a.serve_requests, r, a.serve_time_scale, a.serve_chunk, seed=a.serve_seed,
