@@ -327,23 +327,36 @@ void coord_entries(const Params &P, Put put)
 
 // Face-kernel tiling of the problem, chosen once per context by timing: for
 // each active direction, the face kernel alone with 4, 2, 8 and 1 waves sharing
-// one LDS copy of the face geometry (one warm-up launch, then three timed),
-// on the problem's own primitives.  The face kernels write only the face
-// arrays, which the next stage rewrites, and every tiling gives the same
+// one LDS copy of the face geometry, and with the geometry evaluated by its own
+// kernel into the stage's table (one warm-up launch, then three timed), on the
+// problem's own primitives.  The face kernels write only the face arrays (and
+// the table), which the next stage rewrites, and every form gives the same
 // fluxes, so tuning changes no result.  Grids in the fused stage form keep
 // groups of four.
+// GRHD_TRACE_TUNE=1 prints the tuner's timings to stderr.
+bool trace_tune()
+{
+    static const bool on = [] {
+        const char *e = std::getenv("GRHD_TRACE_TUNE");
+        return e && e[0] == '1';
+    }();
+    return on;
+}
+
 int tune_faces(grhd_ctx *c)
 {
     c->faces_tuned = true;
     if (single_stage_fused(c->P)) return GRHD_OK;
-    static const int cand[] = {4, 2, 8, 1};
+    static const int cand[] = {4, 2, 8, 1, 0};
     for (int d = 0; d < 3; d++) {
         if (!(d == 0 || (d == 1 && c->P.ny > 1) || (d == 2 && c->P.nz > 1))) continue;
         float best = 0.0f;
         int best_gw = 4;
         for (int gw : cand) {
+            if (gw == 0 && !c->geo[d]) continue;
             Params P = c->P;
             P.face_gw[d] = gw;
+            P.geo_tab[d] = (gw == 0) ? c->geo[d] : nullptr;
             for (int rep = 0; rep < 4; rep++) {
                 if (rep == 1) HIPCHK(c, hipEventRecord(c->ev0, c->stream));
                 if (c->mixed) launch_face_dir_f32(P, c->stream, c->p, c->F, c->fstr, d + 1);
@@ -353,12 +366,18 @@ int tune_faces(grhd_ctx *c)
             HIPCHK(c, hipEventSynchronize(c->ev1));
             float ms = 0.0f;
             HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+            if (trace_tune()) {
+                std::fprintf(stderr, "grhd tune: direction %d, %s %d: %.4f ms per launch\n", d + 1,
+                             gw ? "waves per geometry group" : "geometry table", gw, ms / 3.0);
+                std::fflush(stderr);
+            }
             if (best == 0.0f || ms < best) {
                 best = ms;
                 best_gw = gw;
             }
         }
         c->P.face_gw[d] = best_gw;
+        c->P.geo_tab[d] = (best_gw == 0) ? c->geo[d] : nullptr;
     }
     HIPCHK(c, hipGetLastError());
     return GRHD_OK;
@@ -410,6 +429,10 @@ uint64_t ctx_footprint(const grhd_config &cfg)
                             P.nz > 1 ? (uint64_t)P.nx * P.ny * (P.nz + 1) : 1};
     uint64_t b = (uint64_t)P.vstride * NEQ * sizeof(double) + 2 * ncell * NEQ * sizeof(double);
     for (int d = 0; d < 3; d++) b += nf[d] * (d == 0 || nf[d] > 1 ? NEQ : 1) * rsz;
+    if (!single_stage_fused(P))  // face-geometry tables
+        for (int d = 0; d < 3; d++)
+            if (d == 0 || (d == 1 && P.ny > 1) || (d == 2 && P.nz > 1))
+                b += (uint64_t)NFG * (P.nx + (d == 0)) * (P.ny + (d == 1)) * rsz;
     b += (uint64_t)NEQ * P.ng * P.ny * P.nz * sizeof(double) + 8;
     b += (uint64_t)std::max<int64_t>((((int64_t)P.nx * P.ny + 255) / 256) * 4, std::max<int64_t>(4096, P.nz)) *
              sizeof(double) +
@@ -492,6 +515,14 @@ int grhd_create(const grhd_config *cfg, grhd_ctx **out)
         const size_t n = active ? (size_t)c->fstr[d] * NEQ : 1;
         if ((e = hipMalloc(&c->F[d], n * rsz)) != hipSuccess) return bail(e, "hipMalloc flux");
     }
+    if (!single_stage_fused(P)) {
+        for (int d = 0; d < 3; d++) {
+            const bool active = (d == 0) || (d == 1 && P.ny > 1) || (d == 2 && P.nz > 1);
+            if (!active) continue;
+            const size_t nab = (size_t)(P.nx + (d == 0)) * (P.ny + (d == 1));
+            if ((e = hipMalloc(&c->geo[d], (size_t)NFG * nab * rsz)) != hipSuccess) return bail(e, "hipMalloc geometry");
+        }
+    }
     if ((e = hipMalloc(&c->injector, (size_t)NEQ * P.ng * P.ny * P.nz * sizeof(double) + 8)) != hipSuccess)
         return bail(e, "hipMalloc injector");
     c->partial_n = (int)(((int64_t)P.nx * P.ny + 255) / 256);
@@ -529,7 +560,10 @@ int grhd_destroy(grhd_ctx *c)
     (void)hipFree(c->p);
     (void)hipFree(c->u[0]);
     (void)hipFree(c->u[1]);
-    for (int d = 0; d < 3; d++) (void)hipFree(c->F[d]);
+    for (int d = 0; d < 3; d++) {
+        (void)hipFree(c->F[d]);
+        (void)hipFree(c->geo[d]);
+    }
     (void)hipFree(c->injector);
     (void)hipFree(c->partial);
     (void)hipFree(c->counters);

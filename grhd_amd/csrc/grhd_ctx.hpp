@@ -19,6 +19,7 @@ struct grhd_ctx {
     double *u[2] = {nullptr, nullptr};  // conserved, SoA [5][nz][ny][nx]
     int ucur = 0;             // u[ucur] holds U^n
     void *F[3] = {nullptr, nullptr, nullptr};
+    void *geo[3] = {nullptr, nullptr, nullptr};  // per-stage face-geometry tables (tuner's table form)
     int64_t fstr[3] = {0, 0, 0};
     double *injector = nullptr;
     double *partial = nullptr;

@@ -47,9 +47,12 @@ struct Params {
     // outflow ghost fill there instead of the trans-polar one
     int open_lo, open_hi;
     // waves of a face kernel sharing one LDS copy of the face geometry, per
-    // direction (1, 2, 4 or 8): the single-problem launchers' tiling, chosen
-    // by timing the face kernels on the problem (tune_faces)
+    // direction (1, 2, 4 or 8), or 0: the geometry evaluated once per stage
+    // into geo_tab[d] by its own kernel (no barrier in the face kernel).  The
+    // single-problem launchers' tiling, chosen by timing the face kernels on
+    // the problem (tune_faces).
     int face_gw[3];
+    void *geo_tab[3];
     double gam, g1, mass, spin;
     double rho_floor, p_floor;
     double x_min, dx, y_min, dy, z_min, dz;
@@ -567,8 +570,9 @@ GRHD_HD constexpr int sym_index(int a, int b)
 }
 
 template <typename R> struct FaceGeo {
-    const R *col;  // LDS column base: field f of this lane at col[f * 64]
-    GRHD_HD R get(int f) const { return col[f * 64]; }
+    const R *col;          // field f of this face column at col[f * stride]:
+    int64_t stride = 64;   // an LDS column (64 lanes) or a per-stage table
+    GRHD_HD R get(int f) const { return col[f * stride]; }
     GRHD_HD R alpha() const { return get(FG_ALPHA); }
     GRHD_HD R beta(int d) const { return get(FG_BETA + d); }
     GRHD_HD R gam(int a, int b) const { return get(FG_G + sym_index(a, b)); }

@@ -33,7 +33,7 @@ __device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned nb)
 // (fluxes.f90:195-205, 402-473), and the zero-flux faces on the cylindrical
 // axis and at the poles (evolution.f90:69-71, 122-124).
 template <int MK, int DIR, typename R>
-__device__ __forceinline__ void face_geometry(const Params &P, int i, int j, R *col)
+__device__ __forceinline__ void face_geometry(const Params &P, int i, int j, R *col, int64_t stride = 64)
 {
     const double eps64 = 64.0 * DBL_EPSILON;
     bool zero = false;
@@ -63,24 +63,25 @@ __device__ __forceinline__ void face_geometry(const Params &P, int i, int j, R *
                         face_frame<R, DIR>(g, cof, gdd);
         const R w = ok ? rdiv(beta[DIR - 1], alpha * sqrt(gdd)) : (R)0;
         if (ok) flags |= FG_FRAME_OK;
-        col[FG_ALPHA * 64] = alpha;
-        for (int x = 0; x < 3; x++) col[(FG_BETA + x) * 64] = beta[x];
+        col[FG_ALPHA * stride] = alpha;
+        for (int x = 0; x < 3; x++) col[(FG_BETA + x) * stride] = beta[x];
         for (int x = 0; x < 3; x++)
-            for (int y = x; y < 3; y++) col[(FG_G + sym_index(x, y)) * 64] = g[x][y];
-        col[FG_SQG * 64] = sqg;
+            for (int y = x; y < 3; y++) col[(FG_G + sym_index(x, y)) * stride] = g[x][y];
+        col[FG_SQG * stride] = sqg;
         for (int x = 0; x < 3; x++)
-            for (int y = 0; y < 3; y++) col[(FG_COF + 3 * x + y) * 64] = ok ? cof[x][y] : (R)0;
-        col[FG_GDD * 64] = gdd;
-        col[FG_W * 64] = w;
-        col[FG_GII * 64] = gii;
+            for (int y = 0; y < 3; y++) col[(FG_COF + 3 * x + y) * stride] = ok ? cof[x][y] : (R)0;
+        col[FG_GDD * stride] = gdd;
+        col[FG_W * stride] = w;
+        col[FG_GII * stride] = gii;
     }
-    col[FG_FLAGS * 64] = (R)flags;
+    col[FG_FLAGS * stride] = (R)flags;
 }
 
+// Face states q_L, q_R of face fi along DIR (cells fi-2 .. fi+3 of row
+// (i, j, k)): stencil loads and reconstruct_1d_core.
 template <int MK, int DIR, typename R, int SCH = 0>
-__device__ __forceinline__ void flux_face(const Params &P, const double *__restrict__ p,
-                                          const FaceGeo<R> &f, int i, int j, int k, int fi, int nodes,
-                                          R out[NEQ])
+__device__ __forceinline__ void face_states(const Params &P, const double *__restrict__ p, int i, int j, int k,
+                                            int fi, int nodes, R qL[NEQ], R qR[NEQ])
 {
     R s[NEQ][6];
     const int ng = SCH ? ghost_depth(kSchemes[SCH].rec) : P.ng;
@@ -99,9 +100,36 @@ __device__ __forceinline__ void flux_face(const Params &P, const double *__restr
             s[v][o + 2] = in ? x : (R)0;
         }
     }
-    R qL[NEQ], qR[NEQ];
     recon_face<R, SCH>(P, s, fi, nodes, DIR, qL, qR);
-    riemann<R, DIR, kSRHD<MK>>(SCH ? kSchemes[SCH].rs : P.rs, (R)P.g1, (R)P.gam, qL, qR, f, out);
+}
+
+// Flux of one face (reconstruction + resolve_riemann_problem, fluxes.f90:56-72).
+// FP64 faces with HLLC: the HLLE fallback (19 of 1.6 million faces on the 3D
+// torus) recomputes the face states instead of keeping them live through the
+// HLLC solve, which frees their registers there (the FP64 face kernels then
+// spill nothing at 128 VGPRs; +7 % on the 3D torus); the pointer is made
+// opaque so that the recomputation is not merged with the first one.  Both
+// are the same pure function of p: the fallback sees the same states.  The
+// FP32 face kernels measured slower with it and keep the states.
+template <int MK, int DIR, typename R, int SCH = 0, typename FG>
+__device__ __forceinline__ void flux_face(const Params &P, const double *__restrict__ p,
+                                          const FG &f, int i, int j, int k, int fi, int nodes,
+                                          R out[NEQ])
+{
+    R qL[NEQ], qR[NEQ];
+    face_states<MK, DIR, R, SCH>(P, p, i, j, k, fi, nodes, qL, qR);
+    const int rs = SCH ? kSchemes[SCH].rs : P.rs;
+    if constexpr (sizeof(R) != sizeof(double)) {
+        riemann<R, DIR, kSRHD<MK>>(rs, (R)P.g1, (R)P.gam, qL, qR, f, out);
+        return;
+    }
+    if (rs == 2) {
+        if (hllc_try<R, DIR>((R)P.g1, (R)P.gam, qL, qR, f, out)) return;
+        const double *pr = p;
+        asm volatile("" : "+v"(pr));
+        face_states<MK, DIR, R, SCH>(P, pr, i, j, k, fi, nodes, qL, qR);
+    }
+    hlle<R, DIR, kSRHD<MK>>((R)P.g1, (R)P.gam, qL, qR, f, out);
 }
 
 // Tiling of the face kernels: a tile is 64 consecutive faces of one (x, y)
@@ -192,11 +220,69 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
                                reinterpret_cast<R *>(lds_geo));
 }
 
+// Face geometry of every face column of direction DIR into the table tab
+// (field f of column ab at tab[f * nab + ab]), once per stage.
+template <int MK, int DIR, typename R>
+__global__ __launch_bounds__(256) void k_geo(Params P, R *__restrict__ tab)
+{
+    const int nfx = P.nx + (DIR == 1), nfy = P.ny + (DIR == 2);
+    const int64_t nab = (int64_t)nfx * nfy;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nab) return;
+    const int a = (int)(t % nfx), b = (int)(t / nfx);
+    face_geometry<MK, DIR, R>(P, a + (DIR != 1), b + (DIR != 2), tab + t, nab);
+}
+
+// Faces of direction DIR reading their geometry from the stage's table: the
+// blocks, lanes and planes of flux_body with groups of four, with neither the
+// geometry evaluation nor the barrier.
+template <int MK, int DIR, typename R, int SCH>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_flux_tab(
+    Params P, const double *__restrict__ p, R *__restrict__ F, int64_t fstride, const R *__restrict__ tab)
+{
+    const int nfx = P.nx + (DIR == 1), nfy = P.ny + (DIR == 2), nfz = P.nz + (DIR == 3);
+    const int64_t nab = (int64_t)nfx * nfy;
+    const int64_t nchunk = (nab + 63) / 64;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int nct = (nfz + 3) / 4;
+    const int64_t blk = xcd_block(blockIdx.x, gridDim.x);
+    const int64_t cg = (DIR == 3) ? blk / nct : blk % nchunk;
+    const int ctile = (DIR == 3) ? (int)(blk % nct) : (int)(blk / nchunk);
+    const int64_t ab = cg * 64 + lane;
+    const int c = ctile * 4 + wave;
+    if (ab >= nab || c >= nfz) return;
+    const int a = (int)(ab % nfx), b = (int)(ab / nfx);
+    const int i = a + (DIR != 1), j = b + (DIR != 2);
+    const FaceGeo<R> fg{tab + ab, nab};
+    const int nodes = (DIR == 1) ? P.nx : (DIR == 2) ? P.ny : P.nz;
+    const int k = c + (DIR != 3);
+    const int fi = (DIR == 1) ? i : (DIR == 2) ? j : k;
+    R out[NEQ];
+    if (fg.flags() & FG_ZERO) {
+#pragma unroll
+        for (int v = 0; v < NEQ; v++) out[v] = 0;
+    } else {
+        flux_face<MK, DIR, R, SCH>(P, p, fg, i, j, k, fi, nodes, out);
+    }
+    const int64_t fo = (int64_t)c * nab + ab;
+#pragma unroll
+    for (int v = 0; v < NEQ; v++) F[v * fstride + fo] = out[v];
+}
+
 template <int MK, int DIR, typename R, int SCH>
 void launch_flux(const Params &P, hipStream_t s, const double *p, R *F, int64_t fstr)
 {
+    if (P.face_gw[DIR - 1] == 0 && P.geo_tab[DIR - 1]) {
+        R *tab = static_cast<R *>(P.geo_tab[DIR - 1]);
+        const int nfx = P.nx + (DIR == 1), nfy = P.ny + (DIR == 2), nfz = P.nz + (DIR == 3);
+        const int64_t nab = (int64_t)nfx * nfy;
+        k_geo<MK, DIR, R><<<blocks_for(nab, 256), 256, 0, s>>>(P, tab);
+        const int64_t nb = ((nab + 63) / 64) * ((nfz + 3) / 4);
+        k_flux_tab<MK, DIR, R, SCH><<<(unsigned)nb, 256, 0, s>>>(P, p, F, fstr, tab);
+        return;
+    }
     const int nfz = P.nz + (DIR == 3);
-    const FluxTiling t = flux_tiling(nfz, P.face_gw[DIR - 1]);
+    const FluxTiling t = flux_tiling(nfz, P.face_gw[DIR - 1] >= 1 ? P.face_gw[DIR - 1] : 4);
     const int waves = t.gw > 4 ? t.gw : 4;
     const size_t lds = (size_t)(waves / t.gw) * NFG * 64 * sizeof(R);
     k_flux<MK, DIR, R, SCH><<<(unsigned)flux_blocks<DIR>(P, t, waves), 64 * waves, lds, s>>>(P, p, F, fstr, t.kt,
@@ -454,7 +540,9 @@ void launch_stage_mk(const Params &P, hipStream_t s, double *p, void *const F[3]
     int groups = 1;
     for (int d = 0; d < 3; d++) {
         const bool on = (d == 0) || (d == 1 && P.ny > 1) || (d == 2 && P.nz > 1);
-        const FluxTiling t = flux_tiling(P.nz + (d == 2), P.face_gw[d] > 4 ? 4 : P.face_gw[d]);  // 256-thread blocks
+        // the fused kernel's tiling: 256-thread blocks, LDS geometry
+        const int gw = (P.face_gw[d] >= 1 && P.face_gw[d] <= 4) ? P.face_gw[d] : 4;
+        const FluxTiling t = flux_tiling(P.nz + (d == 2), gw);
         f.kt[d] = t.kt;
         f.gw[d] = t.gw;
         f.nb[d] = !on ? 0 : (d == 0) ? flux_blocks<1>(P, t) : (d == 1) ? flux_blocks<2>(P, t) : flux_blocks<3>(P, t);
