@@ -297,7 +297,7 @@ riemann_solver = hlle
 """
 
 
-def equatorial_torus(steps=1000):
+def equatorial_torus(steps=1000, batch_forms=True):
     """C3: Fishbone-Moncrief equatorial torus, Kerr-Schild a = 0.9, log-r
     spheroidal grid 400 x 1 x 200 (the reference's preset grid), r_in = 6,
     pressure maximum at r = 12, K for rho_max = 1, 1e-2 seeded pressure noise
@@ -337,7 +337,7 @@ def equatorial_torus(steps=1000):
         # each launch form (capture included in the time); the state must equal
         # the direct run's bit for bit
         forms = {}
-        for v in range(1, 6):
+        for v in (range(1, 6) if batch_forms else ()):
             with Batch(capacity=1) as B:
                 B.set_variant(v)
                 slot = B.add(cfg)
@@ -350,10 +350,11 @@ def equatorial_torus(steps=1000):
                 wall_b = time.perf_counter() - t0
                 assert taken == steps and np.array_equal(view.get_conserved(), u_direct), (prec, v)
             forms[v] = ncell * steps / wall_b
-        best = max(forms, key=forms.get)
-        out[prec]["batched_cell_updates_per_s"] = forms[best]
-        out[prec]["batched_launch_form"] = best
-        out[prec]["batched_by_form"] = {str(k): round(x / 1e6, 1) for k, x in forms.items()}
+        if forms:
+            best = max(forms, key=forms.get)
+            out[prec]["batched_cell_updates_per_s"] = forms[best]
+            out[prec]["batched_launch_form"] = best
+            out[prec]["batched_by_form"] = {str(k): round(x / 1e6, 1) for k, x in forms.items()}
     a, b = rho["mixed"], rho["fp64"]
     rec = {"workload": "fishbone_equatorial Kerr-Schild a=0.9 log-r 400x1x200, r_in=6, r_pmax=12, "
                        "K for rho_max=1, 1e-2 pressure noise, WENO5+HLLE",
@@ -407,6 +408,8 @@ def main():
                     help="seed of the request stream (arrivals and request draws)")
     ap.add_argument("--c3", action="store_true",
                     help="only the equatorial torus: 1000 RK3 steps in FP64 and mixed precision")
+    ap.add_argument("--c3-direct", action="store_true",
+                    help="with --c3: the direct path only (no batch-of-one launch forms)")
     ap.add_argument("--michel", action="store_true",
                     help="only the Michel C2 convergence study (nr 256/1024/4096, FP64 and mixed)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -419,7 +422,7 @@ def main():
         michel_convergence()
         return
     if a.c3:
-        equatorial_torus()
+        equatorial_torus(batch_forms=not a.c3_direct)
         return
     cnx, cny, cnz = (int(v) for v in a.cpu_grid.split(","))
     if a.cpu_child:

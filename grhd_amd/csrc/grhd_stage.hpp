@@ -104,13 +104,15 @@ __device__ __forceinline__ void face_states(const Params &P, const double *__res
 }
 
 // Flux of one face (reconstruction + resolve_riemann_problem, fluxes.f90:56-72).
-// FP64 faces with HLLC: the HLLE fallback (19 of 1.6 million faces on the 3D
-// torus) recomputes the face states instead of keeping them live through the
-// HLLC solve, which frees their registers there (the FP64 face kernels then
-// spill nothing at 128 VGPRs; +7 % on the 3D torus); the pointer is made
-// opaque so that the recomputation is not merged with the first one.  Both
-// are the same pure function of p: the fallback sees the same states.  The
-// FP32 face kernels measured slower with it and keep the states.
+// FP64 faces of a compiled scheme with HLLC: the HLLE fallback (19 of 1.6
+// million faces on the 3D torus) recomputes the face states instead of
+// keeping them live through the HLLC solve, which frees their registers there
+// (those kernels then spill nothing at 128 VGPRs; +7 % on the 3D torus); the
+// pointer is made opaque so that the recomputation is not merged with the
+// first one.  Both are the same pure function of p: the fallback sees the
+// same states.  The FP32 face kernels measured slower with it, and the
+// run-time scheme (every reconstruction inlined twice) spills more with it
+// (48 -> 224 B per lane): both keep the states.
 template <int MK, int DIR, typename R, int SCH = 0, typename FG>
 __device__ __forceinline__ void flux_face(const Params &P, const double *__restrict__ p,
                                           const FG &f, int i, int j, int k, int fi, int nodes,
@@ -119,7 +121,7 @@ __device__ __forceinline__ void flux_face(const Params &P, const double *__restr
     R qL[NEQ], qR[NEQ];
     face_states<MK, DIR, R, SCH>(P, p, i, j, k, fi, nodes, qL, qR);
     const int rs = SCH ? kSchemes[SCH].rs : P.rs;
-    if constexpr (sizeof(R) != sizeof(double)) {
+    if constexpr (sizeof(R) != sizeof(double) || SCH == 0) {
         riemann<R, DIR, kSRHD<MK>>(rs, (R)P.g1, (R)P.gam, qL, qR, f, out);
         return;
     }
