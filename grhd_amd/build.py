@@ -14,8 +14,10 @@ PKG = Path(__file__).resolve().parent
 ROOT = PKG.parent
 CSRC = PKG / "csrc"
 LIB = PKG / "libgrhd_hip.so"
-SOURCES = ["grhd_batch_f64_a.hip", "grhd_batch_f64_b.hip", "grhd_batch_mixed_a.hip", "grhd_batch_mixed_b.hip",
-           "grhd_kernels.hip", "grhd_kernels_mixed.hip", "grhd_initial.hip", "grhd_capi.hip", "grhd_batch.hip"]
+KINDS = ["kslog", "ks", "ef", "mink"]
+SOURCES = ([f"grhd_stage_f64_{k}.hip" for k in KINDS] + [f"grhd_stage_f32_{k}.hip" for k in KINDS] +
+           ["grhd_batch_f64_a.hip", "grhd_batch_f64_b.hip", "grhd_batch_mixed_a.hip", "grhd_batch_mixed_b.hip",
+            "grhd_kernels.hip", "grhd_stage_dispatch.hip", "grhd_initial.hip", "grhd_capi.hip", "grhd_batch.hip"])
 # The mixed-precision stage is held to an error bound against the FP64 path, so
 # it may contract multiply-adds; every other unit keeps the reference's
 # rounding sequence.  Contraction is "on" (within one source expression), not
@@ -26,8 +28,8 @@ SOURCES = ["grhd_batch_f64_a.hip", "grhd_batch_f64_b.hip", "grhd_batch_mixed_a.h
 # instructions (1 ulp) instead of the correctly rounded sequences; FP64
 # arithmetic in these units (metric, sources, con2prim, RK) is unaffected.
 MIXED_FLAGS = ["-ffp-contract=on", "-fno-hip-fp32-correctly-rounded-divide-sqrt"]
-EXTRA = {"grhd_kernels_mixed.hip": MIXED_FLAGS, "grhd_batch_mixed_a.hip": MIXED_FLAGS,
-         "grhd_batch_mixed_b.hip": MIXED_FLAGS}
+EXTRA = {"grhd_batch_mixed_a.hip": MIXED_FLAGS, "grhd_batch_mixed_b.hip": MIXED_FLAGS,
+         **{f"grhd_stage_f32_{k}.hip": MIXED_FLAGS for k in KINDS}}
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("GRHD_OFFLOAD_ARCH", "gfx950")

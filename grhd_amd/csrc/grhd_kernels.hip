@@ -4,9 +4,9 @@
 // (flux divergence, sources, RK combination, con2prim) + one boundary kernel.
 // Kernels are templated on the metric kind so each launch evaluates its
 // analytic metric with compile-time structure; reconstruction and the Riemann
-// solver are uniform runtime branches.  This unit instantiates the FP64 stage
-// (compiled without FMA contraction) and the boundary, conversion and CFL
-// kernels.
+// solver are uniform runtime branches.  This unit holds the boundary,
+// conversion, CFL and observer kernels; the stage kernels are instantiated per
+// metric kind by grhd_stage_{f64,f32}_*.hip.
 #include <algorithm>
 
 #include "grhd_stage.hpp"
@@ -280,19 +280,6 @@ void launch_boundaries_mk(const Params &P, hipStream_t s, double *p, const doubl
     case MK_KS: { constexpr int MKC = MK_KS; CALL; } break;                     \
     default: { constexpr int MKC = MK_KS_LOG; CALL; } break;                    \
     }
-
-void launch_stage_f64(const Params &P, hipStream_t s, double *p, void *const F[3],
-                      const int64_t fstr[3], const double *u0, double *u, int stage, double dt,
-                      unsigned long long *counters, bool fused)
-{
-    GRHD_STAGE_DISPATCH(double)
-}
-
-void launch_face_dir_f64(const Params &P, hipStream_t s, const double *p, void *const F[3], const int64_t fstr[3],
-                         int dir)
-{
-    GRHD_FACE_DIR_DISPATCH(double)
-}
 
 void launch_stage(const Params &P, hipStream_t s, double *p, void *const F[3], const int64_t fstr[3],
                   const double *u0, double *u, int stage, double dt, unsigned long long *counters,

@@ -7,6 +7,17 @@ namespace grhd {
 template <int MK>
 void launch_boundaries_mk(const Params &P, hipStream_t s, double *p, const double *injector);
 
+// The stage launcher and the single-direction face launcher of one metric kind
+// and face arithmetic R (double: FP64 faces; float: FP32 faces), defined in
+// grhd_stage.hpp and instantiated per kind by the grhd_stage_*.hip units.
+template <int MK, typename R>
+void launch_stage_mk(const Params &P, hipStream_t s, double *p, void *const F[3], const int64_t fstr[3],
+                     const double *u0, double *u, int stage, double dt, unsigned long long *counters,
+                     bool fused);
+template <int MK, typename R>
+void launch_face_dir_mk(const Params &P, hipStream_t s, const double *p, void *const F[3], const int64_t fstr[3],
+                        int dir);
+
 // The stage kernels (face fluxes + cell update) with FP64 or FP32 faces.
 void launch_stage_f64(const Params &P, hipStream_t s, double *p, void *const F[3],
                       const int64_t fstr[3], const double *u0, double *u, int stage, double dt,

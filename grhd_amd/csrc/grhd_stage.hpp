@@ -1,7 +1,7 @@
 // grhd_stage.hpp -- templated kernels of one RK stage (face fluxes and the
-// cell update) and their launcher.  Included by the two translation units that
-// instantiate them: grhd_kernels.hip (FP64 faces, reference rounding sequence)
-// and grhd_kernels_mixed.hip (FP32 faces).
+// cell update) and their launcher.  Instantiated per metric kind by
+// grhd_stage_f64_*.hip (FP64 faces, the reference's rounding sequence) and
+// grhd_stage_f32_*.hip (FP32 faces).
 #pragma once
 #include <algorithm>
 #include "grhd_kernels.hpp"
@@ -335,18 +335,6 @@ void launch_face_dir_mk(const Params &P, hipStream_t s, const double *p, void *c
 #undef GRHD_FACE_DIR
 }
 
-#define GRHD_FACE_DIR_DISPATCH(R)                                                                   \
-    switch (P.mk) {                                                                                 \
-    case MK_MINK_CART: launch_face_dir_mk<MK_MINK_CART, R>(P, s, p, F, fstr, dir); break;          \
-    case MK_MINK_CYL: launch_face_dir_mk<MK_MINK_CYL, R>(P, s, p, F, fstr, dir); break;            \
-    case MK_MINK_SPH: launch_face_dir_mk<MK_MINK_SPH, R>(P, s, p, F, fstr, dir); break;            \
-    case MK_MINK_SPH_LOG: launch_face_dir_mk<MK_MINK_SPH_LOG, R>(P, s, p, F, fstr, dir); break;    \
-    case MK_EF: launch_face_dir_mk<MK_EF, R>(P, s, p, F, fstr, dir); break;                        \
-    case MK_EF_LOG: launch_face_dir_mk<MK_EF_LOG, R>(P, s, p, F, fstr, dir); break;                \
-    case MK_KS: launch_face_dir_mk<MK_KS, R>(P, s, p, F, fstr, dir); break;                        \
-    default: launch_face_dir_mk<MK_KS_LOG, R>(P, s, p, F, fstr, dir); break;                       \
-    }
-
 // --------------------------------------------------------- cell update --
 // Cell t (0 <= t < nx ny nz, x fastest) of one problem.  stage 1..3: rhs
 // (evolution.f90:85-88, 137-139, 180-182) and the RK combination
@@ -585,17 +573,14 @@ void launch_stage_mk(const Params &P, hipStream_t s, double *p, void *const F[3]
                                                      stage, dt);
 }
 
-// Instantiates the stage launcher of one face arithmetic for every metric kind.
-#define GRHD_STAGE_DISPATCH(R)                                                                    \
-    switch (P.mk) {                                                                               \
-    case MK_MINK_CART: launch_stage_mk<MK_MINK_CART, R>(P, s, p, F, fstr, u0, u, stage, dt, counters, fused); break; \
-    case MK_MINK_CYL: launch_stage_mk<MK_MINK_CYL, R>(P, s, p, F, fstr, u0, u, stage, dt, counters, fused); break;   \
-    case MK_MINK_SPH: launch_stage_mk<MK_MINK_SPH, R>(P, s, p, F, fstr, u0, u, stage, dt, counters, fused); break;   \
-    case MK_MINK_SPH_LOG: launch_stage_mk<MK_MINK_SPH_LOG, R>(P, s, p, F, fstr, u0, u, stage, dt, counters, fused); break; \
-    case MK_EF: launch_stage_mk<MK_EF, R>(P, s, p, F, fstr, u0, u, stage, dt, counters, fused); break;               \
-    case MK_EF_LOG: launch_stage_mk<MK_EF_LOG, R>(P, s, p, F, fstr, u0, u, stage, dt, counters, fused); break;       \
-    case MK_KS: launch_stage_mk<MK_KS, R>(P, s, p, F, fstr, u0, u, stage, dt, counters, fused); break;               \
-    default: launch_stage_mk<MK_KS_LOG, R>(P, s, p, F, fstr, u0, u, stage, dt, counters, fused); break;              \
-    }
+// The stage and face launchers of one metric kind and face arithmetic,
+// instantiated by the translation unit of that kind (grhd_stage_*.hip), so
+// that the kinds compile in parallel; grhd_stage_dispatch.hip selects them.
+#define GRHD_STAGE_UNIT(MKV, R)                                                                          \
+    template void launch_stage_mk<MKV, R>(const Params &, hipStream_t, double *, void *const[3],        \
+                                          const int64_t[3], const double *, double *, int, double,      \
+                                          unsigned long long *, bool);                                  \
+    template void launch_face_dir_mk<MKV, R>(const Params &, hipStream_t, const double *, void *const[3], \
+                                             const int64_t[3], int);
 
 }  // namespace grhd
