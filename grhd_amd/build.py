@@ -46,6 +46,8 @@ FLAGS = [
     # packed FP32 instructions are kept out of generated code
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
+    # validity flags of the flat HLLC solver are combined with & on purpose
+    "-Wno-bitwise-instead-of-logical",
 ]
 
 

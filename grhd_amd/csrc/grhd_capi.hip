@@ -343,7 +343,9 @@ bool trace_tune()
     return on;
 }
 
-int tune_faces(grhd_ctx *c)
+int tune_stage_form(grhd_ctx *c, double dt);
+
+int tune_faces(grhd_ctx *c, double dt)
 {
     c->faces_tuned = true;
     if (single_stage_fused(c->P)) return GRHD_OK;
@@ -380,6 +382,84 @@ int tune_faces(grhd_ctx *c)
         c->P.geo_tab[d] = (best_gw == 0) ? c->geo[d] : nullptr;
     }
     HIPCHK(c, hipGetLastError());
+    const int rc = tune_stage_form(c, dt);
+    if (rc != GRHD_OK) return rc;
+    // every active direction timed its table form, which filled its table
+    c->P.geo_ready = 1;
+    return GRHD_OK;
+}
+
+// Stage form of a 3D problem, chosen once per context after the face tiling
+// by timing one stage (stage 1, the step's dt) of each candidate: the split
+// form (faces of every direction into their arrays, then k_cell) and the
+// z-march form with 16-cell segments (8-cell segments measured the same on
+// the 3D torus, 32-cell ones 25 % slower: too few waves).  Every form gives
+// the same results bit for bit.  The timing runs write only the stage buffer u1, the
+// face arrays and tables, the second primitive array and the tuner's
+// counters: the split form runs on a copy of the primitives in p_alt.
+bool zmarch_possible(const Params &P) { return !single_stage_fused(P) && P.ny > 1 && P.nz > 1; }
+
+int tune_stage_form(grhd_ctx *c, double dt)
+{
+    const Params &P0 = c->P;
+    c->P.zseg = 0;
+    if (!zmarch_possible(P0) || !c->geo[2] || (c->forced_zseg < 0 && P0.nz < 16)) return GRHD_OK;
+    const size_t pbytes = (size_t)P0.vstride * NEQ * sizeof(double);
+    if (c->forced_zseg == 0) {
+        c->P.zseg = 0;
+        return GRHD_OK;
+    }
+    if (!c->p_alt) {
+        const hipError_t e = hipMalloc(&c->p_alt, pbytes);
+        if (e != hipSuccess) {  // no room for a second primitive array: split form
+            (void)hipGetLastError();
+            c->p_alt = nullptr;
+            if (c->forced_zseg > 0) return fail(c, GRHD_ERR_OUT_OF_MEMORY, "second primitive array");
+            return GRHD_OK;
+        }
+    }
+    if (c->forced_zseg > 0) {
+        c->P.zseg = c->forced_zseg;
+        return GRHD_OK;
+    }
+    double *u0 = c->u[c->ucur], *u1 = c->u[1 - c->ucur];
+    unsigned long long *cnt = c->counters + 4;
+    static const int cand[] = {0, 16};
+    float best = 0.0f;
+    int best_seg = 0;
+    for (int seg : cand) {
+        if (seg > P0.nz) continue;
+        Params P = P0;
+        P.zseg = seg;
+        if (seg == 0) HIPCHK(c, hipMemcpyAsync(c->p_alt, c->p, pbytes, hipMemcpyDeviceToDevice, c->stream));
+        for (int rep = 0; rep < 3; rep++) {
+            if (rep == 1) HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+            if (seg == 0) launch_stage(P, c->stream, c->p_alt, c->F, c->fstr, u0, u1, 1, dt, cnt, c->mixed,
+                                       c->injector, false);
+            else launch_stage_z(P, c->stream, c->p, c->p_alt, c->F, c->fstr, c->geo[2], u0, u1, 1, dt, cnt,
+                                c->mixed);
+        }
+        HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+        HIPCHK(c, hipEventSynchronize(c->ev1));
+        float ms = 0.0f;
+        HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        if (trace_tune()) {
+            std::fprintf(stderr, "grhd tune: stage form %s %d: %.4f ms per stage\n", seg ? "z-march segment" : "split",
+                         seg, ms / 2.0);
+            std::fflush(stderr);
+        }
+        if (best == 0.0f || ms < best) {
+            best = ms;
+            best_seg = seg;
+        }
+    }
+    HIPCHK(c, hipGetLastError());
+    c->P.zseg = best_seg;
+    if (best_seg == 0) {
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        (void)hipFree(c->p_alt);
+        c->p_alt = nullptr;
+    }
     return GRHD_OK;
 }
 
@@ -387,22 +467,47 @@ int run_steps(grhd_ctx *c, const double *dts, int n)
 {
     if (!c->faces_tuned && n > 0) {
         launch_boundaries(c->P, c->stream, c->p, c->injector);
-        const int rc = tune_faces(c);
+        const int rc = tune_faces(c, dts[0]);
         if (rc != GRHD_OK) return rc;
     }
     const Params &P = c->P;
+    const bool zmarch = P.zseg > 0 && c->p_alt;
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));
     int64_t launches = 0;
     launch_boundaries(P, c->stream, c->p, c->injector);  // grhd2.f90:233
     launches++;
-    const int per_stage = single_stage_fused(P)
-                              ? 3  // faces, cells, ghosts
-                              : 1 + (P.ny > 1) + (P.nz > 1) + (c->mixed ? 2 : 1) + 1;
+    // the z-march form writes the stage's primitives to the other array; its
+    // cells that no ghost fill rewrites (the outermost ghost layer) must hold
+    // the same values in both
+    if (zmarch && n > 0)
+        HIPCHK(c, hipMemcpyAsync(c->p_alt, c->p, (size_t)P.vstride * NEQ * sizeof(double), hipMemcpyDeviceToDevice,
+                                 c->stream));
+    // launches per stage: the face kernels of each active direction (and the
+    // geometry-table kernel of a direction in the table form, until the
+    // tables are filled), the cell kernel (z-march: the z faces with it) and
+    // the ghost fill
+    int per_stage = 3;  // fused form: all faces, cells, ghosts
+    if (!single_stage_fused(P)) {
+        per_stage = 2;
+        for (int d = 0; d < 3; d++) {
+            const bool on = (d == 0) || (d == 1 && P.ny > 1) || (d == 2 && P.nz > 1);
+            if (!on || (zmarch && d == 2)) continue;
+            per_stage += 1 + (P.face_gw[d] == 0 && !P.geo_ready);
+        }
+        if (zmarch && !P.geo_ready) per_stage += 1;
+    }
     for (int s = 0; s < n; s++) {
         double *u0 = c->u[c->ucur], *u1 = c->u[1 - c->ucur];
         for (int rk = 1; rk <= 3; rk++) {
-            launch_stage(P, c->stream, c->p, c->F, c->fstr, u0, u1, rk, dts[s], c->counters, c->mixed,
-                         c->injector, true);
+            if (zmarch) {
+                launch_stage_z(P, c->stream, c->p, c->p_alt, c->F, c->fstr, c->geo[2], u0, u1, rk, dts[s],
+                               c->counters, c->mixed);
+                launch_boundaries(P, c->stream, c->p_alt, c->injector);
+                std::swap(c->p, c->p_alt);
+            } else {
+                launch_stage(P, c->stream, c->p, c->F, c->fstr, u0, u1, rk, dts[s], c->counters, c->mixed,
+                             c->injector, true);
+            }
             launches += per_stage;
         }
         c->ucur = 1 - c->ucur;
@@ -436,7 +541,7 @@ uint64_t ctx_footprint(const grhd_config &cfg)
     b += (uint64_t)NEQ * P.ng * P.ny * P.nz * sizeof(double) + 8;
     b += (uint64_t)std::max<int64_t>((((int64_t)P.nx * P.ny + 255) / 256) * 4, std::max<int64_t>(4096, P.nz)) *
              sizeof(double) +
-         4 * sizeof(unsigned long long);
+         8 * sizeof(unsigned long long);
     b += (uint64_t)coord_table_size(P.nx, P.ny, P.ng) * sizeof(double);
     return b;
 }
@@ -529,7 +634,7 @@ int grhd_create(const grhd_config *cfg, grhd_ctx **out)
     c->partial_cap = std::max<int64_t>((int64_t)c->partial_n * 4, std::max<int64_t>(4096, P.nz));
     if ((e = hipMalloc(&c->partial, (size_t)c->partial_cap * sizeof(double))) != hipSuccess)
         return bail(e, "hipMalloc partial");
-    if ((e = hipMalloc(&c->counters, 4 * sizeof(unsigned long long))) != hipSuccess)
+    if ((e = hipMalloc(&c->counters, 8 * sizeof(unsigned long long))) != hipSuccess)
         return bail(e, "hipMalloc counters");
     {
         // coordinate table: the grid's x, r, e^{2x}, sin and cos evaluated once
@@ -542,7 +647,7 @@ int grhd_create(const grhd_config *cfg, grhd_ctx **out)
             return bail(e, "hipMemcpy coord");
         c->P.coord = c->coord;
     }
-    if ((e = hipMemsetAsync(c->counters, 0, 4 * sizeof(unsigned long long), c->stream)) != hipSuccess)
+    if ((e = hipMemsetAsync(c->counters, 0, 8 * sizeof(unsigned long long), c->stream)) != hipSuccess)
         return bail(e, "hipMemset");
     if ((e = hipMemsetAsync(c->p, 0, (size_t)P.vstride * NEQ * sizeof(double), c->stream)) != hipSuccess)
         return bail(e, "hipMemset");
@@ -558,6 +663,7 @@ int grhd_destroy(grhd_ctx *c)
     if (!c) return GRHD_OK;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     (void)hipFree(c->p);
+    (void)hipFree(c->p_alt);
     (void)hipFree(c->u[0]);
     (void)hipFree(c->u[1]);
     for (int d = 0; d < 3; d++) {
@@ -1028,6 +1134,16 @@ int grhd_synchronize(grhd_ctx *c)
 {
     if (!c) return GRHD_ERR_INVALID_ARGUMENT;
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    return GRHD_OK;
+}
+
+int grhd_set_stage_form(grhd_ctx *c, int32_t zseg)
+{
+    if (!c || zseg < -1) return GRHD_ERR_INVALID_ARGUMENT;
+    if (zseg > 0 && (!zmarch_possible(c->P) || !c->geo[2]))
+        return fail(c, GRHD_ERR_UNSUPPORTED, "the z-march form needs a 3D grid above 65,536 cells");
+    c->forced_zseg = zseg;
+    c->faces_tuned = false;  // applied (or timed) before the next step
     return GRHD_OK;
 }
 

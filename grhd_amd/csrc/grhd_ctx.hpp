@@ -15,7 +15,12 @@ struct grhd_ctx {
     bool own_stream = false;
     bool mixed = false;
     bool faces_tuned = false;  // face-kernel tiling chosen (tune_faces)
+    int forced_zseg = -1;      // grhd_set_stage_form: -1 timed choice, else P.zseg
     double *p = nullptr;      // ghosted primitives, SoA [5][NZs][NYs][NXs]
+    // second primitive array of the z-march stage form (P.zseg > 0), which
+    // reads one and writes the other; the two are swapped after each stage,
+    // so p is always the current one.  Allocated by the stage-form tuner.
+    double *p_alt = nullptr;
     double *u[2] = {nullptr, nullptr};  // conserved, SoA [5][nz][ny][nx]
     int ucur = 0;             // u[ucur] holds U^n
     void *F[3] = {nullptr, nullptr, nullptr};
@@ -25,7 +30,7 @@ struct grhd_ctx {
     double *partial = nullptr;
     int partial_n = 0;        // CFL partials (4 doubles each)
     int64_t partial_cap = 0;  // doubles in partial (reductions and observers)
-    unsigned long long *counters = nullptr;
+    unsigned long long *counters = nullptr;  // 4 counters, then 4 the tuner's timing runs count into
     double *coord = nullptr;  // coordinate table (Params::coord)
     int64_t steps = 0;
     double dt_cfl = 0.0;

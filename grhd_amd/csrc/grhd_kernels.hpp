@@ -17,6 +17,10 @@ void launch_stage_mk(const Params &P, hipStream_t s, double *p, void *const F[3]
 template <int MK, typename R>
 void launch_face_dir_mk(const Params &P, hipStream_t s, const double *p, void *const F[3], const int64_t fstr[3],
                         int dir);
+template <int MK, typename R>
+void launch_stage_z_mk(const Params &P, hipStream_t s, const double *pin, double *pout, void *const F[3],
+                       const int64_t fstr[3], void *geo_z, const double *u0, double *u, int stage, double dt,
+                       unsigned long long *counters);
 
 // The stage kernels (face fluxes + cell update) with FP64 or FP32 faces.
 void launch_stage_f64(const Params &P, hipStream_t s, double *p, void *const F[3],
@@ -48,6 +52,13 @@ void launch_face_dir_f64(const Params &P, hipStream_t s, const double *p, void *
                          int dir);
 void launch_face_dir_f32(const Params &P, hipStream_t s, const double *p, void *const F[3], const int64_t fstr[3],
                          int dir);
+
+// One RK stage (1..3) of a 3D grid in the z-march form (P.zseg > 0): reads
+// the primitives from pin and writes them to pout (interior only; the ghost
+// fill of pout follows separately).  geo_z: the z-face geometry table.
+void launch_stage_z(const Params &P, hipStream_t s, const double *pin, double *pout, void *const F[3],
+                    const int64_t fstr[3], void *geo_z, const double *u0, double *u, int stage, double dt,
+                    unsigned long long *counters, bool mixed);
 
 // One RK stage: face fluxes, cell update (stage 1..3; 0 = rhs only into u),
 // then the ghost fill when with_boundaries.

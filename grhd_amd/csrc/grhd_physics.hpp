@@ -53,6 +53,15 @@ struct Params {
     // the problem (tune_faces).
     int face_gw[3];
     void *geo_tab[3];
+    // 1: the face-geometry tables geo_tab hold this problem's geometry (the
+    // metric is static: filled once, when the single-problem tuner timed the
+    // table form), so the stage skips the kernels that evaluate them
+    int geo_ready;
+    // z-march segment of the single-problem stage: each lane walks this many
+    // cells of one (x, y) column, computing the z faces in order and
+    // differencing them in registers (k_zcell); 0: separate z-face and cell
+    // kernels.  Chosen by timing the stage forms on the problem (tune_faces).
+    int zseg;
     double gam, g1, mass, spin;
     double rho_floor, p_floor;
     double x_min, dx, y_min, dy, z_min, dz;
@@ -816,11 +825,233 @@ GRHD_HD bool local_to_coord(const R Ul[NEQ], const R Fl[NEQ], R w, R alpha, R sq
     return true;
 }
 
+// The same HLLC solver without early exits, for FP32 faces: every quantity of
+// the reference's path is evaluated unconditionally, in the same operations
+// and order, and each fallback test clears a validity flag instead of
+// returning.  A rejected face may then compute values from a degenerate state
+// (infinities, NaN; no traps on the device), which the flag discards; an
+// accepted face gets the branching form's result bit for bit.  The face
+// kernel becomes one long basic block the compiler schedules across: 2 %
+// faster mixed stage on the 3D torus.  FP64 faces keep the branching form,
+// which needs fewer registers: the flat one spills 400 B per lane at 128
+// VGPRs and halved the FP64 stage rate.
+
+// calc_srhd_local_state (fluxes.f90:509-568): U, F and the speeds of the
+// local state; returns the reference's validity.
+template <typename R>
+GRHD_HD bool srhd_local_flat(R g1, R gam, const R ql[NEQ], R U[NEQ], R F[NEQ], R &wmin, R &wmax)
+{
+    const R vmax = (R)(1.0 - 1.0e-5);
+    const R rho = ql[EQ_DE], pr = ql[EQ_PR];
+    const R v0 = ql[EQ_VX], v1 = ql[EQ_VY], v2 = ql[EQ_VZ];
+    bool ok = finite(rho) & finite(pr) & finite(v0) & finite(v1) & finite(v2) & !(rho <= (R)0) & !(pr < (R)0);
+    const R vsq = v0 * v0 + v1 * v1 + v2 * v2;
+    ok = ok & finite(vsq) & !(vsq < (R)0) & !(vsq >= vmax);
+    const R h = (R)1 + rdiv(g1 * pr, rho);
+    ok = ok & finite(h) & !(h <= (R)0);
+    const R cs2 = rdiv(gam * pr, rho * h);
+    ok = ok & finite(cs2) & !(cs2 < (R)0) & !(cs2 >= (R)1);
+    const R W = rrsqrt((R)1 - vsq);
+    const R D = rho * W;
+    const R tau = rho * h * (W * W) - pr - D;
+    const R m0 = rho * h * (W * W) * v0, m1 = rho * h * (W * W) * v1, m2 = rho * h * (W * W) * v2;
+    U[EQ_DE] = D;
+    U[EQ_PR] = tau;
+    U[EQ_VX] = m0;
+    U[EQ_VY] = m1;
+    U[EQ_VZ] = m2;
+    F[EQ_DE] = D * v0;
+    F[EQ_PR] = m0 - D * v0;
+    F[EQ_VX] = m0 * v0 + pr;
+    F[EQ_VY] = m1 * v0;
+    F[EQ_VZ] = m2 * v0;
+    const R den = (R)1 - vsq * cs2;
+    const R rad = ((R)1 - vsq) * ((R)1 - vsq * cs2 - (v0 * v0) * ((R)1 - cs2));
+    ok = ok & !(den <= (R)1.0e-14) & !(rad < (R)-1.0e-12);
+    const R ac = sqrt(cs2) * sqrt(rmax((R)0, rad));
+    const R rden = rcp(den);
+    wmin = quot(v0 * ((R)1 - cs2) - ac, den, rden);
+    wmax = quot(v0 * ((R)1 - cs2) + ac, den, rden);
+#pragma unroll
+    for (int e = 0; e < NEQ; e++) ok = ok & finite(U[e]) & finite(F[e]);
+    return ok & finite(wmin) & finite(wmax);
+}
+
+// conservative_state_is_admissible (fluxes.f90:571-591).
+template <typename R>
+GRHD_HD bool admissible_flat(const R U[NEQ])
+{
+    bool ok = true;
+#pragma unroll
+    for (int e = 0; e < NEQ; e++) ok = ok & finite(U[e]);
+    const R E = U[EQ_PR] + U[EQ_DE];
+    const R msq = U[EQ_VX] * U[EQ_VX] + U[EQ_VY] * U[EQ_VY] + U[EQ_VZ] * U[EQ_VZ];
+    const R scale = rmax(rmax(Lim<R>::tiny(), E * E), U[EQ_DE] * U[EQ_DE] + msq);
+    return ok & !(U[EQ_DE] <= (R)0) & !(E <= (R)0) & !(E * E - U[EQ_DE] * U[EQ_DE] - msq <= (R)1.0e-13 * scale);
+}
+
+// local_flux_to_coordinate (fluxes.f90:596-624).
+template <typename R, typename FG>
+GRHD_HD bool local_to_coord_flat(const R Ul[NEQ], const R Fl[NEQ], R w, R alpha, R sqg, R gdd,
+                            const FG &fg, R F[NEQ])
+{
+    R mf[NEQ];
+    bool ok = true;
+#pragma unroll
+    for (int e = 0; e < NEQ; e++) {
+        mf[e] = Fl[e] - w * Ul[e];
+        ok = ok & finite(mf[e]);
+    }
+    const R fac = sqg * alpha * sqrt(gdd);
+    ok = ok & finite(fac);
+    F[EQ_DE] = fac * mf[EQ_DE];
+    F[EQ_PR] = fac * mf[EQ_PR];
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        F[EQ_VX + i] = 0;
+#pragma unroll
+        for (int a = 0; a < 3; a++) F[EQ_VX + i] = F[EQ_VX + i] + fac * fg.cof(a, i) * mf[EQ_VX + a];
+    }
+#pragma unroll
+    for (int e = 0; e < NEQ; e++) ok = ok & finite(F[e]);
+    return ok;
+}
+
+// calc_hllc_fluxes (fluxes.f90:174-397).  Returns false where the reference
+// falls back to HLLE; the caller then evaluates HLLE.
+template <typename R, int DIR, typename FG>
+GRHD_HD bool hllc_flat(R g1, R gam, const R qL[NEQ], const R qR[NEQ], const FG &f, R F[NEQ])
+{
+    R qLh[NEQ], qRh[NEQ], uL[NEQ], uR[NEQ], fL[NEQ], fR[NEQ];
+    R lLmin, lLmax, lRmin, lRmax;
+    // lapse, sqrt(gamma) and the face frame were checked with the geometry
+    // (fluxes.f90:195-205, 402-473)
+    bool ok = (f.flags() & FG_FRAME_OK) != 0;
+    const R alpha = f.alpha(), sqg = f.sqg(), gdd = f.gdd();
+#pragma unroll
+    for (int e = 0; e < NEQ; e++) {
+        qLh[e] = qL[e];
+        qRh[e] = qR[e];
+    }
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        qLh[EQ_VX + a] = 0;
+        qRh[EQ_VX + a] = 0;
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            qLh[EQ_VX + a] = qLh[EQ_VX + a] + f.cof(a, i) * qL[EQ_VX + i];
+            qRh[EQ_VX + a] = qRh[EQ_VX + a] + f.cof(a, i) * qR[EQ_VX + i];
+        }
+    }
+    ok = ok & srhd_local_flat(g1, gam, qLh, uL, fL, lLmin, lLmax);
+    ok = ok & srhd_local_flat(g1, gam, qRh, uR, fR, lRmin, lRmax);
+    const R ll = rmin(lLmin, lRmin), lr = rmax(lLmax, lRmax);
+    const R w = f.wface();
+    ok = ok & finite(w) & !(lr - ll <= (R)1.0e-13);
+    // supersonic faces take an outer state; the star region is then not used
+    const bool outL = (w <= ll), outR = !outL & (w >= lr);
+    R uh[NEQ], fh[NEQ];
+    const R dhll = lr - ll;
+    const R rhll = rcp(dhll);
+    bool okS = true;
+#pragma unroll
+    for (int e = 0; e < NEQ; e++) {
+        uh[e] = quot(lr * uR[e] - ll * uL[e] + fL[e] - fR[e], dhll, rhll);
+        fh[e] = quot(lr * fL[e] - ll * fR[e] + lr * ll * (uR[e] - uL[e]), dhll, rhll);
+        okS = okS & finite(uh[e]) & finite(fh[e]);
+    }
+    const R Eh = uh[EQ_PR] + uh[EQ_DE], FEh = fh[EQ_PR] + fh[EQ_DE];
+    const R A = FEh, B = -(Eh + fh[EQ_VX]), Cq = uh[EQ_VX];
+    const R disc0 = B * B - (R)4 * A * Cq;
+    const R dscale = rmax(rmax(Lim<R>::tiny(), B * B), fabs((R)4 * A * Cq));
+    okS = okS & finite(disc0) & !(disc0 < (R)-1.0e-12 * dscale);
+    const R disc = rmax((R)0, disc0);
+    const R cscale = rmax(rmax(Lim<R>::tiny(), fabs(B)), fabs(Cq));
+    const bool quad = fabs(A) > (R)1.0e-13 * cscale;
+    // the root lambda*: linear equation, or the cancellation-free root of the
+    // quadratic, or its alternate form; one division of the chosen operands
+    const R sd = sqrt(disc);
+    const R dthr = (R)1.0e-14 * rmax(rmax(Lim<R>::tiny(), fabs(B)), sd);
+    const R den1 = -B - sd, den2 = -B + sd;
+    const bool first = fabs(den1) > dthr;
+    okS = okS & (quad ? (first | !(fabs(den2) <= dthr))
+                      : !(fabs(B) <= (R)1.0e-14 * rmax(Lim<R>::tiny(), fabs(Cq))));
+    const R lnum = !quad ? -Cq : first ? den1 : (R)2 * Cq;
+    const R lden = !quad ? B : first ? (R)2 * A : den2;
+    R ls = rdiv(lnum, lden);
+    const R rtol = (R)1.0e-10 * rmax(rmax((R)1, fabs(ll)), fabs(lr));
+    bool valid = finite(ls) & (ls >= ll - rtol) & (ls <= lr + rtol) & (fabs(ls) <= (R)1 + rtol);
+    const R ra = rdiv(den2, (R)2 * A);
+    const bool alt = !valid & quad & finite(ra) & (ra >= ll - rtol) & (ra <= lr + rtol) & (fabs(ra) <= (R)1 + rtol);
+    ls = alt ? ra : ls;
+    okS = okS & (valid | alt);
+    ls = rmin(lr, rmax(ll, ls));
+    const R AL = ll * (uL[EQ_PR] + uL[EQ_DE]) - uL[EQ_VX];
+    const R BL = uL[EQ_VX] * (ll - qLh[EQ_VX]) - qLh[EQ_PR];
+    const R AR = lr * (uR[EQ_PR] + uR[EQ_DE]) - uR[EQ_VX];
+    const R BR = uR[EQ_VX] * (lr - qRh[EQ_VX]) - qRh[EQ_PR];
+    const R denL = (R)1 - ll * ls, denR = (R)1 - lr * ls;
+    okS = okS & !(fabs(denL) <= (R)1.0e-13) & !(fabs(denR) <= (R)1.0e-13);
+    const R psL = rdiv(AL * ls - BL, denL);
+    const R psR = rdiv(AR * ls - BR, denR);
+    const R sscale = rmax(rmax(rmax(rmax(Lim<R>::tiny(), fabs(uL[EQ_PR] + uL[EQ_DE])),
+                                    fabs(uR[EQ_PR] + uR[EQ_DE])),
+                               fabs(uL[EQ_VX])),
+                          fabs(uR[EQ_VX]));
+    const R ptol = (R)1.0e-8 * rmax(rmax(rmax(rmax(Lim<R>::tiny(), fabs(qLh[EQ_PR])),
+                                              fabs(qRh[EQ_PR])),
+                                         fabs(psL)),
+                                    fabs(psR)) +
+                   (R)1.0e2 * Lim<R>::eps() * sscale;
+    okS = okS & finite(psL) & finite(psR) & !(fabs(psL - psR) > ptol);
+    const R ps = (R)0.5 * (psL + psR);
+    okS = okS & finite(ps) & !(fabs(ll - ls) <= (R)1.0e-13) & !(fabs(lr - ls) <= (R)1.0e-13);
+    // star states (Mignone-Bodo): both must be admissible with finite fluxes
+    // (fluxes.f90:351-380); the one on the side of the face speed is used.
+    // The other side is checked first and dropped, which keeps one star state
+    // live instead of two.
+    const bool starL = (w <= ls);
+    bool okO = true;
+    {
+        const R lo = starL ? lr : ll, dso = lo - ls, rdso = rcp(dso);
+        R uo[NEQ];
+#pragma unroll
+        for (int e = 0; e < NEQ; e++) {
+            const R uk = starL ? uR[e] : uL[e], fk = starL ? fR[e] : fL[e];
+            const R pv = (e == EQ_PR) ? ls : (e == EQ_VX) ? (R)1 : (R)0;
+            uo[e] = quot(lo * uk - fk + ps * pv, dso, rdso);
+            okO = okO & finite(fk + lo * (uo[e] - uk));
+        }
+        okO = okO & admissible_flat(uo);
+    }
+    const R lk = starL ? ll : lr, dsk = lk - ls, rdsk = rcp(dsk);
+    R us[NEQ], fs[NEQ];
+#pragma unroll
+    for (int e = 0; e < NEQ; e++) {
+        const R uk = starL ? uL[e] : uR[e], fk = starL ? fL[e] : fR[e];
+        const R pv = (e == EQ_PR) ? ls : (e == EQ_VX) ? (R)1 : (R)0;
+        us[e] = quot(lk * uk - fk + ps * pv, dsk, rdsk);
+        fs[e] = fk + lk * (us[e] - uk);
+        okS = okS & finite(fs[e]);
+    }
+    okS = okS & okO & admissible_flat(us);
+    // the state the face speed falls in, back to coordinates
+    R Us[NEQ], Fs[NEQ];
+#pragma unroll
+    for (int e = 0; e < NEQ; e++) {
+        Us[e] = outL ? uL[e] : outR ? uR[e] : us[e];
+        Fs[e] = outL ? fL[e] : outR ? fR[e] : fs[e];
+    }
+    const bool okT = local_to_coord_flat(Us, Fs, w, alpha, sqg, gdd, f, F);
+    return ok & (outL | outR | okS) & okT;
+}
+
 // calc_hllc_fluxes (fluxes.f90:174-397).  Returns false where the reference
 // falls back to HLLE; the caller then evaluates HLLE.
 template <typename R, int DIR, typename FG>
 GRHD_HD bool hllc_try(R g1, R gam, const R qL[NEQ], const R qR[NEQ], const FG &f, R F[NEQ])
 {
+    if constexpr (sizeof(R) == sizeof(float)) return hllc_flat<R, DIR>(g1, gam, qL, qR, f, F);
     R qLh[NEQ], qRh[NEQ], uL[NEQ], uR[NEQ], fL[NEQ], fR[NEQ];
     R lLmin, lLmax, lRmin, lRmax;
     // lapse, sqrt(gamma) and the face frame were checked with the geometry

@@ -278,7 +278,7 @@ void launch_flux(const Params &P, hipStream_t s, const double *p, R *F, int64_t 
         R *tab = static_cast<R *>(P.geo_tab[DIR - 1]);
         const int nfx = P.nx + (DIR == 1), nfy = P.ny + (DIR == 2), nfz = P.nz + (DIR == 3);
         const int64_t nab = (int64_t)nfx * nfy;
-        k_geo<MK, DIR, R><<<blocks_for(nab, 256), 256, 0, s>>>(P, tab);
+        if (!P.geo_ready) k_geo<MK, DIR, R><<<blocks_for(nab, 256), 256, 0, s>>>(P, tab);
         const int64_t nb = ((nab + 63) / 64) * ((nfz + 3) / 4);
         k_flux_tab<MK, DIR, R, SCH><<<(unsigned)nb, 256, 0, s>>>(P, p, F, fstr, tab);
         return;
@@ -339,13 +339,16 @@ void launch_face_dir_mk(const Params &P, hipStream_t s, const double *p, void *c
 // Cell t (0 <= t < nx ny nz, x fastest) of one problem.  stage 1..3: rhs
 // (evolution.f90:85-88, 137-139, 180-182) and the RK combination
 // (grhd2.f90:243-249) into u; cons_to_prim follows in c2p_body.  stage 0
-// writes the rhs itself to u (parity entry point).
-template <int MK, typename R>
+// writes the rhs itself to u (parity entry point).  ZREG: the z-face fluxes
+// of the cell are zl (left) and zr (right), held in registers by the caller,
+// instead of entries of Fz.
+template <int MK, typename R, bool ZREG = false>
 __device__ __forceinline__ void update_body(const Params &P, const double *__restrict__ p,
                                             const R *__restrict__ Fx, const R *__restrict__ Fy,
                                             const R *__restrict__ Fz, int64_t fsx, int64_t fsy,
                                             int64_t fsz, const double *__restrict__ u0,
-                                            double *__restrict__ u, int stage, double dt, int64_t t)
+                                            double *__restrict__ u, int stage, double dt, int64_t t,
+                                            const R *zl = nullptr, const R *zr = nullptr)
 {
     const int64_t ncell = (int64_t)P.nx * P.ny * P.nz;
     const int i0 = (int)(t % P.nx);
@@ -386,8 +389,13 @@ __device__ __forceinline__ void update_body(const Params &P, const double *__res
         fx1[v] = Fx[v * fsx + fax + 1];
         fy0[v] = Fyp[v * fsyp + fay];
         fy1[v] = Fyp[v * fsyp + fay + sy];
-        fz0[v] = Fzp[v * fszp + faz];
-        fz1[v] = Fzp[v * fszp + faz + sz];
+        if constexpr (ZREG) {
+            fz0[v] = zl[v];
+            fz1[v] = zr[v];
+        } else {
+            fz0[v] = Fzp[v * fszp + faz];
+            fz1[v] = Fzp[v * fszp + faz + sz];
+        }
         a0[v] = u0p[v * ncell + tr];
         a1[v] = u1p[v * ncell + tr];
     }
@@ -431,11 +439,13 @@ __global__ __launch_bounds__(256) void k_update(Params P, const double *__restri
 
 // cons_to_prim after the RK combination (grhd2.f90:252-260) of cell t: a
 // serial, divide-heavy root solve per cell.  WARM starts the root solve from
-// the cell's previous Lorentz factor (mixed-precision path).
-template <int MK, bool WARM>
+// the cell's previous Lorentz factor (mixed-precision path).  The primitives
+// are written to p in place, or to pout with SPLIT (p is then only read).
+template <int MK, bool WARM, bool SPLIT = false>
 __device__ __forceinline__ void c2p_body(const Params &P, double *__restrict__ p,
                                          double *__restrict__ u,
-                                         unsigned long long *__restrict__ counters, int64_t t)
+                                         unsigned long long *__restrict__ counters, int64_t t,
+                                         double *__restrict__ pout = nullptr)
 {
     const int64_t ncell = (int64_t)P.nx * P.ny * P.nz;
     const int i = (int)(t % P.nx) + 1;
@@ -459,8 +469,9 @@ __device__ __forceinline__ void c2p_body(const Params &P, double *__restrict__ p
     const bool atm = c2p(P.g1, P.rho_floor, P.p_floor, g, gi, sqg, c, q, brent, w_start, &repaired);
     if (atm) atomicAdd(&counters[0], 1ull);
     if (brent) atomicAdd(&counters[1], 1ull);
+    double *pw = SPLIT ? pout : p;
 #pragma unroll
-    for (int v = 0; v < NEQ; v++) p[v * P.vstride + pc] = q[v];
+    for (int v = 0; v < NEQ; v++) pw[v * P.vstride + pc] = q[v];
     // U changes only where con2prim rescales the momentum or resets the cell
     if (repaired) {
 #pragma unroll
@@ -510,6 +521,107 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
     if (t >= (int64_t)P.nx * P.ny * P.nz) return;
     update_body<MK, R>(P, p, Fx, Fy, Fz, fsx, fsy, fsz, u0, u, stage, dt, t);
     c2p_body<MK, sizeof(R) == sizeof(float)>(P, p, u, counters, t);
+}
+
+// z-march form of the stage (3D grids): each lane owns one (x, y) column and
+// walks a segment of P.zseg cells along z.  It computes the z faces in order
+// (the face geometry from the stage's table, as k_flux_tab reads it), keeps
+// the last one in registers and differences it with the next, so the z fluxes
+// never go through HBM; the x and y fluxes come from their face arrays.  The
+// update and con2prim of each cell follow in the same thread, as in k_cell.
+// A segment computes its first face, which the previous segment also
+// computes (zseg + 1 faces for zseg cells).  The primitives are read from
+// pin and written to pout: a later face of the column, or a neighbouring
+// segment, still reads the cells this one has updated.  The arithmetic is
+// that of k_flux_tab<3> and k_cell, so the results are theirs bit for bit.
+template <int MK, typename R, int SCH>
+__device__ __forceinline__ void z_face(const Params &P, const double *__restrict__ p, const FaceGeo<R> &fg,
+                                       bool zero, int i, int j, int c, R out[NEQ])
+{
+    if (zero) {
+#pragma unroll
+        for (int v = 0; v < NEQ; v++) out[v] = 0;
+    } else {
+        flux_face<MK, 3, R, SCH>(P, p, fg, i, j, c, c, P.nz, out);
+    }
+}
+
+template <int MK, typename R, int SCH>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_zcell(
+    Params P, const double *__restrict__ pin, double *__restrict__ pout, const R *__restrict__ Fx,
+    const R *__restrict__ Fy, int64_t fsx, int64_t fsy, const R *__restrict__ tab, const double *__restrict__ u0,
+    double *__restrict__ u, int stage, double dt, unsigned long long *__restrict__ counters)
+{
+    const int64_t nab = (int64_t)P.nx * P.ny;
+    const int64_t nchunk = (nab + 63) / 64;
+    // consecutive waves take neighbouring column chunks of one segment, which
+    // read the same rows of y faces
+    const int64_t w = (int64_t)xcd_block(blockIdx.x, gridDim.x) * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const int64_t ab = (w % nchunk) * 64 + (threadIdx.x & 63);
+    const int kb = (int)(w / nchunk) * P.zseg + 1;
+    if (ab >= nab || kb > P.nz) return;
+    const int ke = min(kb + P.zseg, P.nz + 1);
+    const int i = (int)(ab % P.nx) + 1, j = (int)(ab / P.nx) + 1;
+    const bool zero = (int)tab[FG_FLAGS * nab + ab] & FG_ZERO;
+    double *pr = const_cast<double *>(pin);  // read only: c2p_body writes pout
+    // the previous face of the column waits in LDS (not in registers, which
+    // the face and cell code need in full); each lane touches only its own slots
+    __shared__ R zprev[4][NEQ][64];
+    R(*fl)[64] = zprev[threadIdx.x >> 6];
+    const int lane = threadIdx.x & 63;
+    // face c lies between cells c and c + 1; one copy of the face code serves
+    // the segment's first face and the others
+    for (int c = kb - 1; c < ke; c++) {
+        // The face geometry and the centre metric depend on (x, y) only; the
+        // loop re-reads them every cell instead of holding them in registers
+        // over the loop (the compiler would hoist them, and spill)
+        const R *tp = tab + ab;
+        const double *cp = P.coord;
+        asm volatile("" : "+v"(tp), "+s"(cp));
+        Params Pc = P;
+        Pc.coord = cp;
+        const FaceGeo<R> fg{tp, nab};
+        R fr[NEQ];
+        z_face<MK, R, SCH>(Pc, pin, fg, zero, i, j, c, fr);
+        if (c >= kb) {
+            R zl[NEQ];
+#pragma unroll
+            for (int v = 0; v < NEQ; v++) zl[v] = fl[v][lane];
+            const int64_t t = ((int64_t)(c - 1) * P.ny + (j - 1)) * P.nx + (i - 1);
+            update_body<MK, R, true>(Pc, pin, Fx, Fy, nullptr, fsx, fsy, 0, u0, u, stage, dt, t, zl, fr);
+            c2p_body<MK, sizeof(R) == sizeof(float), true>(Pc, pr, u, counters, t, pout);
+        }
+#pragma unroll
+        for (int v = 0; v < NEQ; v++) fl[v][lane] = fr[v];
+    }
+}
+
+// Stage 1..3 in the z-march form: x and y faces (as in the split form, into
+// their arrays), the z-face geometry into geo_z, then k_zcell.
+template <int MK, typename R>
+void launch_stage_z_mk(const Params &P, hipStream_t s, const double *pin, double *pout, void *const F[3],
+                       const int64_t fstr[3], void *geo_z, const double *u0, double *u, int stage, double dt,
+                       unsigned long long *counters)
+{
+    R *Fx = (R *)F[0], *Fy = (R *)F[1], *tab = (R *)geo_z;
+    const int64_t nab = (int64_t)P.nx * P.ny;
+    const int64_t waves = ((nab + 63) / 64) * ((P.nz + P.zseg - 1) / P.zseg);
+    const unsigned nb = (unsigned)((waves + 3) / 4);
+#define GRHD_ZSTAGE(SCHV)                                                                                        \
+    do {                                                                                                         \
+        launch_flux<MK, 1, R, SCHV>(P, s, pin, Fx, fstr[0]);                                                     \
+        launch_flux<MK, 2, R, SCHV>(P, s, pin, Fy, fstr[1]);                                                     \
+        if (!P.geo_ready) k_geo<MK, 3, R><<<blocks_for(nab, 256), 256, 0, s>>>(P, tab);                          \
+        k_zcell<MK, R, SCHV><<<nb, 256, 0, s>>>(P, pin, pout, Fx, Fy, fstr[0], fstr[1], tab, u0, u, stage, dt,  \
+                                                counters);                                                       \
+    } while (0)
+    if constexpr (MK == MK_KS || MK == MK_KS_LOG) {
+        const int sch = face_scheme<MK>(P);
+        if (sch == 1) { GRHD_ZSTAGE(1); return; }
+        if (sch == 2) { GRHD_ZSTAGE(2); return; }
+    }
+    GRHD_ZSTAGE(0);
+#undef GRHD_ZSTAGE
 }
 
 // fused: one face launch for every direction (for small grids, where launch
@@ -581,6 +693,9 @@ void launch_stage_mk(const Params &P, hipStream_t s, double *p, void *const F[3]
                                           const int64_t[3], const double *, double *, int, double,      \
                                           unsigned long long *, bool);                                  \
     template void launch_face_dir_mk<MKV, R>(const Params &, hipStream_t, const double *, void *const[3], \
-                                             const int64_t[3], int);
+                                             const int64_t[3], int);                                    \
+    template void launch_stage_z_mk<MKV, R>(const Params &, hipStream_t, const double *, double *,       \
+                                            void *const[3], const int64_t[3], void *, const double *,   \
+                                            double *, int, double, unsigned long long *);
 
 }  // namespace grhd

@@ -205,6 +205,11 @@ class Solver:
     def synchronize(self):
         self._chk(self._L.grhd_synchronize(self.h))
 
+    def set_stage_form(self, zseg: int):
+        """-1: timed choice on the first step; 0: split form; n > 0: z-march
+        form with n-cell segments (3D grids above 65,536 cells)."""
+        self._chk(self._L.grhd_set_stage_form(self.h, int(zseg)))
+
     def last_timing(self):
         ms = ctypes.c_double()
         n = ctypes.c_int64()

@@ -234,6 +234,16 @@ int grhd_synchronize(grhd_ctx *ctx);
  * launches it made. */
 int grhd_last_timing(grhd_ctx *ctx, double *ms, int64_t *launches);
 
+/* Stage form of a 3D problem above 65,536 cells (no counterpart in the
+ * reference, whose strip loops have one structure, evolution.f90:48-189).
+ * -1 (the default): chosen on the first step by timing one stage of each form;
+ * 0: the split form (face fluxes of every direction into their arrays, then
+ * the cell kernel); n > 0: the z-march form, in which each thread walks
+ * n-cell segments of one column and differences the z fluxes in registers
+ * (a second primitive array is allocated).  Every form gives the same results
+ * bit for bit.  GRHD_ERR_UNSUPPORTED for n > 0 on other grids. */
+int grhd_set_stage_form(grhd_ctx *ctx, int32_t zseg);
+
 /* Device bytes grhd_create allocates for cfg: the ghosted primitives, two
  * conserved buffers and the face fluxes (the reference allocates p, u, up and
  * rhs, initialization.f90:211-215, plus its metric cache).  Used to admit

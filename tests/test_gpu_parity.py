@@ -165,15 +165,20 @@ def _large(name):
     return cfg
 
 
-@pytest.mark.parametrize("name", ["fm3d_ks_mp5_hllc_sensor", "fm3d_ks_weno5_hlle"])
-def test_large_grid_one_step_matches_oracle(name):
-    """Grids above 65,536 cells take the per-direction stage form (one face
-    launch per direction, the cell kernel) that the 3D torus benchmark runs,
-    with the face kernels compiled for the scheme (MP5 + sensor + HLLC, WENO5
-    + HLLE); smaller grids the fused form the other tests cover."""
+@pytest.mark.parametrize("name,zseg", [("fm3d_ks_mp5_hllc_sensor", -1), ("fm3d_ks_mp5_hllc_sensor", 0),
+                                       ("fm3d_ks_mp5_hllc_sensor", 16), ("fm3d_ks_weno5_hlle", -1),
+                                       ("fm3d_ks_weno5_hlle", 8)])
+def test_large_grid_one_step_matches_oracle(name, zseg):
+    """Grids above 65,536 cells take the per-direction stage forms that the 3D
+    torus benchmark runs, with the face kernels compiled for the scheme (MP5 +
+    sensor + HLLC, WENO5 + HLLE): the split form (one face launch per
+    direction, then the cell kernel, zseg 0), the z-march form (zseg > 0) and
+    the timed choice between them (-1); smaller grids the fused form the other
+    tests cover."""
     cfg = _large(name)
-    print("large grid one step", name, cfg.nx, cfg.ny, cfg.nz, flush=True)
+    print("large grid one step", name, cfg.nx, cfg.ny, cfg.nz, "stage form", zseg, flush=True)
     with Solver(cfg) as S:
+        S.set_stage_form(zseg)
         O = oracle.Oracle(cfg)
         S.initial_conditions()
         p = seeded_noise(S.get_primitives(), S.nx, S.ny, S.nz, S.ng, 2, 1.0e-2, 3435)
@@ -192,6 +197,38 @@ def test_large_grid_one_step_matches_oracle(name):
     # cascade), which shows in the Linf of the small S_theta: 2e-12 of its
     # maximum.  Linf is held to 1e-11.
     assert np.all(l1 <= TOL_STEP) and np.all(li <= 1e-11), (l1, li)
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_stage_forms_agree_bit_for_bit(precision):
+    """The split form and the z-march form with segments that divide nz (8)
+    or not (16, 32 of nz = 40) give the same state after three steps, bit for
+    bit, in FP64 and with FP32 faces."""
+    cfg = _large("fm3d_ks_mp5_hllc_sensor").replace(precision=precision)
+    print("stage forms", cfg.nx, cfg.ny, cfg.nz, "precision", precision, flush=True)
+    out = {}
+    for zseg in (0, 8, 16, 32):
+        with Solver(cfg) as S:
+            S.set_stage_form(zseg)
+            S.initial_conditions()
+            p = seeded_noise(S.get_primitives(), S.nx, S.ny, S.nz, S.ng, 2, 1.0e-2, 3435)
+            S.set_primitives(p)
+            dt = S.compute_dt()
+            S.step(dt, 2)
+            S.step(dt, 1)
+            out[zseg] = (S.get_conserved(), S.get_primitives(), S.counters())
+    for zseg in (8, 16, 32):
+        assert np.array_equal(out[zseg][0], out[0][0]), zseg
+        assert np.array_equal(out[zseg][1], out[0][1]), zseg
+        assert out[zseg][2] == out[0][2], zseg
+
+
+def test_stage_form_refused_on_small_grids():
+    cfg = config_from_par(CASES["fm3d_ks_mp5_hllc_sensor"])
+    with Solver(cfg) as S:
+        with pytest.raises(Exception):
+            S.set_stage_form(8)
+        S.set_stage_form(0)
 
 
 @pytest.mark.parametrize("name,grid", [("fm_equatorial_ks_log", ("nx = 48", "nx = 400", "nz = 16", "nz = 200")),
