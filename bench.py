@@ -412,12 +412,20 @@ def main():
                     help="with --c3: the direct path only (no batch-of-one launch forms)")
     ap.add_argument("--michel", action="store_true",
                     help="only the Michel C2 convergence study (nr 256/1024/4096, FP64 and mixed)")
+    ap.add_argument("--valu-rate", action="store_true",
+                    help="only run the VALU issue-rate probe (grhd_amd/bin/valu_rate): cycles per wave64 "
+                         "FP64/FP32 instruction per SIMD, the ceiling of the VALU-bound stage kernels")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain device copy of known size (for HBM counter scaling)")
     a = ap.parse_args()
     if a.copy_calibration:
         copy_calibration()
         return
+    if a.valu_rate:
+        probe = ROOT / "grhd_amd" / "bin" / "valu_rate"
+        if not probe.exists():
+            raise SystemExit(f"{probe} is not built: run `python -m grhd_amd.build`")
+        raise SystemExit(subprocess.call([str(probe)]))
     if a.michel:
         michel_convergence()
         return

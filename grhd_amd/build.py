@@ -105,6 +105,20 @@ def build_driver(verbose: bool = True) -> Path:
     return DRIVER
 
 
+PROBE = PKG / "bin" / "valu_rate"
+
+
+def build_probe(verbose: bool = True) -> Path:
+    """VALU issue-rate probe (grhd_amd/bin/valu_rate), run by `bench.py --valu-rate`."""
+    PROBE.parent.mkdir(parents=True, exist_ok=True)
+    cmd = [HIPCC, *FLAGS, "-o", str(PROBE), str(CSRC / "valu_rate.hip")]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.check_call(cmd)
+    return PROBE
+
+
 if __name__ == "__main__":
     build(verbose="-q" not in sys.argv)
     build_driver(verbose="-q" not in sys.argv)
+    build_probe(verbose="-q" not in sys.argv)
