@@ -152,16 +152,16 @@ template <typename R> GRHD_HD R sgn1(R a) { return copysign((R)1, a); }
 GRHD_HD bool finite(double a) { return isfinite(a); }
 GRHD_HD bool finite(float a) { return isfinite(a); }
 
-// FP64 reciprocal and quotient on the device in eight instructions: v_rcp_f64,
-// two Newton steps and the remainder correction -- the operations, in order,
-// of the compiler's IEEE sequence (v_div_scale, v_rcp_f64, four FMAs, a
-// multiply, an FMA, v_div_fmas, v_div_fixup) without its range scaling and
-// special-case fix-up, which leave the result unchanged when the operands and
-// the quotient are normal numbers, as every divisor of the path is (lengths,
-// densities, Lorentz factors, metric components).  The result is then the
-// correctly rounded quotient, bit for bit the IEEE division's; only a zero
-// numerator -0 gives +0, as in quot below.  The host (initial data, set-up)
-// divides with the IEEE operator.  Used in the cell kernel's metric and
+// FP64 reciprocal and quotient on the device in nine instructions: v_rcp_f64,
+// two Newton steps, the remainder correction and v_div_fixup -- the
+// operations, in order, of the compiler's IEEE sequence (v_div_scale twice,
+// v_rcp_f64, four FMAs, a multiply, an FMA, v_div_fmas, v_div_fixup) without
+// the range scaling, which leaves the result unchanged unless the quotient's
+// exponent is beyond 2^768 or its operands near the ends of the exponent
+// range; no operand of the path comes near (lengths, densities, Lorentz
+// factors, metric components).  v_div_fixup gives zero, infinite and NaN
+// operands the IEEE results.  The result is the IEEE division's bit for bit.
+// The host (initial data, set-up) divides with the IEEE operator.  Used in the cell kernel's metric and
 // con2prim (its root solve divides four times per iteration): 7 % fewer
 // instructions there and +2.6 % on the FP64 stage of the 3D torus, with the
 // final state unchanged bit for bit.  The face kernels keep the IEEE
@@ -173,23 +173,13 @@ GRHD_HD double div64(double a, double b)
     const double y0 = __builtin_amdgcn_rcp(b);
     const double y1 = fma(fma(-b, y0, 1.0), y0, y0);
     const double y2 = fma(fma(-b, y1, 1.0), y1, y1);
-    const double q = a * y2;
-    return fma(fma(-b, q, a), y2, q);
+    const double q0 = a * y2;
+    return __builtin_amdgcn_div_fixup(fma(fma(-b, q0, a), y2, q0), b, a);
 #else
     return a / b;
 #endif
 }
-GRHD_HD double rcp64(double b)
-{
-#if defined(__HIP_DEVICE_COMPILE__)
-    const double y0 = __builtin_amdgcn_rcp(b);
-    const double y1 = fma(fma(-b, y0, 1.0), y0, y0);
-    const double y2 = fma(fma(-b, y1, 1.0), y1, y1);
-    return fma(fma(-b, y2, 1.0), y2, y2);
-#else
-    return 1.0 / b;
-#endif
-}
+GRHD_HD double rcp64(double b) { return div64(1.0, b); }
 
 // a / b, given rb = 1 / b, for divisions that share a divisor or divide by a
 // constant.  An IEEE FP64 division costs a reciprocal and about ten dependent
@@ -776,109 +766,25 @@ GRHD_HD bool face_frame(const R g[3][3], R cof[3][3], R &gdd)
     return true;
 }
 
-// calc_srhd_local_state (fluxes.f90:509-568).
-template <typename R>
-GRHD_HD bool srhd_local(R g1, R gam, const R ql[NEQ], R U[NEQ], R F[NEQ], R &wmin, R &wmax)
-{
-    const R vmax = (R)(1.0 - 1.0e-5);
-    for (int e = 0; e < NEQ; e++) {
-        U[e] = 0;
-        F[e] = 0;
-    }
-    wmin = 0;
-    wmax = 0;
-    const R rho = ql[EQ_DE], pr = ql[EQ_PR];
-    const R v0 = ql[EQ_VX], v1 = ql[EQ_VY], v2 = ql[EQ_VZ];
-    if (!finite(rho) || !finite(pr) || !finite(v0) || !finite(v1) || !finite(v2) || rho <= (R)0 ||
-        pr < (R)0)
-        return false;
-    const R vsq = v0 * v0 + v1 * v1 + v2 * v2;
-    if (!finite(vsq) || vsq < (R)0) return false;
-    if (vsq >= vmax) return false;
-    const R h = (R)1 + rdiv(g1 * pr, rho);
-    if (!finite(h) || h <= (R)0) return false;
-    const R cs2 = rdiv(gam * pr, rho * h);
-    if (!finite(cs2) || cs2 < (R)0 || cs2 >= (R)1) return false;
-    const R W = rrsqrt((R)1 - vsq);
-    const R D = rho * W;
-    const R tau = rho * h * (W * W) - pr - D;
-    const R m0 = rho * h * (W * W) * v0, m1 = rho * h * (W * W) * v1, m2 = rho * h * (W * W) * v2;
-    U[EQ_DE] = D;
-    U[EQ_PR] = tau;
-    U[EQ_VX] = m0;
-    U[EQ_VY] = m1;
-    U[EQ_VZ] = m2;
-    F[EQ_DE] = D * v0;
-    F[EQ_PR] = m0 - D * v0;
-    F[EQ_VX] = m0 * v0 + pr;
-    F[EQ_VY] = m1 * v0;
-    F[EQ_VZ] = m2 * v0;
-    const R den = (R)1 - vsq * cs2;
-    const R rad = ((R)1 - vsq) * ((R)1 - vsq * cs2 - (v0 * v0) * ((R)1 - cs2));
-    if (den <= (R)1.0e-14 || rad < (R)-1.0e-12) return false;
-    const R ac = sqrt(cs2) * sqrt(rmax((R)0, rad));
-    const R rden = rcp(den);
-    wmin = quot(v0 * ((R)1 - cs2) - ac, den, rden);
-    wmax = quot(v0 * ((R)1 - cs2) + ac, den, rden);
-    for (int e = 0; e < NEQ; e++)
-        if (!finite(U[e]) || !finite(F[e])) return false;
-    if (!finite(wmin) || !finite(wmax)) return false;
-    return true;
-}
-
-// conservative_state_is_admissible (fluxes.f90:571-591).
-template <typename R>
-GRHD_HD bool admissible(const R U[NEQ])
-{
-    for (int e = 0; e < NEQ; e++)
-        if (!finite(U[e])) return false;
-    if (U[EQ_DE] <= (R)0) return false;
-    const R E = U[EQ_PR] + U[EQ_DE];
-    const R msq = U[EQ_VX] * U[EQ_VX] + U[EQ_VY] * U[EQ_VY] + U[EQ_VZ] * U[EQ_VZ];
-    const R scale = rmax(rmax(Lim<R>::tiny(), E * E), U[EQ_DE] * U[EQ_DE] + msq);
-    if (E <= (R)0) return false;
-    if (E * E - U[EQ_DE] * U[EQ_DE] - msq <= (R)1.0e-13 * scale) return false;
-    return true;
-}
-
-// local_flux_to_coordinate (fluxes.f90:596-624).
-template <typename R, typename FG>
-GRHD_HD bool local_to_coord(const R Ul[NEQ], const R Fl[NEQ], R w, R alpha, R sqg, R gdd,
-                            const FG &fg, R F[NEQ])
-{
-    R mf[NEQ];
-    for (int e = 0; e < NEQ; e++) {
-        F[e] = 0;
-        mf[e] = Fl[e] - w * Ul[e];
-    }
-    const R fac = sqg * alpha * sqrt(gdd);
-    if (!finite(fac)) return false;
-    for (int e = 0; e < NEQ; e++)
-        if (!finite(mf[e])) return false;
-    F[EQ_DE] = fac * mf[EQ_DE];
-    F[EQ_PR] = fac * mf[EQ_PR];
-    for (int i = 0; i < 3; i++)
-        for (int a = 0; a < 3; a++) F[EQ_VX + i] = F[EQ_VX + i] + fac * fg.cof(a, i) * mf[EQ_VX + a];
-    for (int e = 0; e < NEQ; e++)
-        if (!finite(F[e])) return false;
-    return true;
-}
-
-// The same HLLC solver without early exits, for FP32 faces: every quantity of
-// the reference's path is evaluated unconditionally, in the same operations
-// and order, and each fallback test clears a validity flag instead of
-// returning.  A rejected face may then compute values from a degenerate state
-// (infinities, NaN; no traps on the device), which the flag discards; an
-// accepted face gets the branching form's result bit for bit.  The face
-// kernel becomes one long basic block the compiler schedules across: 2 %
-// faster mixed stage on the 3D torus.  FP64 faces keep the branching form,
-// which needs fewer registers: the flat one spills 400 B per lane at 128
+// The HLLC helpers below evaluate every quantity of the reference's path
+// unconditionally, in the reference's operations and order, and each of its
+// fallback tests clears a validity flag instead of returning.  A rejected
+// state may then compute values from a degenerate input (infinities, NaN; no
+// traps on the device), which the flag discards; an accepted one gets the
+// branching form's result bit for bit.  Each helper is one basic block the
+// compiler schedules across instead of up to eight short ones: +5 % on the
+// FP64 stage of the 3D torus.
+//
+// hllc_flat, the solver itself in the same form (used for FP32 faces: SALU
+// instructions per wave halve and the mixed stage measured 2 % faster).  FP64
+// faces keep the solver's own early returns (hllc_try): the flat solver holds
+// both sides' star states and more at once, spills 400 B per lane at 128
 // VGPRs and halved the FP64 stage rate.
 
 // calc_srhd_local_state (fluxes.f90:509-568): U, F and the speeds of the
 // local state; returns the reference's validity.
 template <typename R>
-GRHD_HD bool srhd_local_flat(R g1, R gam, const R ql[NEQ], R U[NEQ], R F[NEQ], R &wmin, R &wmax)
+GRHD_HD bool srhd_local(R g1, R gam, const R ql[NEQ], R U[NEQ], R F[NEQ], R &wmin, R &wmax)
 {
     const R vmax = (R)(1.0 - 1.0e-5);
     const R rho = ql[EQ_DE], pr = ql[EQ_PR];
@@ -918,7 +824,7 @@ GRHD_HD bool srhd_local_flat(R g1, R gam, const R ql[NEQ], R U[NEQ], R F[NEQ], R
 
 // conservative_state_is_admissible (fluxes.f90:571-591).
 template <typename R>
-GRHD_HD bool admissible_flat(const R U[NEQ])
+GRHD_HD bool admissible(const R U[NEQ])
 {
     bool ok = true;
 #pragma unroll
@@ -931,7 +837,7 @@ GRHD_HD bool admissible_flat(const R U[NEQ])
 
 // local_flux_to_coordinate (fluxes.f90:596-624).
 template <typename R, typename FG>
-GRHD_HD bool local_to_coord_flat(const R Ul[NEQ], const R Fl[NEQ], R w, R alpha, R sqg, R gdd,
+GRHD_HD bool local_to_coord(const R Ul[NEQ], const R Fl[NEQ], R w, R alpha, R sqg, R gdd,
                             const FG &fg, R F[NEQ])
 {
     R mf[NEQ];
@@ -982,8 +888,8 @@ GRHD_HD bool hllc_flat(R g1, R gam, const R qL[NEQ], const R qR[NEQ], const FG &
             qRh[EQ_VX + a] = qRh[EQ_VX + a] + f.cof(a, i) * qR[EQ_VX + i];
         }
     }
-    ok = ok & srhd_local_flat(g1, gam, qLh, uL, fL, lLmin, lLmax);
-    ok = ok & srhd_local_flat(g1, gam, qRh, uR, fR, lRmin, lRmax);
+    ok = ok & srhd_local(g1, gam, qLh, uL, fL, lLmin, lLmax);
+    ok = ok & srhd_local(g1, gam, qRh, uR, fR, lRmin, lRmax);
     const R ll = rmin(lLmin, lRmin), lr = rmax(lLmax, lRmax);
     const R w = f.wface();
     ok = ok & finite(w) & !(lr - ll <= (R)1.0e-13);
@@ -1061,7 +967,7 @@ GRHD_HD bool hllc_flat(R g1, R gam, const R qL[NEQ], const R qR[NEQ], const FG &
             uo[e] = quot(lo * uk - fk + ps * pv, dso, rdso);
             okO = okO & finite(fk + lo * (uo[e] - uk));
         }
-        okO = okO & admissible_flat(uo);
+        okO = okO & admissible(uo);
     }
     const R lk = starL ? ll : lr, dsk = lk - ls, rdsk = rcp(dsk);
     R us[NEQ], fs[NEQ];
@@ -1073,7 +979,7 @@ GRHD_HD bool hllc_flat(R g1, R gam, const R qL[NEQ], const R qR[NEQ], const FG &
         fs[e] = fk + lk * (us[e] - uk);
         okS = okS & finite(fs[e]);
     }
-    okS = okS & okO & admissible_flat(us);
+    okS = okS & okO & admissible(us);
     // the state the face speed falls in, back to coordinates
     R Us[NEQ], Fs[NEQ];
 #pragma unroll
@@ -1081,7 +987,7 @@ GRHD_HD bool hllc_flat(R g1, R gam, const R qL[NEQ], const R qR[NEQ], const FG &
         Us[e] = outL ? uL[e] : outR ? uR[e] : us[e];
         Fs[e] = outL ? fL[e] : outR ? fR[e] : fs[e];
     }
-    const bool okT = local_to_coord_flat(Us, Fs, w, alpha, sqg, gdd, f, F);
+    const bool okT = local_to_coord(Us, Fs, w, alpha, sqg, gdd, f, F);
     return ok & (outL | outR | okS) & okT;
 }
 
@@ -1192,7 +1098,7 @@ GRHD_HD bool hllc_try(R g1, R gam, const R qL[NEQ], const R qR[NEQ], const FG &f
         fsL[e] = fL[e] + ll * (usL[e] - uL[e]);
         fsR[e] = fR[e] + lr * (usR[e] - uR[e]);
     }
-    if (!admissible(usL) || !admissible(usR)) return false;
+    if (!(admissible(usL) & admissible(usR))) return false;
     for (int e = 0; e < NEQ; e++)
         if (!finite(fsL[e]) || !finite(fsR[e])) return false;
     if (w <= ls) return local_to_coord(usL, fsL, w, alpha, sqg, gdd, f, F);
@@ -1489,7 +1395,13 @@ GRHD_HD double root_newton(double g1, double rho_floor, double D, double tau, do
     const double wg = (sqrt(Q) + D) / rmax(D, rho_floor);
     const double x2 = rmax(1.0e3, wg * 2.0);
     double xl, xh, f, df, flo, fhi, dx, dxold, root, tmp;
-    eval_res(g1, x1, D, tau, Q, flo, df);
+    // The residual at the lower bracket W = 1: eval_res's operations with
+    // W = 1 (multiplications by 1 are exact, 1 - 1/1^2 = 0), which leaves
+    // f(1) = V(1)^2 0 - Q, the same value in every case (NaN when V(1) or its
+    // square is not finite), without dV and the derivative, which the solve
+    // overwrites before use.
+    const double V1 = div64(g1 * (tau + D) - D, g1 - 1.0);
+    flo = (V1 * V1) * 0.0 - Q;
     eval_res(g1, x2, D, tau, Q, fhi, df);
     if (fabs(flo) < 1.0e-14) return x1;
     if (fabs(fhi) < 1.0e-14) return x2;
