@@ -122,6 +122,7 @@ _SIGS = {
     "grhd_last_timing": ([ctypes.c_void_p, ctypes.POINTER(_D), ctypes.POINTER(ctypes.c_int64)],
                          ctypes.c_int),
     "grhd_set_stage_form": ([ctypes.c_void_p, _I], ctypes.c_int),
+    "grhd_set_face_form": ([ctypes.c_void_p, _I], ctypes.c_int),
     "grhd_footprint": ([ctypes.POINTER(Config), ctypes.POINTER(ctypes.c_uint64)], ctypes.c_int),
     "grhd_step_noalloc": ([ctypes.POINTER(Config), ctypes.c_void_p, ctypes.c_void_p, _D, _I, ctypes.c_void_p],
                           ctypes.c_int),

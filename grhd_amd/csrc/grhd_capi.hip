@@ -349,16 +349,23 @@ int tune_faces(grhd_ctx *c, double dt)
 {
     c->faces_tuned = true;
     if (single_stage_fused(c->P)) return GRHD_OK;
-    static const int cand[] = {4, 2, 8, 1, 0};
+    static const int cand[] = {4, 2, 8, 1, 0, kFaceTableDefer};
     for (int d = 0; d < 3; d++) {
         if (!(d == 0 || (d == 1 && c->P.ny > 1) || (d == 2 && c->P.nz > 1))) continue;
         float best = 0.0f;
         int best_gw = 4;
         for (int gw : cand) {
-            if (gw == 0 && !c->geo[d]) continue;
+            const bool table = (gw == 0 || gw == kFaceTableDefer);
+            if (table && !c->geo[d]) continue;
+            if (gw == kFaceTableDefer && !c->defer_list[d]) continue;
+            // a forced form: only that candidate (its table filled by the
+            // warm-up launch)
+            if (c->forced_face >= 0 && gw != c->forced_face) continue;
             Params P = c->P;
             P.face_gw[d] = gw;
-            P.geo_tab[d] = (gw == 0) ? c->geo[d] : nullptr;
+            P.geo_tab[d] = table ? c->geo[d] : nullptr;
+            P.geo_ready = c->geo_filled[d] ? 1 : 0;  // the warm-up launch fills the table
+            if (table) c->geo_filled[d] = true;
             for (int rep = 0; rep < 4; rep++) {
                 if (rep == 1) HIPCHK(c, hipEventRecord(c->ev0, c->stream));
                 if (c->mixed) launch_face_dir_f32(P, c->stream, c->p, c->F, c->fstr, d + 1);
@@ -370,7 +377,10 @@ int tune_faces(grhd_ctx *c, double dt)
             HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
             if (trace_tune()) {
                 std::fprintf(stderr, "grhd tune: direction %d, %s %d: %.4f ms per launch\n", d + 1,
-                             gw ? "waves per geometry group" : "geometry table", gw, ms / 3.0);
+                             gw == kFaceTableDefer ? "geometry table, deferred faces"
+                             : gw             ? "waves per geometry group"
+                                              : "geometry table",
+                             gw, ms / 3.0);
                 std::fflush(stderr);
             }
             if (best == 0.0f || ms < best) {
@@ -379,13 +389,18 @@ int tune_faces(grhd_ctx *c, double dt)
             }
         }
         c->P.face_gw[d] = best_gw;
-        c->P.geo_tab[d] = (best_gw == 0) ? c->geo[d] : nullptr;
+        c->P.geo_tab[d] = (best_gw == 0 || best_gw == kFaceTableDefer) ? c->geo[d] : nullptr;
     }
     HIPCHK(c, hipGetLastError());
     const int rc = tune_stage_form(c, dt);
     if (rc != GRHD_OK) return rc;
-    // every active direction timed its table form, which filled its table
-    c->P.geo_ready = 1;
+    // the stage skips the geometry kernels once every active direction's
+    // table is filled (a forced form other than the table forms leaves them
+    // unfilled, and the z-march form then fills the z table every stage)
+    bool filled = true;
+    for (int d = 0; d < 3; d++)
+        if ((d == 0 || (d == 1 && c->P.ny > 1) || (d == 2 && c->P.nz > 1)) && !c->geo_filled[d]) filled = false;
+    c->P.geo_ready = filled ? 1 : 0;
     return GRHD_OK;
 }
 
@@ -483,8 +498,8 @@ int run_steps(grhd_ctx *c, const double *dts, int n)
         HIPCHK(c, hipMemcpyAsync(c->p_alt, c->p, (size_t)P.vstride * NEQ * sizeof(double), hipMemcpyDeviceToDevice,
                                  c->stream));
     // launches per stage: the face kernels of each active direction (and the
-    // geometry-table kernel of a direction in the table form, until the
-    // tables are filled), the cell kernel (z-march: the z faces with it) and
+    // geometry-table kernel of a direction in the table forms, until the
+    // tables are filled, and the deferred-face kernel), the cell kernel (z-march: the z faces with it) and
     // the ghost fill
     int per_stage = 3;  // fused form: all faces, cells, ghosts
     if (!single_stage_fused(P)) {
@@ -492,7 +507,8 @@ int run_steps(grhd_ctx *c, const double *dts, int n)
         for (int d = 0; d < 3; d++) {
             const bool on = (d == 0) || (d == 1 && P.ny > 1) || (d == 2 && P.nz > 1);
             if (!on || (zmarch && d == 2)) continue;
-            per_stage += 1 + (P.face_gw[d] == 0 && !P.geo_ready);
+            const bool table = (P.face_gw[d] == 0 || P.face_gw[d] == kFaceTableDefer);
+            per_stage += 1 + (table && !P.geo_ready) + (P.face_gw[d] == kFaceTableDefer && P.defer_list[d]);
         }
         if (zmarch && !P.geo_ready) per_stage += 1;
     }
@@ -538,6 +554,8 @@ uint64_t ctx_footprint(const grhd_config &cfg)
         for (int d = 0; d < 3; d++)
             if (d == 0 || (d == 1 && P.ny > 1) || (d == 2 && P.nz > 1))
                 b += (uint64_t)NFG * (P.nx + (d == 0)) * (P.ny + (d == 1)) * rsz;
+    if (defer_faces(P))  // deferred-face lists and counters
+        for (int d = 0; d < 3; d++) b += (nf[d] > 1 ? nf[d] : 0) * sizeof(unsigned) + 2 * sizeof(unsigned);
     b += (uint64_t)NEQ * P.ng * P.ny * P.nz * sizeof(double) + 8;
     b += (uint64_t)std::max<int64_t>((((int64_t)P.nx * P.ny + 255) / 256) * 4, std::max<int64_t>(4096, P.nz)) *
              sizeof(double) +
@@ -628,6 +646,20 @@ int grhd_create(const grhd_config *cfg, grhd_ctx **out)
             if ((e = hipMalloc(&c->geo[d], (size_t)NFG * nab * rsz)) != hipSuccess) return bail(e, "hipMalloc geometry");
         }
     }
+    if (defer_faces(P)) {
+        for (int d = 0; d < 3; d++) {
+            const bool active = (d == 0) || (d == 1 && P.ny > 1) || (d == 2 && P.nz > 1);
+            if (!active) continue;
+            if ((e = hipMalloc(&c->defer_list[d], (size_t)c->fstr[d] * sizeof(unsigned))) != hipSuccess)
+                return bail(e, "hipMalloc deferred faces");
+            c->P.defer_list[d] = c->defer_list[d];
+        }
+        if ((e = hipMalloc(&c->defer_ctr, 6 * sizeof(unsigned))) != hipSuccess)
+            return bail(e, "hipMalloc deferred-face counters");
+        if ((e = hipMemsetAsync(c->defer_ctr, 0, 6 * sizeof(unsigned), c->stream)) != hipSuccess)
+            return bail(e, "hipMemset");
+        c->P.defer_ctr = c->defer_ctr;
+    }
     if ((e = hipMalloc(&c->injector, (size_t)NEQ * P.ng * P.ny * P.nz * sizeof(double) + 8)) != hipSuccess)
         return bail(e, "hipMalloc injector");
     c->partial_n = (int)(((int64_t)P.nx * P.ny + 255) / 256);
@@ -669,7 +701,9 @@ int grhd_destroy(grhd_ctx *c)
     for (int d = 0; d < 3; d++) {
         (void)hipFree(c->F[d]);
         (void)hipFree(c->geo[d]);
+        (void)hipFree(c->defer_list[d]);
     }
+    (void)hipFree(c->defer_ctr);
     (void)hipFree(c->injector);
     (void)hipFree(c->partial);
     (void)hipFree(c->counters);
@@ -1144,6 +1178,19 @@ int grhd_set_stage_form(grhd_ctx *c, int32_t zseg)
         return fail(c, GRHD_ERR_UNSUPPORTED, "the z-march form needs a 3D grid above 65,536 cells");
     c->forced_zseg = zseg;
     c->faces_tuned = false;  // applied (or timed) before the next step
+    return GRHD_OK;
+}
+
+int grhd_set_face_form(grhd_ctx *c, int32_t form)
+{
+    if (!c || !(form == -1 || form == 0 || form == 1 || form == 2 || form == 4 || form == 8 || form == kFaceTableDefer))
+        return GRHD_ERR_INVALID_ARGUMENT;
+    if ((form == 0 || form == kFaceTableDefer) && !c->geo[0])
+        return fail(c, GRHD_ERR_UNSUPPORTED, "the table forms need a grid outside the fused stage form");
+    if (form == kFaceTableDefer && !c->defer_list[0])
+        return fail(c, GRHD_ERR_UNSUPPORTED, "deferred faces need the compiled MP5 + sensor + HLLC scheme");
+    c->forced_face = form;
+    c->faces_tuned = false;  // applied before the next step
     return GRHD_OK;
 }
 

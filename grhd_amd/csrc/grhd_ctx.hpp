@@ -16,6 +16,7 @@ struct grhd_ctx {
     bool mixed = false;
     bool faces_tuned = false;  // face-kernel tiling chosen (tune_faces)
     int forced_zseg = -1;      // grhd_set_stage_form: -1 timed choice, else P.zseg
+    int forced_face = -1;      // grhd_set_face_form: -1 timed choice, else P.face_gw of every direction
     double *p = nullptr;      // ghosted primitives, SoA [5][NZs][NYs][NXs]
     // second primitive array of the z-march stage form (P.zseg > 0), which
     // reads one and writes the other; the two are swapped after each stage,
@@ -25,7 +26,11 @@ struct grhd_ctx {
     int ucur = 0;             // u[ucur] holds U^n
     void *F[3] = {nullptr, nullptr, nullptr};
     void *geo[3] = {nullptr, nullptr, nullptr};  // per-stage face-geometry tables (tuner's table form)
+    bool geo_filled[3] = {false, false, false};  // geo[d] holds the problem's face geometry
     int64_t fstr[3] = {0, 0, 0};
+    // deferred-face lists and their counters (Params::defer_list, defer_faces)
+    unsigned *defer_list[3] = {nullptr, nullptr, nullptr};
+    unsigned *defer_ctr = nullptr;
     double *injector = nullptr;
     double *partial = nullptr;
     int partial_n = 0;        // CFL partials (4 doubles each)

@@ -47,6 +47,17 @@ inline bool single_stage_fused(const Params &P)
     return n <= kFusedStageCells || ((P.ny == 1 || P.nz == 1) && n <= kFusedStage2DCells);
 }
 
+// The single-problem face kernels can defer their HLLC fallbacks to a second
+// kernel (Params::defer_list): on grids with face-geometry tables, for the
+// compiled MP5 + sensor + HLLC scheme on the Kerr-Schild metrics, with face
+// indices that fit 32 bits.
+inline bool defer_faces(const Params &P)
+{
+    const int64_t nmax = (int64_t)(P.nx + 1) * (P.ny + 1) * (P.nz + 1);
+    return !single_stage_fused(P) && (P.mk == MK_KS || P.mk == MK_KS_LOG) && P.rec == 4 && P.sensor != 0 &&
+           P.rs == 2 && nmax < ((int64_t)1 << 32);
+}
+
 // The face kernel of direction dir (1..3) alone (face-tiling tuner).
 void launch_face_dir_f64(const Params &P, hipStream_t s, const double *p, void *const F[3], const int64_t fstr[3],
                          int dir);

@@ -35,6 +35,9 @@ enum MetricKind : int {
     MK_KS_LOG = 7,
 };
 
+// Params::face_gw of the geometry-table face form with deferred faces.
+constexpr int kFaceTableDefer = 16;
+
 // Problem constants broadcast to every kernel (passed by value).
 struct Params {
     int problem;
@@ -48,7 +51,8 @@ struct Params {
     int open_lo, open_hi;
     // waves of a face kernel sharing one LDS copy of the face geometry, per
     // direction (1, 2, 4 or 8), or 0: the geometry evaluated once per stage
-    // into geo_tab[d] by its own kernel (no barrier in the face kernel).  The
+    // into geo_tab[d] by its own kernel (no barrier in the face kernel), or
+    // kFaceTableDefer: the table form with deferred faces (defer_list).  The
     // single-problem launchers' tiling, chosen by timing the face kernels on
     // the problem (tune_faces).
     int face_gw[3];
@@ -57,6 +61,14 @@ struct Params {
     // metric is static: filled once, when the single-problem tuner timed the
     // table form), so the stage skips the kernels that evaluate them
     int geo_ready;
+    // Deferred faces of the table-form face kernels (compiled MP5 + sensor +
+    // HLLC scheme): a face whose HLLC solve falls back to HLLE is appended to
+    // defer_list[d] (face indices, capacity = the direction's face count) and
+    // computed by a second kernel over the list.  defer_ctr[2 d] counts the
+    // entries and defer_ctr[2 d + 1] the second kernel's finished blocks; its
+    // last block clears both.  Null: every face is computed where it lies.
+    unsigned *defer_list[3];
+    unsigned *defer_ctr;
     // z-march segment of the single-problem stage: each lane walks this many
     // cells of one (x, y) column, computing the z faces in order and
     // differencing them in registers (k_zcell); 0: separate z-face and cell
@@ -1216,9 +1228,42 @@ constexpr Scheme kSchemes[] = {{0, 0, 0}, {4, 1, 2}, {5, 0, 1}};
 // ghost depth of a reconstruction (initialization.f90:27-52)
 GRHD_HD constexpr int ghost_depth(int rec) { return rec == 1 ? 1 : (rec == 2 || rec == 3) ? 2 : 3; }
 
-// One face of reconstruct_1d_core (reconstruction.f90:285-657).
-// s[v][o + 2] holds the primitive v at offset o = -2..3 from the face's left
-// cell.  i is the face index (0..nodes) along direction dir (1..3).
+// The shock sensor's choice for one face (reconstruction.f90:292-310, 331-520): the
+// relative pressure and density jumps over the five cells around the face
+// select Godunov, TVD-MC, WENO3 or the base scheme.  s[v][o + 2] holds the
+// primitive v at offset o = -2..3 from the face's left cell; i is the face
+// index (0..nodes) along direction dir (1..3).
+struct SensorPick {
+    bool god, tvdmc, w3, mp5w3;
+};
+template <typename R>
+GRHD_HD SensorPick sensor_pick(const Params &P, const R s[NEQ][6], int i, int nodes, int dir)
+{
+#define S_(v, o) s[v][(o) + 2]
+    const bool mink = (P.mk <= MK_MINK_SPH_LOG);
+    const R rf = (R)P.rho_floor, pf = (R)P.p_floor;
+    const bool radial0 = (!mink && dir == 1 && i <= 0);
+    R pmax = S_(EQ_PR, -2), pmin = S_(EQ_PR, -2), rmx = S_(EQ_DE, -2), rmn = S_(EQ_DE, -2);
+    for (int o = -1; o <= 2; o++) {
+        pmax = rmax(pmax, S_(EQ_PR, o));
+        pmin = rmin(pmin, S_(EQ_PR, o));
+        rmx = rmax(rmx, S_(EQ_DE, o));
+        rmn = rmin(rmn, S_(EQ_DE, o));
+    }
+    const R pj = (pmax > (R)1.0e3 * pf) ? rdiv(pmax - pmin, rmax(pmin, pf)) : (R)0;
+    const R rj = (rmx > (R)1.0e3 * rf) ? rdiv(rmx - rmn, rmax(rmn, rf)) : (R)0;
+    SensorPick sp;
+    sp.god = radial0 || S_(EQ_DE, 0) <= (R)1.0e2 * rf || S_(EQ_DE, 1) <= (R)1.0e2 * rf || pj >= (R)10 ||
+             rj >= (R)10;
+    sp.tvdmc = (pj >= (R)0.80 || rj >= (R)0.50);
+    sp.w3 = (pj >= (R)0.40 || rj >= (R)0.15);
+    sp.mp5w3 = sp.w3 || (!mink && dir == 1 && i <= 3) || (!mink && dir == 2 && (i <= 2 || i >= nodes - 2));
+    return sp;
+#undef S_
+}
+
+// One face of reconstruct_1d_core (reconstruction.f90:285-657), with the
+// stencil s and face index i of sensor_pick.
 template <typename R, int SCH = 0>
 GRHD_HD void recon_face(const Params &P, const R s[NEQ][6], int i, int nodes, int dir,
                         R qL[NEQ], R qR[NEQ])
@@ -1236,21 +1281,8 @@ GRHD_HD void recon_face(const Params &P, const R s[NEQ][6], int i, int nodes, in
             qR[v] = S_(v, 1);
         }
     } else if (sensor) {
-        R pmax = S_(EQ_PR, -2), pmin = S_(EQ_PR, -2), rmx = S_(EQ_DE, -2), rmn = S_(EQ_DE, -2);
-        for (int o = -1; o <= 2; o++) {
-            pmax = rmax(pmax, S_(EQ_PR, o));
-            pmin = rmin(pmin, S_(EQ_PR, o));
-            rmx = rmax(rmx, S_(EQ_DE, o));
-            rmn = rmin(rmn, S_(EQ_DE, o));
-        }
-        const R pj = (pmax > (R)1.0e3 * pf) ? rdiv(pmax - pmin, rmax(pmin, pf)) : (R)0;
-        const R rj = (rmx > (R)1.0e3 * rf) ? rdiv(rmx - rmn, rmax(rmn, rf)) : (R)0;
-        const bool god = radial0 || S_(EQ_DE, 0) <= (R)1.0e2 * rf || S_(EQ_DE, 1) <= (R)1.0e2 * rf ||
-                         pj >= (R)10 || rj >= (R)10;
-        const bool tvdmc = (pj >= (R)0.80 || rj >= (R)0.50);
-        const bool w3 = (pj >= (R)0.40 || rj >= (R)0.15);
-        const bool mp5w3 = w3 || (!mink && dir == 1 && i <= 3) ||
-                           (!mink && dir == 2 && (i <= 2 || i >= nodes - 2));
+        const SensorPick sp = sensor_pick<R>(P, s, i, nodes, dir);
+        const bool god = sp.god, tvdmc = sp.tvdmc, w3 = sp.w3, mp5w3 = sp.mp5w3;
 #pragma unroll
         for (int v = 0; v < NEQ; v++) {
             if (god) {

@@ -77,13 +77,12 @@ __device__ __forceinline__ void face_geometry(const Params &P, int i, int j, R *
     col[FG_FLAGS * stride] = (R)flags;
 }
 
-// Face states q_L, q_R of face fi along DIR (cells fi-2 .. fi+3 of row
-// (i, j, k)): stencil loads and reconstruct_1d_core.
+// The stencil of face fi along DIR (cells fi-2 .. fi+3 of row (i, j, k)):
+// s[v][o + 2] = primitive v at offset o from the face's left cell.
 template <int MK, int DIR, typename R, int SCH = 0>
-__device__ __forceinline__ void face_states(const Params &P, const double *__restrict__ p, int i, int j, int k,
-                                            int fi, int nodes, R qL[NEQ], R qR[NEQ])
+__device__ __forceinline__ void load_stencil(const Params &P, const double *__restrict__ p, int i, int j, int k,
+                                             int fi, R s[NEQ][6])
 {
-    R s[NEQ][6];
     const int ng = SCH ? ghost_depth(kSchemes[SCH].rec) : P.ng;
     const int lo = (ng >= 2) ? -2 : 0, hi = (ng >= 3) ? 3 : (ng == 2 ? 2 : 1);
     const int64_t step = (DIR == 1) ? 1 : (DIR == 2) ? (int64_t)P.NXs : (int64_t)P.NXs * P.NYs;
@@ -100,6 +99,16 @@ __device__ __forceinline__ void face_states(const Params &P, const double *__res
             s[v][o + 2] = in ? x : (R)0;
         }
     }
+}
+
+// Face states q_L, q_R of face fi along DIR: stencil loads and
+// reconstruct_1d_core.
+template <int MK, int DIR, typename R, int SCH = 0>
+__device__ __forceinline__ void face_states(const Params &P, const double *__restrict__ p, int i, int j, int k,
+                                            int fi, int nodes, R qL[NEQ], R qR[NEQ])
+{
+    R s[NEQ][6];
+    load_stencil<MK, DIR, R, SCH>(P, p, i, j, k, fi, s);
     recon_face<R, SCH>(P, s, fi, nodes, DIR, qL, qR);
 }
 
@@ -132,6 +141,48 @@ __device__ __forceinline__ void flux_face(const Params &P, const double *__restr
         face_states<MK, DIR, R, SCH>(P, pr, i, j, k, fi, nodes, qL, qR);
     }
     hlle<R, DIR, kSRHD<MK>>((R)P.g1, (R)P.gam, qL, qR, f, out);
+}
+
+// The faces of the compiled MP5 + sensor + HLLC scheme (SCH 1) whose HLLC
+// solve succeeds (all but 19 of the 1.6 million faces of the 3D torus).
+// Returns false, with out unset, where HLLC falls back to HLLE: those faces
+// are computed by flux_face in a second kernel over a list of them
+// (k_flux_defer), so the face kernel carries neither the fallback's second
+// reconstruction and HLLE code nor the registers they need (FP64: no spill at
+// 128 VGPRs, against 52 B per lane with the fallback inline; the z faces of
+// the 3D torus 11 % and the x faces 5 % faster).  A face computed here gets
+// flux_face's result bit for bit: the same stencil, reconstruction and
+// solver.  Deferring the sensor's TVD-MC and WENO3 faces too (8-14 % of the
+// x and y faces of the 3D torus) measured slower: the list is then long and
+// its one counter contended.
+template <int MK, int DIR, typename R, typename FG>
+__device__ __forceinline__ bool flux_face_hot(const Params &P, const double *__restrict__ p, const FG &f, int i,
+                                              int j, int k, int fi, int nodes, R out[NEQ])
+{
+    static_assert(kSchemes[1].rec == 4 && kSchemes[1].sensor && kSchemes[1].rs == 2, "scheme 1");
+    R qL[NEQ], qR[NEQ];
+    face_states<MK, DIR, R, 1>(P, p, i, j, k, fi, nodes, qL, qR);
+    return hllc_try<R, DIR>((R)P.g1, (R)P.gam, qL, qR, f, out);
+}
+
+// Appends the calling lanes' faces with defer set to list (one atomic per
+// wave: the wave's count is added once, and each lane writes at the counter's
+// old value plus the number of deferring lanes below it).  Every active lane
+// of the wave must call it.
+__device__ __forceinline__ void defer_push(unsigned *__restrict__ ctr, unsigned *__restrict__ list, unsigned cap,
+                                           unsigned face, bool defer)
+{
+    const unsigned long long m = __ballot(defer);
+    if (m == 0) return;
+    const int lane = threadIdx.x & 63;
+    const int leader = __ffsll((long long)m) - 1;
+    unsigned base = 0;
+    if (lane == leader) base = atomicAdd(ctr, (unsigned)__popcll(m));
+    base = __shfl(base, leader);
+    if (defer) {
+        const unsigned pos = base + (unsigned)__popcll(m & ((1ull << lane) - 1ull));
+        if (pos < cap) list[pos] = face;
+    }
 }
 
 // Tiling of the face kernels: a tile is 64 consecutive faces of one (x, y)
@@ -237,11 +288,14 @@ __global__ __launch_bounds__(256) void k_geo(Params P, R *__restrict__ tab)
 
 // Faces of direction DIR reading their geometry from the stage's table: the
 // blocks, lanes and planes of flux_body with groups of four, with neither the
-// geometry evaluation nor the barrier.
-template <int MK, int DIR, typename R, int SCH>
+// geometry evaluation nor the barrier.  DEFER (scheme 1 only): faces whose
+// HLLC solve succeeds here (flux_face_hot), the HLLC fallbacks appended to
+// the direction's list for k_flux_defer.
+template <int MK, int DIR, typename R, int SCH, bool DEFER = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_flux_tab(
     Params P, const double *__restrict__ p, R *__restrict__ F, int64_t fstride, const R *__restrict__ tab)
 {
+    static_assert(!DEFER || SCH == 1, "deferred faces: scheme 1 only");
     const int nfx = P.nx + (DIR == 1), nfy = P.ny + (DIR == 2), nfz = P.nz + (DIR == 3);
     const int64_t nab = (int64_t)nfx * nfy;
     const int64_t nchunk = (nab + 63) / 64;
@@ -259,32 +313,89 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
     const int nodes = (DIR == 1) ? P.nx : (DIR == 2) ? P.ny : P.nz;
     const int k = c + (DIR != 3);
     const int fi = (DIR == 1) ? i : (DIR == 2) ? j : k;
+    const int64_t fo = (int64_t)c * nab + ab;
     R out[NEQ];
+    bool done = true;
     if (fg.flags() & FG_ZERO) {
 #pragma unroll
         for (int v = 0; v < NEQ; v++) out[v] = 0;
+    } else if constexpr (DEFER) {
+        done = flux_face_hot<MK, DIR>(P, p, fg, i, j, k, fi, nodes, out);
     } else {
         flux_face<MK, DIR, R, SCH>(P, p, fg, i, j, k, fi, nodes, out);
     }
-    const int64_t fo = (int64_t)c * nab + ab;
+    if constexpr (DEFER) {
+        defer_push(P.defer_ctr + 2 * (DIR - 1), P.defer_list[DIR - 1], (unsigned)(nab * nfz), (unsigned)fo, !done);
+        if (!done) return;
+    }
 #pragma unroll
     for (int v = 0; v < NEQ; v++) F[v * fstride + fo] = out[v];
 }
 
+// The deferred faces of one direction (k_flux_tab with DEFER): each thread
+// takes list entries q = thread, thread + grid, ... below the count and
+// computes the face with flux_face (the full scheme: sensor cascade, HLLC and
+// its HLLE fallback).  The block that finishes last clears the count and the
+// block counter for the next stage: every block has read the count before it
+// adds itself to the block counter.
+template <int MK, int DIR, typename R, int SCH>
+__global__ __launch_bounds__(256) void k_flux_defer(Params P, const double *__restrict__ p, R *__restrict__ F,
+                                                    int64_t fstride, const R *__restrict__ tab)
+{
+    unsigned *ctr = P.defer_ctr + 2 * (DIR - 1);
+    const unsigned *list = P.defer_list[DIR - 1];
+    const int nfx = P.nx + (DIR == 1), nfy = P.ny + (DIR == 2);
+    const int64_t nab = (int64_t)nfx * nfy;
+    const int nodes = (DIR == 1) ? P.nx : (DIR == 2) ? P.ny : P.nz;
+    const unsigned n = __hip_atomic_load(ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (unsigned q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
+        const int64_t fo = list[q];
+        const int c = (int)(fo / nab);
+        const int64_t ab = fo - (int64_t)c * nab;
+        const int a = (int)(ab % nfx), b = (int)(ab / nfx);
+        const int i = a + (DIR != 1), j = b + (DIR != 2), k = c + (DIR != 3);
+        const int fi = (DIR == 1) ? i : (DIR == 2) ? j : k;
+        const FaceGeo<R> fg{tab + ab, nab};
+        R out[NEQ];
+        flux_face<MK, DIR, R, SCH>(P, p, fg, i, j, k, fi, nodes, out);
+#pragma unroll
+        for (int v = 0; v < NEQ; v++) F[v * fstride + fo] = out[v];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned fin = __hip_atomic_fetch_add(ctr + 1, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (fin == gridDim.x - 1) {
+            __hip_atomic_store(ctr, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(ctr + 1, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+}
+
+// Blocks of k_flux_defer: the list is short (HLLC fallbacks); longer lists
+// take more passes.
+constexpr unsigned kDeferBlocks = 256;
+
 template <int MK, int DIR, typename R, int SCH>
 void launch_flux(const Params &P, hipStream_t s, const double *p, R *F, int64_t fstr)
 {
-    if (P.face_gw[DIR - 1] == 0 && P.geo_tab[DIR - 1]) {
+    if ((P.face_gw[DIR - 1] == 0 || P.face_gw[DIR - 1] == kFaceTableDefer) && P.geo_tab[DIR - 1]) {
         R *tab = static_cast<R *>(P.geo_tab[DIR - 1]);
         const int nfx = P.nx + (DIR == 1), nfy = P.ny + (DIR == 2), nfz = P.nz + (DIR == 3);
         const int64_t nab = (int64_t)nfx * nfy;
         if (!P.geo_ready) k_geo<MK, DIR, R><<<blocks_for(nab, 256), 256, 0, s>>>(P, tab);
         const int64_t nb = ((nab + 63) / 64) * ((nfz + 3) / 4);
+        if constexpr (SCH == 1) {
+            if (P.face_gw[DIR - 1] == kFaceTableDefer && P.defer_list[DIR - 1]) {
+                k_flux_tab<MK, DIR, R, SCH, true><<<(unsigned)nb, 256, 0, s>>>(P, p, F, fstr, tab);
+                k_flux_defer<MK, DIR, R, SCH><<<kDeferBlocks, 256, 0, s>>>(P, p, F, fstr, tab);
+                return;
+            }
+        }
         k_flux_tab<MK, DIR, R, SCH><<<(unsigned)nb, 256, 0, s>>>(P, p, F, fstr, tab);
         return;
     }
     const int nfz = P.nz + (DIR == 3);
-    const FluxTiling t = flux_tiling(nfz, P.face_gw[DIR - 1] >= 1 ? P.face_gw[DIR - 1] : 4);
+    const FluxTiling t = flux_tiling(nfz, (P.face_gw[DIR - 1] >= 1 && P.face_gw[DIR - 1] <= 8) ? P.face_gw[DIR - 1] : 4);
     const int waves = t.gw > 4 ? t.gw : 4;
     const size_t lds = (size_t)(waves / t.gw) * NFG * 64 * sizeof(R);
     k_flux<MK, DIR, R, SCH><<<(unsigned)flux_blocks<DIR>(P, t, waves), 64 * waves, lds, s>>>(P, p, F, fstr, t.kt,

@@ -210,6 +210,11 @@ class Solver:
         form with n-cell segments (3D grids above 65,536 cells)."""
         self._chk(self._L.grhd_set_stage_form(self.h, int(zseg)))
 
+    def set_face_form(self, form: int):
+        """Face-kernel form (grhd_set_face_form): -1 timed, 1/2/4/8 LDS
+        geometry groups, 0 geometry table, 16 table with deferred faces."""
+        self._chk(self._L.grhd_set_face_form(self.h, int(form)))
+
     def last_timing(self):
         ms = ctypes.c_double()
         n = ctypes.c_int64()

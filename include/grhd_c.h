@@ -244,6 +244,18 @@ int grhd_last_timing(grhd_ctx *ctx, double *ms, int64_t *launches);
  * bit for bit.  GRHD_ERR_UNSUPPORTED for n > 0 on other grids. */
 int grhd_set_stage_form(grhd_ctx *ctx, int32_t zseg);
 
+/* Face-kernel form of every direction of a problem outside the fused stage
+ * form (no counterpart in the reference, whose face fluxes are computed in
+ * its strip loops, evolution.f90:48-189).  -1 (the default): chosen on the
+ * first step by timing each form on the problem; 1, 2, 4 or 8: that many
+ * waves share an LDS copy of the face geometry; 0: the geometry read from a
+ * per-problem table; 16: the table form with deferred faces -- for the MP5 +
+ * sensor + HLLC scheme on the Kerr-Schild metrics, faces whose HLLC solve
+ * falls back to HLLE are computed by a second kernel over a list of them.  Every form gives the same results bit
+ * for bit.  GRHD_ERR_UNSUPPORTED for 0 or 16 where the problem has no tables
+ * or no deferred-face lists. */
+int grhd_set_face_form(grhd_ctx *ctx, int32_t form);
+
 /* Device bytes grhd_create allocates for cfg: the ghosted primitives, two
  * conserved buffers and the face fluxes (the reference allocates p, u, up and
  * rhs, initialization.f90:211-215, plus its metric cache).  Used to admit

@@ -165,20 +165,24 @@ def _large(name):
     return cfg
 
 
-@pytest.mark.parametrize("name,zseg", [("fm3d_ks_mp5_hllc_sensor", -1), ("fm3d_ks_mp5_hllc_sensor", 0),
-                                       ("fm3d_ks_mp5_hllc_sensor", 16), ("fm3d_ks_weno5_hlle", -1),
-                                       ("fm3d_ks_weno5_hlle", 8)])
-def test_large_grid_one_step_matches_oracle(name, zseg):
+@pytest.mark.parametrize("name,zseg,face", [("fm3d_ks_mp5_hllc_sensor", -1, -1), ("fm3d_ks_mp5_hllc_sensor", 0, -1),
+                                            ("fm3d_ks_mp5_hllc_sensor", 16, -1), ("fm3d_ks_mp5_hllc_sensor", 0, 16),
+                                            ("fm3d_ks_mp5_hllc_sensor", 0, 4), ("fm3d_ks_weno5_hlle", -1, -1),
+                                            ("fm3d_ks_weno5_hlle", 8, -1)])
+def test_large_grid_one_step_matches_oracle(name, zseg, face):
     """Grids above 65,536 cells take the per-direction stage forms that the 3D
     torus benchmark runs, with the face kernels compiled for the scheme (MP5 +
     sensor + HLLC, WENO5 + HLLE): the split form (one face launch per
     direction, then the cell kernel, zseg 0), the z-march form (zseg > 0) and
     the timed choice between them (-1); smaller grids the fused form the other
-    tests cover."""
+    tests cover.  face: the face-kernel form (-1 timed, 4 LDS geometry
+    groups, 16 geometry table with the HLLC fallbacks deferred to a second
+    kernel)."""
     cfg = _large(name)
-    print("large grid one step", name, cfg.nx, cfg.ny, cfg.nz, "stage form", zseg, flush=True)
+    print("large grid one step", name, cfg.nx, cfg.ny, cfg.nz, "stage form", zseg, "face form", face, flush=True)
     with Solver(cfg) as S:
         S.set_stage_form(zseg)
+        S.set_face_form(face)
         O = oracle.Oracle(cfg)
         S.initial_conditions()
         p = seeded_noise(S.get_primitives(), S.nx, S.ny, S.nz, S.ng, 2, 1.0e-2, 3435)
@@ -221,6 +225,73 @@ def test_stage_forms_agree_bit_for_bit(precision):
         assert np.array_equal(out[zseg][0], out[0][0]), zseg
         assert np.array_equal(out[zseg][1], out[0][1]), zseg
         assert out[zseg][2] == out[0][2], zseg
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_face_forms_agree_bit_for_bit(precision):
+    """The face-kernel forms -- four waves sharing an LDS copy of the face
+    geometry (4), the geometry table (0) and the table with deferred faces
+    (16: faces whose HLLC solve falls back to HLLE computed by a second kernel
+    over a list of them) -- give the same
+    state and counters after three steps, bit for bit, in FP64 and with FP32
+    faces, in the split and the z-march stage forms."""
+    cfg = _large("fm3d_ks_mp5_hllc_sensor").replace(precision=precision)
+    print("face forms", cfg.nx, cfg.ny, cfg.nz, "precision", precision, flush=True)
+    out = {}
+    for zseg in (0, 16):
+        for form in (4, 0, 16):
+            with Solver(cfg) as S:
+                S.set_stage_form(zseg)
+                S.set_face_form(form)
+                S.initial_conditions()
+                p = seeded_noise(S.get_primitives(), S.nx, S.ny, S.nz, S.ng, 2, 1.0e-2, 3435)
+                S.set_primitives(p)
+                dt = S.compute_dt()
+                S.step(dt, 2)
+                S.step(dt, 1)
+                out[zseg, form] = (S.get_conserved(), S.get_primitives(), S.counters())
+    for key in out:
+        assert np.array_equal(out[key][0], out[0, 4][0]), key
+        assert np.array_equal(out[key][1], out[0, 4][1]), key
+        assert out[key][2] == out[0, 4][2], key
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_deferred_faces_at_workload_size(precision):
+    """The 3D torus benchmark grid (256 x 128 x 128, theta in [0.05 pi,
+    0.95 pi], 1e-2 density noise): two steps with the geometry table alone
+    and with deferred faces give the same state bit for bit."""
+    import bench
+    from grhd_amd.solver import fishbone_unit_density_k
+    cfg = bench.c4_config().replace(precision=precision)
+    cfg.fm_polytropic_constant = fishbone_unit_density_k(cfg)
+    print("deferred faces 256x128x128 precision", precision, flush=True)
+    out = {}
+    for form in (0, 16):
+        with Solver(cfg) as S:
+            S.set_stage_form(0)
+            S.set_face_form(form)
+            S.initial_conditions()
+            S.perturb(2, 1.0e-2, 3435)
+            S.step(S.compute_dt(), 2)
+            out[form] = (S.get_conserved(), S.counters())
+    assert np.isfinite(out[16][0]).all()
+    assert np.array_equal(out[16][0], out[0][0])
+    assert out[16][1] == out[0][1]
+
+
+def test_face_form_refused_where_unsupported():
+    """The table forms need a grid outside the fused stage form, deferred
+    faces the MP5 + sensor + HLLC scheme."""
+    with Solver(config_from_par(CASES["fm3d_ks_mp5_hllc_sensor"])) as S:
+        with pytest.raises(Exception):
+            S.set_face_form(16)
+        with pytest.raises(Exception):
+            S.set_face_form(3)
+    with Solver(_large("fm3d_ks_weno5_hlle")) as S:
+        with pytest.raises(Exception):
+            S.set_face_form(16)
+        S.set_face_form(0)
 
 
 def test_stage_form_refused_on_small_grids():
