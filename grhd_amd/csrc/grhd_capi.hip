@@ -406,9 +406,10 @@ int tune_faces(grhd_ctx *c, double dt)
 
 // Stage form of a 3D problem, chosen once per context after the face tiling
 // by timing one stage (stage 1, the step's dt) of each candidate: the split
-// form (faces of every direction into their arrays, then k_cell) and the
+// form (faces of every direction into their arrays, then k_cell), the
 // z-march form with 16-cell segments (8-cell segments measured the same on
-// the 3D torus, 32-cell ones 25 % slower: too few waves).  Every form gives
+// the 3D torus, 32-cell ones 25 % slower: too few waves) and the x-fused
+// form (x faces computed by the cell kernel, k_xcell).  Every form gives
 // the same results bit for bit.  The timing runs write only the stage buffer u1, the
 // face arrays and tables, the second primitive array and the tuner's
 // counters: the split form runs on a copy of the primitives in p_alt.
@@ -418,7 +419,7 @@ int tune_stage_form(grhd_ctx *c, double dt)
 {
     const Params &P0 = c->P;
     c->P.zseg = 0;
-    if (!zmarch_possible(P0) || !c->geo[2] || (c->forced_zseg < 0 && P0.nz < 16)) return GRHD_OK;
+    if (!zmarch_possible(P0) || !c->geo[2] || (c->forced_zseg == -1 && P0.nz < 16)) return GRHD_OK;
     const size_t pbytes = (size_t)P0.vstride * NEQ * sizeof(double);
     if (c->forced_zseg == 0) {
         c->P.zseg = 0;
@@ -429,21 +430,22 @@ int tune_stage_form(grhd_ctx *c, double dt)
         if (e != hipSuccess) {  // no room for a second primitive array: split form
             (void)hipGetLastError();
             c->p_alt = nullptr;
-            if (c->forced_zseg > 0) return fail(c, GRHD_ERR_OUT_OF_MEMORY, "second primitive array");
+            if (c->forced_zseg != -1) return fail(c, GRHD_ERR_OUT_OF_MEMORY, "second primitive array");
             return GRHD_OK;
         }
     }
-    if (c->forced_zseg > 0) {
+    if (c->forced_zseg != -1) {  // a forced ping-pong form
         c->P.zseg = c->forced_zseg;
         return GRHD_OK;
     }
     double *u0 = c->u[c->ucur], *u1 = c->u[1 - c->ucur];
     unsigned long long *cnt = c->counters + 4;
-    static const int cand[] = {0, 16};
+    static const int cand[] = {0, 16, kStageXFused};
     float best = 0.0f;
     int best_seg = 0;
     for (int seg : cand) {
         if (seg > P0.nz) continue;
+        void *geo = (seg == kStageXFused) ? c->geo[0] : c->geo[2];
         Params P = P0;
         P.zseg = seg;
         if (seg == 0) HIPCHK(c, hipMemcpyAsync(c->p_alt, c->p, pbytes, hipMemcpyDeviceToDevice, c->stream));
@@ -451,16 +453,15 @@ int tune_stage_form(grhd_ctx *c, double dt)
             if (rep == 1) HIPCHK(c, hipEventRecord(c->ev0, c->stream));
             if (seg == 0) launch_stage(P, c->stream, c->p_alt, c->F, c->fstr, u0, u1, 1, dt, cnt, c->mixed,
                                        c->injector, false);
-            else launch_stage_z(P, c->stream, c->p, c->p_alt, c->F, c->fstr, c->geo[2], u0, u1, 1, dt, cnt,
-                                c->mixed);
+            else launch_stage_z(P, c->stream, c->p, c->p_alt, c->F, c->fstr, geo, u0, u1, 1, dt, cnt, c->mixed);
         }
         HIPCHK(c, hipEventRecord(c->ev1, c->stream));
         HIPCHK(c, hipEventSynchronize(c->ev1));
         float ms = 0.0f;
         HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
         if (trace_tune()) {
-            std::fprintf(stderr, "grhd tune: stage form %s %d: %.4f ms per stage\n", seg ? "z-march segment" : "split",
-                         seg, ms / 2.0);
+            std::fprintf(stderr, "grhd tune: stage form %s %d: %.4f ms per stage\n",
+                         seg == kStageXFused ? "x-fused" : seg ? "z-march segment" : "split", seg, ms / 2.0);
             std::fflush(stderr);
         }
         if (best == 0.0f || ms < best) {
@@ -486,12 +487,15 @@ int run_steps(grhd_ctx *c, const double *dts, int n)
         if (rc != GRHD_OK) return rc;
     }
     const Params &P = c->P;
-    const bool zmarch = P.zseg > 0 && c->p_alt;
+    // the ping-pong stage forms (z-march, x-fused) read one primitive array
+    // and write the other
+    const bool zmarch = P.zseg != 0 && c->p_alt;
+    const bool xfused = zmarch && P.zseg == kStageXFused;
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));
     int64_t launches = 0;
     launch_boundaries(P, c->stream, c->p, c->injector);  // grhd2.f90:233
     launches++;
-    // the z-march form writes the stage's primitives to the other array; its
+    // the ping-pong forms write the stage's primitives to the other array; its
     // cells that no ghost fill rewrites (the outermost ghost layer) must hold
     // the same values in both
     if (zmarch && n > 0)
@@ -499,14 +503,15 @@ int run_steps(grhd_ctx *c, const double *dts, int n)
                                  c->stream));
     // launches per stage: the face kernels of each active direction (and the
     // geometry-table kernel of a direction in the table forms, until the
-    // tables are filled, and the deferred-face kernel), the cell kernel (z-march: the z faces with it) and
+    // tables are filled, and the deferred-face kernel), the cell kernel (with
+    // the z faces in the z-march form, the x faces in the x-fused form) and
     // the ghost fill
     int per_stage = 3;  // fused form: all faces, cells, ghosts
     if (!single_stage_fused(P)) {
         per_stage = 2;
         for (int d = 0; d < 3; d++) {
             const bool on = (d == 0) || (d == 1 && P.ny > 1) || (d == 2 && P.nz > 1);
-            if (!on || (zmarch && d == 2)) continue;
+            if (!on || (zmarch && d == (xfused ? 0 : 2))) continue;
             const bool table = (P.face_gw[d] == 0 || P.face_gw[d] == kFaceTableDefer);
             per_stage += 1 + (table && !P.geo_ready) + (P.face_gw[d] == kFaceTableDefer && P.defer_list[d]);
         }
@@ -516,8 +521,8 @@ int run_steps(grhd_ctx *c, const double *dts, int n)
         double *u0 = c->u[c->ucur], *u1 = c->u[1 - c->ucur];
         for (int rk = 1; rk <= 3; rk++) {
             if (zmarch) {
-                launch_stage_z(P, c->stream, c->p, c->p_alt, c->F, c->fstr, c->geo[2], u0, u1, rk, dts[s],
-                               c->counters, c->mixed);
+                launch_stage_z(P, c->stream, c->p, c->p_alt, c->F, c->fstr, c->geo[xfused ? 0 : 2], u0, u1, rk,
+                               dts[s], c->counters, c->mixed);
                 launch_boundaries(P, c->stream, c->p_alt, c->injector);
                 std::swap(c->p, c->p_alt);
             } else {
@@ -1173,9 +1178,9 @@ int grhd_synchronize(grhd_ctx *c)
 
 int grhd_set_stage_form(grhd_ctx *c, int32_t zseg)
 {
-    if (!c || zseg < -1) return GRHD_ERR_INVALID_ARGUMENT;
-    if (zseg > 0 && (!zmarch_possible(c->P) || !c->geo[2]))
-        return fail(c, GRHD_ERR_UNSUPPORTED, "the z-march form needs a 3D grid above 65,536 cells");
+    if (!c || zseg < kStageXFused) return GRHD_ERR_INVALID_ARGUMENT;
+    if ((zseg > 0 || zseg == kStageXFused) && (!zmarch_possible(c->P) || !c->geo[0] || !c->geo[2]))
+        return fail(c, GRHD_ERR_UNSUPPORTED, "the z-march and x-fused forms need a 3D grid above 65,536 cells");
     c->forced_zseg = zseg;
     c->faces_tuned = false;  // applied (or timed) before the next step
     return GRHD_OK;

@@ -15,11 +15,11 @@ struct grhd_ctx {
     bool own_stream = false;
     bool mixed = false;
     bool faces_tuned = false;  // face-kernel tiling chosen (tune_faces)
-    int forced_zseg = -1;      // grhd_set_stage_form: -1 timed choice, else P.zseg
+    int forced_zseg = -1;      // grhd_set_stage_form: -1 timed choice, else P.zseg (kStageXFused: x-fused)
     int forced_face = -1;      // grhd_set_face_form: -1 timed choice, else P.face_gw of every direction
     double *p = nullptr;      // ghosted primitives, SoA [5][NZs][NYs][NXs]
-    // second primitive array of the z-march stage form (P.zseg > 0), which
-    // reads one and writes the other; the two are swapped after each stage,
+    // second primitive array of the ping-pong stage forms (z-march, P.zseg > 0;
+    // x-fused, kStageXFused), which read one and write the other; the two are swapped after each stage,
     // so p is always the current one.  Allocated by the stage-form tuner.
     double *p_alt = nullptr;
     double *u[2] = {nullptr, nullptr};  // conserved, SoA [5][nz][ny][nx]

@@ -37,6 +37,8 @@ enum MetricKind : int {
 
 // Params::face_gw of the geometry-table face form with deferred faces.
 constexpr int kFaceTableDefer = 16;
+// Params::zseg of the x-fused stage form.
+constexpr int kStageXFused = -2;
 
 // Problem constants broadcast to every kernel (passed by value).
 struct Params {
@@ -72,7 +74,9 @@ struct Params {
     // z-march segment of the single-problem stage: each lane walks this many
     // cells of one (x, y) column, computing the z faces in order and
     // differencing them in registers (k_zcell); 0: separate z-face and cell
-    // kernels.  Chosen by timing the stage forms on the problem (tune_faces).
+    // kernels; kStageXFused: the x faces computed by the cell kernel
+    // (k_xcell).  Chosen by timing the stage forms on the problem
+    // (tune_faces).
     int zseg;
     double gam, g1, mass, spin;
     double rho_floor, p_floor;

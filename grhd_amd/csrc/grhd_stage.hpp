@@ -452,14 +452,16 @@ void launch_face_dir_mk(const Params &P, hipStream_t s, const double *p, void *c
 // (grhd2.f90:243-249) into u; cons_to_prim follows in c2p_body.  stage 0
 // writes the rhs itself to u (parity entry point).  ZREG: the z-face fluxes
 // of the cell are zl (left) and zr (right), held in registers by the caller,
-// instead of entries of Fz.
-template <int MK, typename R, bool ZREG = false>
+// instead of entries of Fz.  XREG: the same for the x-face fluxes (xl, xr)
+// on 3D grids; Fx is not read.
+template <int MK, typename R, bool ZREG = false, bool XREG = false>
 __device__ __forceinline__ void update_body(const Params &P, const double *__restrict__ p,
                                             const R *__restrict__ Fx, const R *__restrict__ Fy,
                                             const R *__restrict__ Fz, int64_t fsx, int64_t fsy,
                                             int64_t fsz, const double *__restrict__ u0,
                                             double *__restrict__ u, int stage, double dt, int64_t t,
-                                            const R *zl = nullptr, const R *zr = nullptr)
+                                            const R *zl = nullptr, const R *zr = nullptr,
+                                            const R *xl = nullptr, const R *xr = nullptr)
 {
     const int64_t ncell = (int64_t)P.nx * P.ny * P.nz;
     const int i0 = (int)(t % P.nx);
@@ -496,8 +498,13 @@ __device__ __forceinline__ void update_body(const Params &P, const double *__res
     const double *u0p = rk ? u0 : u, *u1p = (stage >= 2) ? u : u0p;
 #pragma unroll
     for (int v = 0; v < NEQ; v++) {
-        fx0[v] = Fx[v * fsx + fax];
-        fx1[v] = Fx[v * fsx + fax + 1];
+        if constexpr (XREG) {
+            fx0[v] = xl[v];
+            fx1[v] = xr[v];
+        } else {
+            fx0[v] = Fx[v * fsx + fax];
+            fx1[v] = Fx[v * fsx + fax + 1];
+        }
         fy0[v] = Fyp[v * fsyp + fay];
         fy1[v] = Fyp[v * fsyp + fay + sy];
         if constexpr (ZREG) {
@@ -707,6 +714,165 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
     }
 }
 
+// x-fused form of the stage (3D grids): the x faces are computed by the cell
+// kernel itself.  A block of 256 threads takes 256 consecutive x faces of one
+// z plane (flattened over (x, y) as in k_flux_tab), keeps their fluxes in LDS
+// and, after a barrier, 255 threads update the cells between consecutive
+// faces -- the flux divergence, sources, RK combination and con2prim of
+// k_cell -- with the y and z fluxes from their face arrays.  Block b starts
+// at face 255 b, so every face pair of the plane (every cell) falls in one
+// block and one face in 255 is computed twice; a pair that spans two rows
+// (the last face of one row, the first of the next) has no cell.  The x face
+// array never goes through HBM, and the cell work (HBM-bound) shares the CUs
+// with the face work (VALU-bound) of other blocks.  The primitives are read
+// from pin and written to pout: other blocks' stencils still read the cells
+// this one updates.  DEFER (scheme 1 with deferred-face lists): the x faces
+// through flux_face_hot, and a cell next to a face whose HLLC solve falls
+// back to HLLE is appended to defer_list[0] (cell indices; each cell has one
+// owner) for k_xcell_fix, which keeps the fallback's code out of this
+// kernel.  The arithmetic is that of k_flux_tab<1> and k_cell, so the results
+// are theirs bit for bit.
+template <int MK, typename R, int SCH, bool DEFER = false>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_xcell(
+    Params P, const double *__restrict__ pin, double *__restrict__ pout, const R *__restrict__ Fy,
+    const R *__restrict__ Fz, int64_t fsy, int64_t fsz, const R *__restrict__ tab, const double *__restrict__ u0,
+    double *__restrict__ u, int stage, double dt, unsigned long long *__restrict__ counters)
+{
+    static_assert(!DEFER || SCH == 1, "deferred faces: scheme 1 only");
+    __shared__ R fl[NEQ][256];
+    __shared__ int okf[256];
+    const int nfx = P.nx + 1;
+    const int64_t nab = (int64_t)nfx * P.ny;
+    const int64_t nchunk = (nab + 253) / 255;
+    const int64_t blk = xcd_block(blockIdx.x, gridDim.x);
+    const int k0 = (int)(blk / nchunk);
+    const int64_t ab = (blk % nchunk) * 255 + threadIdx.x;
+    R out[NEQ];
+#pragma unroll
+    for (int v = 0; v < NEQ; v++) out[v] = 0;
+    bool ok = true;
+    if (ab < nab) {
+        const int a = (int)(ab % nfx), b = (int)(ab / nfx);
+        const FaceGeo<R> fg{tab + ab, nab};
+        if (!(fg.flags() & FG_ZERO)) {
+            if constexpr (DEFER) ok = flux_face_hot<MK, 1>(P, pin, fg, a, b + 1, k0 + 1, a, P.nx, out);
+            else flux_face<MK, 1, R, SCH>(P, pin, fg, a, b + 1, k0 + 1, a, P.nx, out);
+        }
+    }
+#pragma unroll
+    for (int v = 0; v < NEQ; v++) fl[v][threadIdx.x] = out[v];
+    if constexpr (DEFER) okf[threadIdx.x] = ok ? 1 : 0;
+    __syncthreads();
+    if (threadIdx.x == 255 || ab + 1 >= nab) return;
+    const int a = (int)(ab % nfx);
+    if (a >= P.nx) return;
+    const int b = (int)(ab / nfx);
+    const int64_t t = ((int64_t)k0 * P.ny + b) * P.nx + a;
+    if constexpr (DEFER) {
+        const bool fix = !(okf[threadIdx.x] && okf[threadIdx.x + 1]);
+        defer_push(P.defer_ctr, P.defer_list[0], (unsigned)((int64_t)nfx * P.ny * P.nz), (unsigned)t, fix);
+        if (fix) return;
+    }
+    R xl[NEQ], xr[NEQ];
+#pragma unroll
+    for (int v = 0; v < NEQ; v++) {
+        xl[v] = fl[v][threadIdx.x];
+        xr[v] = fl[v][threadIdx.x + 1];
+    }
+    update_body<MK, R, false, true>(P, pin, nullptr, Fy, Fz, 0, fsy, fsz, u0, u, stage, dt, t, nullptr, nullptr,
+                                    xl, xr);
+    c2p_body<MK, sizeof(R) == sizeof(float), true>(P, const_cast<double *>(pin), u, counters, t, pout);
+}
+
+// The cells k_xcell deferred: each thread takes list entries q = thread,
+// thread + grid, ... below the count, computes the cell's two x faces with
+// flux_face (HLLC with its HLLE fallback) and runs the cell's update and
+// con2prim as k_xcell does.  The block that finishes last clears the count and
+// the block counter (as in k_flux_defer).
+template <int MK, typename R, int SCH>
+__global__ __launch_bounds__(256) void k_xcell_fix(Params P, const double *__restrict__ pin, double *__restrict__ pout,
+                                                   const R *__restrict__ Fy, const R *__restrict__ Fz, int64_t fsy,
+                                                   int64_t fsz, const R *__restrict__ tab,
+                                                   const double *__restrict__ u0, double *__restrict__ u, int stage,
+                                                   double dt, unsigned long long *__restrict__ counters)
+{
+    unsigned *ctr = P.defer_ctr;
+    const unsigned *list = P.defer_list[0];
+    const int64_t nab = (int64_t)(P.nx + 1) * P.ny;
+    const unsigned n = __hip_atomic_load(ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (unsigned q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
+        const int64_t t = list[q];
+        const int a = (int)(t % P.nx);
+        const int64_t rem = t / P.nx;
+        const int b = (int)(rem % P.ny), k0 = (int)(rem / P.ny);
+        R xl[NEQ], xr[NEQ];
+        for (int side = 0; side < 2; side++) {
+            const int64_t ab = (int64_t)b * (P.nx + 1) + a + side;
+            const FaceGeo<R> fg{tab + ab, nab};
+            R *o = side ? xr : xl;
+#pragma unroll
+            for (int v = 0; v < NEQ; v++) o[v] = 0;
+            if (!(fg.flags() & FG_ZERO)) flux_face<MK, 1, R, SCH>(P, pin, fg, a + side, b + 1, k0 + 1, a + side, P.nx, o);
+        }
+        update_body<MK, R, false, true>(P, pin, nullptr, Fy, Fz, 0, fsy, fsz, u0, u, stage, dt, t, nullptr, nullptr,
+                                        xl, xr);
+        c2p_body<MK, sizeof(R) == sizeof(float), true>(P, const_cast<double *>(pin), u, counters, t, pout);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned fin = __hip_atomic_fetch_add(ctr + 1, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (fin == gridDim.x - 1) {
+            __hip_atomic_store(ctr, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(ctr + 1, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+}
+
+// Blocks of k_xcell.
+inline int64_t xcell_blocks(const Params &P)
+{
+    const int64_t nab = (int64_t)(P.nx + 1) * P.ny;
+    return ((nab + 253) / 255) * P.nz;
+}
+
+// Stage 1..3 in the x-fused form: y and z faces (as in the split form, into
+// their arrays), the x-face geometry into geo_x until it is filled, then
+// k_xcell (and k_xcell_fix for its deferred cells).
+template <int MK, typename R>
+void launch_stage_x_mk(const Params &P, hipStream_t s, const double *pin, double *pout, void *const F[3],
+                       const int64_t fstr[3], void *geo_x, const double *u0, double *u, int stage, double dt,
+                       unsigned long long *counters)
+{
+    R *Fy = (R *)F[1], *Fz = (R *)F[2], *tab = (R *)geo_x;
+    const int64_t nab = (int64_t)(P.nx + 1) * P.ny;
+    const unsigned nb = (unsigned)xcell_blocks(P);
+#define GRHD_XSTAGE(SCHV)                                                                                        \
+    do {                                                                                                         \
+        launch_flux<MK, 2, R, SCHV>(P, s, pin, Fy, fstr[1]);                                                     \
+        launch_flux<MK, 3, R, SCHV>(P, s, pin, Fz, fstr[2]);                                                     \
+        if (!P.geo_ready) k_geo<MK, 1, R><<<blocks_for(nab, 256), 256, 0, s>>>(P, tab);                          \
+        k_xcell<MK, R, SCHV><<<nb, 256, 0, s>>>(P, pin, pout, Fy, Fz, fstr[1], fstr[2], tab, u0, u, stage, dt,   \
+                                                counters);                                                       \
+    } while (0)
+    if constexpr (MK == MK_KS || MK == MK_KS_LOG) {
+        const int sch = face_scheme<MK>(P);
+        if (sch == 1 && P.defer_list[0]) {
+            launch_flux<MK, 2, R, 1>(P, s, pin, Fy, fstr[1]);
+            launch_flux<MK, 3, R, 1>(P, s, pin, Fz, fstr[2]);
+            if (!P.geo_ready) k_geo<MK, 1, R><<<blocks_for(nab, 256), 256, 0, s>>>(P, tab);
+            k_xcell<MK, R, 1, true><<<nb, 256, 0, s>>>(P, pin, pout, Fy, Fz, fstr[1], fstr[2], tab, u0, u, stage, dt,
+                                                       counters);
+            k_xcell_fix<MK, R, 1><<<kDeferBlocks, 256, 0, s>>>(P, pin, pout, Fy, Fz, fstr[1], fstr[2], tab, u0, u,
+                                                               stage, dt, counters);
+            return;
+        }
+        if (sch == 1) { GRHD_XSTAGE(1); return; }
+        if (sch == 2) { GRHD_XSTAGE(2); return; }
+    }
+    GRHD_XSTAGE(0);
+#undef GRHD_XSTAGE
+}
+
 // Stage 1..3 in the z-march form: x and y faces (as in the split form, into
 // their arrays), the z-face geometry into geo_z, then k_zcell.
 template <int MK, typename R>
@@ -714,6 +880,10 @@ void launch_stage_z_mk(const Params &P, hipStream_t s, const double *pin, double
                        const int64_t fstr[3], void *geo_z, const double *u0, double *u, int stage, double dt,
                        unsigned long long *counters)
 {
+    if (P.zseg == kStageXFused) {
+        launch_stage_x_mk<MK, R>(P, s, pin, pout, F, fstr, geo_z, u0, u, stage, dt, counters);
+        return;
+    }
     R *Fx = (R *)F[0], *Fy = (R *)F[1], *tab = (R *)geo_z;
     const int64_t nab = (int64_t)P.nx * P.ny;
     const int64_t waves = ((nab + 63) / 64) * ((P.nz + P.zseg - 1) / P.zseg);

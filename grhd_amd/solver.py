@@ -207,7 +207,8 @@ class Solver:
 
     def set_stage_form(self, zseg: int):
         """-1: timed choice on the first step; 0: split form; n > 0: z-march
-        form with n-cell segments (3D grids above 65,536 cells)."""
+        form with n-cell segments; -2: x-fused form (3D grids above 65,536
+        cells)."""
         self._chk(self._L.grhd_set_stage_form(self.h, int(zseg)))
 
     def set_face_form(self, form: int):

@@ -239,9 +239,12 @@ int grhd_last_timing(grhd_ctx *ctx, double *ms, int64_t *launches);
  * -1 (the default): chosen on the first step by timing one stage of each form;
  * 0: the split form (face fluxes of every direction into their arrays, then
  * the cell kernel); n > 0: the z-march form, in which each thread walks
- * n-cell segments of one column and differences the z fluxes in registers
- * (a second primitive array is allocated).  Every form gives the same results
- * bit for bit.  GRHD_ERR_UNSUPPORTED for n > 0 on other grids. */
+ * n-cell segments of one column and differences the z fluxes in registers;
+ * -2: the x-fused form, in which the cell kernel computes the x faces itself
+ * (256 consecutive faces per block, differenced through LDS).  The z-march
+ * and x-fused forms allocate a second primitive array.  Every form gives the
+ * same results bit for bit.  GRHD_ERR_UNSUPPORTED for n > 0 or -2 on other
+ * grids. */
 int grhd_set_stage_form(grhd_ctx *ctx, int32_t zseg);
 
 /* Face-kernel form of every direction of a problem outside the fused stage

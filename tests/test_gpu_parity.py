@@ -167,14 +167,16 @@ def _large(name):
 
 @pytest.mark.parametrize("name,zseg,face", [("fm3d_ks_mp5_hllc_sensor", -1, -1), ("fm3d_ks_mp5_hllc_sensor", 0, -1),
                                             ("fm3d_ks_mp5_hllc_sensor", 16, -1), ("fm3d_ks_mp5_hllc_sensor", 0, 16),
-                                            ("fm3d_ks_mp5_hllc_sensor", 0, 4), ("fm3d_ks_weno5_hlle", -1, -1),
-                                            ("fm3d_ks_weno5_hlle", 8, -1)])
+                                            ("fm3d_ks_mp5_hllc_sensor", 0, 4), ("fm3d_ks_mp5_hllc_sensor", -2, -1),
+                                            ("fm3d_ks_weno5_hlle", -1, -1), ("fm3d_ks_weno5_hlle", 8, -1),
+                                            ("fm3d_ks_weno5_hlle", -2, -1)])
 def test_large_grid_one_step_matches_oracle(name, zseg, face):
     """Grids above 65,536 cells take the per-direction stage forms that the 3D
     torus benchmark runs, with the face kernels compiled for the scheme (MP5 +
     sensor + HLLC, WENO5 + HLLE): the split form (one face launch per
-    direction, then the cell kernel, zseg 0), the z-march form (zseg > 0) and
-    the timed choice between them (-1); smaller grids the fused form the other
+    direction, then the cell kernel, zseg 0), the z-march form (zseg > 0), the
+    x-fused form (x faces computed by the cell kernel, -2) and the timed
+    choice between them (-1); smaller grids the fused form the other
     tests cover.  face: the face-kernel form (-1 timed, 4 LDS geometry
     groups, 16 geometry table with the HLLC fallbacks deferred to a second
     kernel)."""
@@ -205,13 +207,13 @@ def test_large_grid_one_step_matches_oracle(name, zseg, face):
 
 @pytest.mark.parametrize("precision", [0, 1])
 def test_stage_forms_agree_bit_for_bit(precision):
-    """The split form and the z-march form with segments that divide nz (8)
-    or not (16, 32 of nz = 40) give the same state after three steps, bit for
-    bit, in FP64 and with FP32 faces."""
+    """The split form, the z-march form with segments that divide nz (8) or
+    not (16, 32 of nz = 40) and the x-fused form (-2) give the same state
+    after three steps, bit for bit, in FP64 and with FP32 faces."""
     cfg = _large("fm3d_ks_mp5_hllc_sensor").replace(precision=precision)
     print("stage forms", cfg.nx, cfg.ny, cfg.nz, "precision", precision, flush=True)
     out = {}
-    for zseg in (0, 8, 16, 32):
+    for zseg in (0, 8, 16, 32, -2):
         with Solver(cfg) as S:
             S.set_stage_form(zseg)
             S.initial_conditions()
@@ -221,7 +223,7 @@ def test_stage_forms_agree_bit_for_bit(precision):
             S.step(dt, 2)
             S.step(dt, 1)
             out[zseg] = (S.get_conserved(), S.get_primitives(), S.counters())
-    for zseg in (8, 16, 32):
+    for zseg in (8, 16, 32, -2):
         assert np.array_equal(out[zseg][0], out[0][0]), zseg
         assert np.array_equal(out[zseg][1], out[0][1]), zseg
         assert out[zseg][2] == out[0][2], zseg
@@ -234,11 +236,13 @@ def test_face_forms_agree_bit_for_bit(precision):
     (16: faces whose HLLC solve falls back to HLLE computed by a second kernel
     over a list of them) -- give the same
     state and counters after three steps, bit for bit, in FP64 and with FP32
-    faces, in the split and the z-march stage forms."""
+    faces, in the split, z-march and x-fused stage forms (the x-fused form
+    computes its x faces itself, in the run-time cascade with the fallback
+    inline)."""
     cfg = _large("fm3d_ks_mp5_hllc_sensor").replace(precision=precision)
     print("face forms", cfg.nx, cfg.ny, cfg.nz, "precision", precision, flush=True)
     out = {}
-    for zseg in (0, 16):
+    for zseg in (0, 16, -2):
         for form in (4, 0, 16):
             with Solver(cfg) as S:
                 S.set_stage_form(zseg)
@@ -257,27 +261,29 @@ def test_face_forms_agree_bit_for_bit(precision):
 
 
 @pytest.mark.parametrize("precision", [0, 1])
-def test_deferred_faces_at_workload_size(precision):
+def test_forms_agree_at_workload_size(precision):
     """The 3D torus benchmark grid (256 x 128 x 128, theta in [0.05 pi,
-    0.95 pi], 1e-2 density noise): two steps with the geometry table alone
-    and with deferred faces give the same state bit for bit."""
+    0.95 pi], 1e-2 density noise): two steps in the split form with the
+    geometry table alone and with deferred faces, and in the x-fused form,
+    give the same state bit for bit."""
     import bench
     from grhd_amd.solver import fishbone_unit_density_k
     cfg = bench.c4_config().replace(precision=precision)
     cfg.fm_polytropic_constant = fishbone_unit_density_k(cfg)
-    print("deferred faces 256x128x128 precision", precision, flush=True)
+    print("stage and face forms 256x128x128 precision", precision, flush=True)
     out = {}
-    for form in (0, 16):
+    for stage, face in ((0, 0), (0, 16), (-2, 16)):
         with Solver(cfg) as S:
-            S.set_stage_form(0)
-            S.set_face_form(form)
+            S.set_stage_form(stage)
+            S.set_face_form(face)
             S.initial_conditions()
             S.perturb(2, 1.0e-2, 3435)
             S.step(S.compute_dt(), 2)
-            out[form] = (S.get_conserved(), S.counters())
-    assert np.isfinite(out[16][0]).all()
-    assert np.array_equal(out[16][0], out[0][0])
-    assert out[16][1] == out[0][1]
+            out[stage, face] = (S.get_conserved(), S.counters())
+    assert np.isfinite(out[0, 0][0]).all()
+    for key in out:
+        assert np.array_equal(out[key][0], out[0, 0][0]), key
+        assert out[key][1] == out[0, 0][1], key
 
 
 def test_face_form_refused_where_unsupported():
@@ -299,6 +305,10 @@ def test_stage_form_refused_on_small_grids():
     with Solver(cfg) as S:
         with pytest.raises(Exception):
             S.set_stage_form(8)
+        with pytest.raises(Exception):
+            S.set_stage_form(-2)
+        with pytest.raises(Exception):
+            S.set_stage_form(-3)
         S.set_stage_form(0)
 
 
