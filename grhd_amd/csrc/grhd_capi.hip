@@ -405,7 +405,9 @@ int tune_faces(grhd_ctx *c, double dt)
 }
 
 // Stage form of a 3D problem, chosen once per context after the face tiling
-// by timing one stage (stage 1, the step's dt) of each candidate: the split
+// by timing the three stages of one step (the step's dt) of each candidate
+// (timing stage 1 alone, which does not read the stage buffer, favoured the
+// x-fused form, which measured 0.5 % slower over whole steps): the split
 // form (faces of every direction into their arrays, then k_cell), the
 // z-march form with 16-cell segments (8-cell segments measured the same on
 // the 3D torus, 32-cell ones 25 % slower: too few waves) and the x-fused
@@ -449,11 +451,15 @@ int tune_stage_form(grhd_ctx *c, double dt)
         Params P = P0;
         P.zseg = seg;
         if (seg == 0) HIPCHK(c, hipMemcpyAsync(c->p_alt, c->p, pbytes, hipMemcpyDeviceToDevice, c->stream));
-        for (int rep = 0; rep < 3; rep++) {
+        // a warm-up stage 1, then stages 1, 2 and 3 (stages 2 and 3 also
+        // read the stage buffer): the mix of one step
+        for (int rep = 0; rep < 4; rep++) {
+            const int stage = (rep == 0) ? 1 : rep;
             if (rep == 1) HIPCHK(c, hipEventRecord(c->ev0, c->stream));
-            if (seg == 0) launch_stage(P, c->stream, c->p_alt, c->F, c->fstr, u0, u1, 1, dt, cnt, c->mixed,
+            if (seg == 0) launch_stage(P, c->stream, c->p_alt, c->F, c->fstr, u0, u1, stage, dt, cnt, c->mixed,
                                        c->injector, false);
-            else launch_stage_z(P, c->stream, c->p, c->p_alt, c->F, c->fstr, geo, u0, u1, 1, dt, cnt, c->mixed);
+            else launch_stage_z(P, c->stream, c->p, c->p_alt, c->F, c->fstr, geo, u0, u1, stage, dt, cnt,
+                                c->mixed);
         }
         HIPCHK(c, hipEventRecord(c->ev1, c->stream));
         HIPCHK(c, hipEventSynchronize(c->ev1));
@@ -461,7 +467,7 @@ int tune_stage_form(grhd_ctx *c, double dt)
         HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
         if (trace_tune()) {
             std::fprintf(stderr, "grhd tune: stage form %s %d: %.4f ms per stage\n",
-                         seg == kStageXFused ? "x-fused" : seg ? "z-march segment" : "split", seg, ms / 2.0);
+                         seg == kStageXFused ? "x-fused" : seg ? "z-march segment" : "split", seg, ms / 3.0);
             std::fflush(stderr);
         }
         if (best == 0.0f || ms < best) {

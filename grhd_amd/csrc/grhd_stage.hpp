@@ -334,12 +334,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
 
 // The deferred faces of one direction (k_flux_tab with DEFER): each thread
 // takes list entries q = thread, thread + grid, ... below the count and
-// computes the face with flux_face (the full scheme: sensor cascade, HLLC and
-// its HLLE fallback).  The block that finishes last clears the count and the
-// block counter for the next stage: every block has read the count before it
-// adds itself to the block counter.
+// computes the face's HLLE flux from its reconstructed states -- what
+// flux_face computes for a face whose HLLC solve fails (fluxes.f90:56-72),
+// without repeating that solve.  The block that finishes last clears the
+// count and the block counter for the next stage: every block has read the
+// count before it adds itself to the block counter.
 template <int MK, int DIR, typename R, int SCH>
-__global__ __launch_bounds__(256) void k_flux_defer(Params P, const double *__restrict__ p, R *__restrict__ F,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_flux_defer(Params P, const double *__restrict__ p, R *__restrict__ F,
                                                     int64_t fstride, const R *__restrict__ tab)
 {
     unsigned *ctr = P.defer_ctr + 2 * (DIR - 1);
@@ -356,8 +357,9 @@ __global__ __launch_bounds__(256) void k_flux_defer(Params P, const double *__re
         const int i = a + (DIR != 1), j = b + (DIR != 2), k = c + (DIR != 3);
         const int fi = (DIR == 1) ? i : (DIR == 2) ? j : k;
         const FaceGeo<R> fg{tab + ab, nab};
-        R out[NEQ];
-        flux_face<MK, DIR, R, SCH>(P, p, fg, i, j, k, fi, nodes, out);
+        R qL[NEQ], qR[NEQ], out[NEQ];
+        face_states<MK, DIR, R, SCH>(P, p, i, j, k, fi, nodes, qL, qR);
+        hlle<R, DIR, kSRHD<MK>>((R)P.g1, (R)P.gam, qL, qR, fg, out);
 #pragma unroll
         for (int v = 0; v < NEQ; v++) F[v * fstride + fo] = out[v];
     }
@@ -371,8 +373,11 @@ __global__ __launch_bounds__(256) void k_flux_defer(Params P, const double *__re
     }
 }
 
-// Blocks of k_flux_defer: the list is short (HLLC fallbacks); longer lists
-// take more passes.
+// Blocks of k_flux_defer and k_xcell_fix: one pass over lists of up to
+// 65,536 entries (the HLLC fallbacks of the 3D torus in FP64: none of the x
+// faces, 1-1.5 % of the y and z faces); longer lists take more passes.  A
+// grid of 1024 blocks measured 15 us slower per launch: the launch pays for
+// its empty blocks.
 constexpr unsigned kDeferBlocks = 256;
 
 template <int MK, int DIR, typename R, int SCH>
