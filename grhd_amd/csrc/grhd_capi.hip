@@ -327,8 +327,9 @@ void coord_entries(const Params &P, Put put)
 
 // Face-kernel tiling of the problem, chosen once per context by timing: for
 // each active direction, the face kernel alone with 4, 2, 8 and 1 waves sharing
-// one LDS copy of the face geometry, and with the geometry evaluated by its own
-// kernel into the stage's table (one warm-up launch, then three timed), on the
+// one LDS copy of the face geometry, with the geometry read from a
+// per-problem table, and with the table and the HLLC fallbacks deferred to a
+// second kernel (one warm-up launch, then three timed; two rounds), on the
 // problem's own primitives.  The face kernels write only the face arrays (and
 // the table), which the next stage rewrites, and every form gives the same
 // fluxes, so tuning changes no result.  Grids in the fused stage form keep
@@ -354,6 +355,11 @@ int tune_faces(grhd_ctx *c, double dt)
         if (!(d == 0 || (d == 1 && c->P.ny > 1) || (d == 2 && c->P.nz > 1))) continue;
         float best = 0.0f;
         int best_gw = 4;
+        // two rounds over the candidates, the fastest time of each kept: the
+        // clock rises over the first few hundred milliseconds of load, which
+        // would favour the candidates timed last, and the first round of a
+        // table form also fills its table
+        for (int round = 0; round < 2; round++)
         for (int gw : cand) {
             const bool table = (gw == 0 || gw == kFaceTableDefer);
             if (table && !c->geo[d]) continue;
@@ -445,6 +451,7 @@ int tune_stage_form(grhd_ctx *c, double dt)
     static const int cand[] = {0, 16, kStageXFused};
     float best = 0.0f;
     int best_seg = 0;
+    for (int round = 0; round < 2; round++)  // as in tune_faces
     for (int seg : cand) {
         if (seg > P0.nz) continue;
         void *geo = (seg == kStageXFused) ? c->geo[0] : c->geo[2];

@@ -180,9 +180,9 @@ GRHD_HD bool finite(float a) { return isfinite(a); }
 // The host (initial data, set-up) divides with the IEEE operator.  Used in the cell kernel's metric and
 // con2prim (its root solve divides four times per iteration): 7 % fewer
 // instructions there and +2.6 % on the FP64 stage of the 3D torus, with the
-// final state unchanged bit for bit.  The face kernels keep the IEEE
-// sequence: with these, their schedules hold more values and spill (FP64
-// stage 15 % slower).
+// final state unchanged bit for bit; and, through rcp and rdiv, in the FP64
+// face arithmetic: +1.7 % on the FP64 stage once the face kernels no longer
+// carry the HLLE fallback (with it inline they spilled and ran 15 % slower).
 GRHD_HD double div64(double a, double b)
 {
 #if defined(__HIP_DEVICE_COMPILE__)
@@ -215,7 +215,8 @@ template <typename R> GRHD_HD R quot(R a, R b, R rb)
 }
 
 // Reciprocal, division and reciprocal square root in the face arithmetic R.
-// FP64: the IEEE operations the reference performs.  FP32 (the mixed path,
+// FP64: the IEEE results the reference computes, through div64 on the
+// device.  FP32 (the mixed path,
 // held to an error bound against FP64): the hardware v_rcp_f32 (about 1 ulp)
 // refined by one Newton step, y + y (1 - b y), then the remainder correction
 // of quot: within about half an ulp, in four instructions instead of the eight
@@ -225,7 +226,7 @@ template <typename R> GRHD_HD R quot(R a, R b, R rb)
 template <typename R> GRHD_HD R rcp(R b)
 {
     if constexpr (sizeof(R) == 8) {
-        return (R)1 / b;
+        return rcp64(b);
     } else {
 #if defined(__HIP_DEVICE_COMPILE__)
         const float y = __builtin_amdgcn_rcpf(b);
@@ -238,7 +239,7 @@ template <typename R> GRHD_HD R rcp(R b)
 template <typename R> GRHD_HD R rdiv(R a, R b)
 {
     if constexpr (sizeof(R) == 8) {
-        return a / b;
+        return div64(a, b);
     } else {
 #if defined(__HIP_DEVICE_COMPILE__)
         return quot(a, b, __builtin_amdgcn_rcpf(b));
