@@ -411,14 +411,15 @@ int tune_faces(grhd_ctx *c, double dt)
 }
 
 // Stage form of a 3D problem, chosen once per context after the face tiling
-// by timing the three stages of one step (the step's dt) of each candidate
-// (timing stage 1 alone, which does not read the stage buffer, favoured the
-// x-fused form, which measured 0.5 % slower over whole steps): the split
-// form (faces of every direction into their arrays, then k_cell), the
-// z-march form with 16-cell segments (8-cell segments measured the same on
-// the 3D torus, 32-cell ones 25 % slower: too few waves) and the x-fused
-// form (x faces computed by the cell kernel, k_xcell).  Every form gives
-// the same results bit for bit.  The timing runs write only the stage buffer u1, the
+// by timing the three stages of one step (the step's dt) of each candidate:
+// the split form (faces of every direction into their arrays, then k_cell)
+// and the z-march form with 16-cell segments (8-cell segments measured the
+// same on the 3D torus, 32-cell ones 25 % slower: too few waves).  The
+// x-fused form (x faces computed by the cell kernel, k_xcell) is selected
+// only by grhd_set_stage_form: over whole steps of the 3D torus it measured
+// 0.5 % slower than the split form in FP64 and 2 % in mixed precision, while
+// these stage timings, on scratch buffers, put it 4-6 % ahead.  Every form
+// gives the same results bit for bit.  The timing runs write only the stage buffer u1, the
 // face arrays and tables, the second primitive array and the tuner's
 // counters: the split form runs on a copy of the primitives in p_alt.
 bool zmarch_possible(const Params &P) { return !single_stage_fused(P) && P.ny > 1 && P.nz > 1; }
@@ -448,13 +449,13 @@ int tune_stage_form(grhd_ctx *c, double dt)
     }
     double *u0 = c->u[c->ucur], *u1 = c->u[1 - c->ucur];
     unsigned long long *cnt = c->counters + 4;
-    static const int cand[] = {0, 16, kStageXFused};
+    static const int cand[] = {0, 16};
     float best = 0.0f;
     int best_seg = 0;
     for (int round = 0; round < 2; round++)  // as in tune_faces
     for (int seg : cand) {
         if (seg > P0.nz) continue;
-        void *geo = (seg == kStageXFused) ? c->geo[0] : c->geo[2];
+        void *geo = c->geo[2];
         Params P = P0;
         P.zseg = seg;
         if (seg == 0) HIPCHK(c, hipMemcpyAsync(c->p_alt, c->p, pbytes, hipMemcpyDeviceToDevice, c->stream));
@@ -477,7 +478,9 @@ int tune_stage_form(grhd_ctx *c, double dt)
                          seg == kStageXFused ? "x-fused" : seg ? "z-march segment" : "split", seg, ms / 3.0);
             std::fflush(stderr);
         }
-        if (best == 0.0f || ms < best) {
+        // a ping-pong form (a second primitive array and a copy per call)
+        // only for a clear gain: 2 % faster than the best so far
+        if (best == 0.0f || ms < (seg == 0 ? best : 0.98f * best)) {
             best = ms;
             best_seg = seg;
         }
