@@ -197,6 +197,37 @@ GRHD_HD double div64(double a, double b)
 }
 GRHD_HD double rcp64(double b) { return div64(1.0, b); }
 
+// FP64 square root on the device: v_rsq_f64 and the compiler's own Newton
+// sequence for sqrt (two multiplies, seven FMAs, then +-0 and +inf returned
+// as given), without its range scaling, which leaves the result unchanged for
+// arguments at or above 2^-767 (the scaling only acts below); NaN and
+// negative arguments give NaN as before.  No argument of the path comes near
+// that bound (velocities, sound speeds, metric determinants, discriminants of
+// conserved-state magnitudes).  The result is the IEEE square root's bit for
+// bit.  The host uses the C library's.
+GRHD_HD double sqrt64(double x)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    const double y = __builtin_amdgcn_rsq(x);
+    double g = x * y, h = y * 0.5;
+    const double r = fma(-h, g, 0.5);
+    g = fma(g, r, g);
+    h = fma(h, r, h);
+    double d = fma(-g, g, x);
+    g = fma(d, h, g);
+    d = fma(-g, g, x);
+    g = fma(d, h, g);
+    return __builtin_amdgcn_class(x, 0x260) ? x : g;  // +-0, +inf
+#else
+    return sqrt(x);
+#endif
+}
+// Square root in the face or cell arithmetic: sqrt64 in FP64, the hardware
+// instruction in FP32 (the mixed path's units take it with
+// -fno-hip-fp32-correctly-rounded-divide-sqrt).
+GRHD_HD double sqrtr(double x) { return sqrt64(x); }
+GRHD_HD float sqrtr(float x) { return sqrtf(x); }
+
 // a / b, given rb = 1 / b, for divisions that share a divisor or divide by a
 // constant.  An IEEE FP64 division costs a reciprocal and about ten dependent
 // VALU operations; this costs one multiply and two FMAs and returns the same
@@ -248,7 +279,7 @@ template <typename R> GRHD_HD R rdiv(R a, R b)
 #endif
     }
 }
-template <typename R> GRHD_HD R rrsqrt(R x) { return rcp(sqrt(x)); }
+template <typename R> GRHD_HD R rrsqrt(R x) { return rcp(sqrtr(x)); }
 
 // ---------------------------------------------------------------- metric --
 // The 3+1 quantities on a face: lapse, shift, spatial metric, sqrt(det).
@@ -308,7 +339,7 @@ GRHD_HD void metric_face(const Params &P, const Coord &q, FaceMetric &m)
         const double M = P.mass;
         const double r = q.r;
         const double s = q.s;
-        m.alpha = 1.0 / sqrt(1.0 + 2.0 * M / r);
+        m.alpha = 1.0 / sqrtr(1.0 + 2.0 * M / r);
         const double g11 = (MK == MK_EF_LOG) ? (1.0 + 2.0 * M / r) * (r * r) : 1.0 + 2.0 * M / r;
         m.beta[0] = (MK == MK_EF_LOG) ? (2.0 * M / r / (1.0 + 2.0 * M / r)) / r
                                       : 2.0 * M / r / (1.0 + 2.0 * M / r);
@@ -323,7 +354,7 @@ GRHD_HD void metric_face(const Params &P, const Coord &q, FaceMetric &m)
         const double s = q.s, c = q.c, s2 = s * s, c2 = c * c;
         const double Sig = r * r + (a * a) * c2;
         const double H = div64(M * r, Sig);
-        m.alpha = rcp64(sqrt(1.0 + 2.0 * H));
+        m.alpha = rcp64(sqrtr(1.0 + 2.0 * H));
         if (MK == MK_KS_LOG) {
             m.beta[0] = div64(div64(2.0 * H, 1.0 + 2.0 * H), r);
             m.g[0][0] = (1.0 + 2.0 * H) * (r * r);
@@ -531,7 +562,7 @@ GRHD_HD void metric_centre(const Params &P, const Coord &q, CentreMetric &m)
     for (int a = 0; a < 3; a++)
         for (int b = 0; b < 3; b++)
             m.ginv[a][b] = m.gm[a + 1][b + 1] + quot(m.beta[a] * m.beta[b], a2, ra2);
-    m.sqg = sqrt(m.det);
+    m.sqg = sqrtr(m.det);
 }
 
 // Lighter centre metric for con2prim and prim_to_cons: gamma_ij, gamma^ij, sqrt(det).
@@ -570,9 +601,9 @@ GRHD_HD R vsq_clamp(const G &g, R v[3])
     for (int a = 0; a < 3; a++)
         for (int b = 0; b < 3; b++) vsq = vsq + g.gam(a, b) * v[a] * v[b];
     if (vsq >= vmax) {
-        v[0] = v[0] * sqrt(rdiv(vmax, vsq));
-        v[1] = v[1] * sqrt(rdiv(vmax, vsq));
-        v[2] = v[2] * sqrt(rdiv(vmax, vsq));
+        v[0] = v[0] * sqrtr(rdiv(vmax, vsq));
+        v[1] = v[1] * sqrtr(rdiv(vmax, vsq));
+        v[2] = v[2] * sqrtr(rdiv(vmax, vsq));
         vsq = vmax;
     }
     return vsq;
@@ -615,7 +646,7 @@ GRHD_HD void p2c(R g1, const R q[NEQ], const G &g, R sqg, R U[NEQ])
 // column of NFG values per lane.
 enum : int {
     FG_ALPHA = 0, FG_BETA = 1, FG_G = 4, FG_SQG = 10, FG_COF = 11, FG_GDD = 20, FG_W = 21,
-    FG_GII = 22, FG_FLAGS = 23, NFG = 24
+    FG_GII = 22, FG_FLAGS = 23, FG_FAC = 24, NFG = 25
 };
 enum : int { FG_FRAME_OK = 1, FG_ZERO = 2 };
 // gamma_ij is symmetric: six stored components 00 01 02 11 12 22.
@@ -636,6 +667,8 @@ template <typename R> struct FaceGeo {
     GRHD_HD R gdd() const { return get(FG_GDD); }
     GRHD_HD R wface() const { return get(FG_W); }
     GRHD_HD R gii() const { return get(FG_GII); }
+    // sqrt(gamma) alpha sqrt(gamma^dd), the factor of local_flux_to_coordinate
+    GRHD_HD R fac() const { return get(FG_FAC); }
     GRHD_HD int flags() const { return (int)get(FG_FLAGS); }
 };
 
@@ -668,12 +701,12 @@ GRHD_HD void wave_speeds(R g1, R gam, const R q[NEQ], const FG &f, R &wmin, R &w
     const R vsq = vsq_clamp<R>(f, v);
     const R enth = (R)1 + rdiv(g1 * q[EQ_PR], q[EQ_DE]);
     const R cs2 = rdiv(gam * q[EQ_PR], q[EQ_DE] * enth);
-    const R cs = sqrt(cs2);
+    const R cs = sqrtr(cs2);
     const R vn = v[DIR - 1];
     R e1, e2, e3;
     if (SRHD) {
         const R t1 = rcp((R)1 - vsq * cs2);
-        const R t2 = cs * sqrt(((R)1 - vsq) * (rcp(t1) - (vn * vn) * ((R)1 - cs2)));
+        const R t2 = cs * sqrtr(((R)1 - vsq) * (rcp(t1) - (vn * vn) * ((R)1 - cs2)));
         e1 = vn;
         e2 = t1 * (vn * ((R)1 - cs2) + t2);
         e3 = t1 * (vn * ((R)1 - cs2) - t2);
@@ -681,7 +714,7 @@ GRHD_HD void wave_speeds(R g1, R gam, const R q[NEQ], const FG &f, R &wmin, R &w
         const R gii = f.gii();
         const R bn = f.beta(DIR - 1);
         const R t1 = rdiv(f.alpha(), (R)1 - vsq * cs2);
-        const R t2 = cs * sqrt(rmax((R)0, ((R)1 - vsq) * (gii * ((R)1 - vsq * cs2) -
+        const R t2 = cs * sqrtr(rmax((R)0, ((R)1 - vsq) * (gii * ((R)1 - vsq * cs2) -
                                                         (vn * vn) * ((R)1 - cs2))));
         e1 = f.alpha() * vn - bn;
         e2 = t1 * (vn * ((R)1 - cs2) + t2) - bn;
@@ -757,7 +790,7 @@ GRHD_HD bool face_frame(const R g[3][3], R cof[3][3], R &gdd)
     const int d = DIR - 1;
     gdd = ginv[d][d];
     if (gdd <= (R)1.0e-30 || !finite(gdd)) return false;
-    const R sgdd = sqrt(gdd), rsgdd = rcp(sgdd);
+    const R sgdd = sqrtr(gdd), rsgdd = rcp(sgdd);
     for (int i = 0; i < 3; i++) frame[i][0] = quot(ginv[i][d], sgdd, rsgdd);
     const int t0 = (DIR == 1) ? 1 : (DIR == 2) ? 2 : 0;
     const int t1 = (DIR == 1) ? 2 : (DIR == 2) ? 0 : 1;
@@ -771,7 +804,7 @@ GRHD_HD bool face_frame(const R g[3][3], R cof[3][3], R &gdd)
         }
         const R nsq = metric_dot(seed, seed, g);
         if (!finite(nsq) || nsq <= (R)1.0e-28) return false;
-        const R snsq = sqrt(nsq), rsnsq = rcp(snsq);
+        const R snsq = sqrtr(nsq), rsnsq = rcp(snsq);
         for (int i = 0; i < 3; i++) frame[i][b] = quot(seed[i], snsq, rsnsq);
     }
     for (int a = 0; a < 3; a++)
@@ -830,7 +863,7 @@ GRHD_HD bool srhd_local(R g1, R gam, const R ql[NEQ], R U[NEQ], R F[NEQ], R &wmi
     const R den = (R)1 - vsq * cs2;
     const R rad = ((R)1 - vsq) * ((R)1 - vsq * cs2 - (v0 * v0) * ((R)1 - cs2));
     ok = ok & !(den <= (R)1.0e-14) & !(rad < (R)-1.0e-12);
-    const R ac = sqrt(cs2) * sqrt(rmax((R)0, rad));
+    const R ac = sqrtr(cs2) * sqrtr(rmax((R)0, rad));
     const R rden = rcp(den);
     wmin = quot(v0 * ((R)1 - cs2) - ac, den, rden);
     wmax = quot(v0 * ((R)1 - cs2) + ac, den, rden);
@@ -852,10 +885,11 @@ GRHD_HD bool admissible(const R U[NEQ])
     return ok & !(U[EQ_DE] <= (R)0) & !(E <= (R)0) & !(E * E - U[EQ_DE] * U[EQ_DE] - msq <= (R)1.0e-13 * scale);
 }
 
-// local_flux_to_coordinate (fluxes.f90:596-624).
+// local_flux_to_coordinate (fluxes.f90:596-624).  The factor
+// sqrt(gamma) alpha sqrt(gamma^dd) depends on the face's position only and
+// comes with its geometry.
 template <typename R, typename FG>
-GRHD_HD bool local_to_coord(const R Ul[NEQ], const R Fl[NEQ], R w, R alpha, R sqg, R gdd,
-                            const FG &fg, R F[NEQ])
+GRHD_HD bool local_to_coord(const R Ul[NEQ], const R Fl[NEQ], R w, const FG &fg, R F[NEQ])
 {
     R mf[NEQ];
     bool ok = true;
@@ -864,7 +898,7 @@ GRHD_HD bool local_to_coord(const R Ul[NEQ], const R Fl[NEQ], R w, R alpha, R sq
         mf[e] = Fl[e] - w * Ul[e];
         ok = ok & finite(mf[e]);
     }
-    const R fac = sqg * alpha * sqrt(gdd);
+    const R fac = fg.fac();  // sqg * alpha * sqrt(gdd), from the geometry
     ok = ok & finite(fac);
     F[EQ_DE] = fac * mf[EQ_DE];
     F[EQ_PR] = fac * mf[EQ_PR];
@@ -889,7 +923,6 @@ GRHD_HD bool hllc_flat(R g1, R gam, const R qL[NEQ], const R qR[NEQ], const FG &
     // lapse, sqrt(gamma) and the face frame were checked with the geometry
     // (fluxes.f90:195-205, 402-473)
     bool ok = (f.flags() & FG_FRAME_OK) != 0;
-    const R alpha = f.alpha(), sqg = f.sqg(), gdd = f.gdd();
 #pragma unroll
     for (int e = 0; e < NEQ; e++) {
         qLh[e] = qL[e];
@@ -932,7 +965,7 @@ GRHD_HD bool hllc_flat(R g1, R gam, const R qL[NEQ], const R qR[NEQ], const FG &
     const bool quad = fabs(A) > (R)1.0e-13 * cscale;
     // the root lambda*: linear equation, or the cancellation-free root of the
     // quadratic, or its alternate form; one division of the chosen operands
-    const R sd = sqrt(disc);
+    const R sd = sqrtr(disc);
     const R dthr = (R)1.0e-14 * rmax(rmax(Lim<R>::tiny(), fabs(B)), sd);
     const R den1 = -B - sd, den2 = -B + sd;
     const bool first = fabs(den1) > dthr;
@@ -1004,7 +1037,7 @@ GRHD_HD bool hllc_flat(R g1, R gam, const R qL[NEQ], const R qR[NEQ], const FG &
         Us[e] = outL ? uL[e] : outR ? uR[e] : us[e];
         Fs[e] = outL ? fL[e] : outR ? fR[e] : fs[e];
     }
-    const bool okT = local_to_coord(Us, Fs, w, alpha, sqg, gdd, f, F);
+    const bool okT = local_to_coord(Us, Fs, w, f, F);
     return ok & (outL | outR | okS) & okT;
 }
 
@@ -1019,7 +1052,6 @@ GRHD_HD bool hllc_try(R g1, R gam, const R qL[NEQ], const R qR[NEQ], const FG &f
     // lapse, sqrt(gamma) and the face frame were checked with the geometry
     // (fluxes.f90:195-205, 402-473)
     if (!(f.flags() & FG_FRAME_OK)) return false;
-    const R alpha = f.alpha(), sqg = f.sqg(), gdd = f.gdd();
     for (int e = 0; e < NEQ; e++) {
         qLh[e] = qL[e];
         qRh[e] = qR[e];
@@ -1037,8 +1069,8 @@ GRHD_HD bool hllc_try(R g1, R gam, const R qL[NEQ], const R qR[NEQ], const FG &f
     const R ll = rmin(lLmin, lRmin), lr = rmax(lLmax, lRmax);
     const R w = f.wface();
     if (!finite(w) || lr - ll <= (R)1.0e-13) return false;
-    if (w <= ll) return local_to_coord(uL, fL, w, alpha, sqg, gdd, f, F);
-    if (w >= lr) return local_to_coord(uR, fR, w, alpha, sqg, gdd, f, F);
+    if (w <= ll) return local_to_coord(uL, fL, w, f, F);
+    if (w >= lr) return local_to_coord(uR, fR, w, f, F);
     R uh[NEQ], fh[NEQ];
     R den = lr - ll;
     const R rhll = rcp(den);
@@ -1061,12 +1093,12 @@ GRHD_HD bool hllc_try(R g1, R gam, const R qL[NEQ], const R qR[NEQ], const FG &f
         if (fabs(B) <= (R)1.0e-14 * rmax(Lim<R>::tiny(), fabs(Cq))) return false;
         ls = rdiv(-Cq, B);
     } else {
-        den = -B - sqrt(disc);
-        if (fabs(den) > (R)1.0e-14 * rmax(rmax(Lim<R>::tiny(), fabs(B)), sqrt(disc))) {
+        den = -B - sqrtr(disc);
+        if (fabs(den) > (R)1.0e-14 * rmax(rmax(Lim<R>::tiny(), fabs(B)), sqrtr(disc))) {
             ls = rdiv(den, (R)2 * A);
         } else {
-            den = -B + sqrt(disc);
-            if (fabs(den) <= (R)1.0e-14 * rmax(rmax(Lim<R>::tiny(), fabs(B)), sqrt(disc)))
+            den = -B + sqrtr(disc);
+            if (fabs(den) <= (R)1.0e-14 * rmax(rmax(Lim<R>::tiny(), fabs(B)), sqrtr(disc)))
                 return false;
             ls = rdiv((R)2 * Cq, den);
         }
@@ -1074,7 +1106,7 @@ GRHD_HD bool hllc_try(R g1, R gam, const R qL[NEQ], const R qR[NEQ], const FG &f
     const R rtol = (R)1.0e-10 * rmax(rmax((R)1, fabs(ll)), fabs(lr));
     bool valid = finite(ls) && ls >= ll - rtol && ls <= lr + rtol && fabs(ls) <= (R)1 + rtol;
     if (!valid && quad) {
-        const R ra = rdiv(-B + sqrt(disc), (R)2 * A);
+        const R ra = rdiv(-B + sqrtr(disc), (R)2 * A);
         if (finite(ra) && ra >= ll - rtol && ra <= lr + rtol && fabs(ra) <= (R)1 + rtol) {
             ls = ra;
             valid = true;
@@ -1118,8 +1150,8 @@ GRHD_HD bool hllc_try(R g1, R gam, const R qL[NEQ], const R qR[NEQ], const FG &f
     if (!(admissible(usL) & admissible(usR))) return false;
     for (int e = 0; e < NEQ; e++)
         if (!finite(fsL[e]) || !finite(fsR[e])) return false;
-    if (w <= ls) return local_to_coord(usL, fsL, w, alpha, sqg, gdd, f, F);
-    return local_to_coord(usR, fsR, w, alpha, sqg, gdd, f, F);
+    if (w <= ls) return local_to_coord(usL, fsL, w, f, F);
+    return local_to_coord(usR, fsR, w, f, F);
 }
 
 // resolve_riemann_problem (fluxes.f90:56-72).
@@ -1357,7 +1389,7 @@ GRHD_HD void eval_res(double g1, double W, double D, double tau, double Q, doubl
 GRHD_HD double root_brent(double g1, double rho_floor, double D, double tau, double Q)
 {
     const double x1 = 1.0, xacc = 1.0e-8, EPS = 3.0e-8;
-    const double wg = (sqrt(Q) + D) / rmax(D, rho_floor);
+    const double wg = (sqrtr(Q) + D) / rmax(D, rho_floor);
     const double x2 = rmax(1.0e4, wg * 2.0);
     double a, b, cc, d = 0.0, e = 0.0, fa, fb, fc, df;
     eval_res(g1, x1, D, tau, Q, fa, df);
@@ -1429,7 +1461,7 @@ GRHD_HD double root_newton(double g1, double rho_floor, double D, double tau, do
                            double w_start = 0.0)
 {
     const double x1 = 1.0, xacc = 1.0e-10;
-    const double wg = (sqrt(Q) + D) / rmax(D, rho_floor);
+    const double wg = (sqrtr(Q) + D) / rmax(D, rho_floor);
     const double x2 = rmax(1.0e3, wg * 2.0);
     double xl, xh, f, df, flo, fhi, dx, dxold, root, tmp;
     // The residual at the lower bracket W = 1: eval_res's operations with
@@ -1493,7 +1525,7 @@ GRHD_HD bool c2p(double g1, double rho_floor, double p_floor, const double g[3][
         const double E = c[EQ_PR] + c[EQ_DE];
         const double Smax = rmax(0.0, E * E - c[EQ_DE] * c[EQ_DE]) * vmax;
         if (Ssq > Smax && Ssq > 1.0e-20) {
-            const double sf = sqrt(Smax / Ssq);
+            const double sf = sqrtr(Smax / Ssq);
             if (repaired) *repaired = true;
             c[EQ_VX] = c[EQ_VX] * sf;
             c[EQ_VY] = c[EQ_VY] * sf;
@@ -1547,7 +1579,7 @@ GRHD_HD void sources(double g1, const CentreMetric &m, const double q[NEQ], doub
     double v[3] = {q[EQ_VX], q[EQ_VY], q[EQ_VZ]}, u4[4], T[4][4];
     const double vsq = vsq_clamp<double>(Mat3<double>{m.g}, v);
     const double h = 1.0 + g1 * q[EQ_PR] / q[EQ_DE];
-    const double W = 1.0 / sqrt(1.0 - vsq);
+    const double W = 1.0 / sqrtr(1.0 - vsq);
     const double ra = 1.0 / m.alpha;
     u4[0] = quot(W, m.alpha, ra);
     for (int i = 0; i < 3; i++) u4[i + 1] = W * (v[i] - quot(m.beta[i], m.alpha, ra));

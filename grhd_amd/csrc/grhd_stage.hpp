@@ -46,7 +46,7 @@ __device__ __forceinline__ void face_geometry(const Params &P, int i, int j, R *
         else if (DIR == 2) metric_face<MK>(P, coord_cf(P, i, j), m);
         else metric_face<MK>(P, coord_cc(P, i, j), m);  // z faces: centre metric
         R g[3][3], beta[3];
-        const R alpha = (R)m.alpha, sqg = (R)sqrt(m.det);
+        const R alpha = (R)m.alpha, sqg = (R)sqrtr(m.det);
         for (int x = 0; x < 3; x++) {
             beta[x] = (R)m.beta[x];
             for (int y = 0; y < 3; y++) g[x][y] = (R)m.g[x][y];
@@ -61,8 +61,9 @@ __device__ __forceinline__ void face_geometry(const Params &P, int i, int j, R *
         R cof[3][3], gdd = 0;
         const bool ok = !(alpha <= (R)0 || sqg <= (R)0 || !finite(alpha) || !finite(sqg)) &&
                         face_frame<R, DIR>(g, cof, gdd);
-        const R w = ok ? rdiv(beta[DIR - 1], alpha * sqrt(gdd)) : (R)0;
+        const R w = ok ? rdiv(beta[DIR - 1], alpha * sqrtr(gdd)) : (R)0;
         if (ok) flags |= FG_FRAME_OK;
+        col[FG_FAC * stride] = sqg * alpha * sqrtr(gdd);
         col[FG_ALPHA * stride] = alpha;
         for (int x = 0; x < 3; x++) col[(FG_BETA + x) * stride] = beta[x];
         for (int x = 0; x < 3; x++)
@@ -583,7 +584,7 @@ __device__ __forceinline__ void c2p_body(const Params &P, double *__restrict__ p
         double vsq = 0.0;
         for (int a = 0; a < 3; a++)
             for (int b = 0; b < 3; b++) vsq = vsq + g[a][b] * v[a] * v[b];
-        w_start = (vsq < 1.0) ? 1.0 / sqrt(1.0 - vsq) : 0.0;
+        w_start = (vsq < 1.0) ? 1.0 / sqrtr(1.0 - vsq) : 0.0;
     }
 #pragma unroll
     for (int v = 0; v < NEQ; v++) c[v] = u[v * ncell + t];
