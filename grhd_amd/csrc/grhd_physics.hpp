@@ -183,14 +183,20 @@ GRHD_HD bool finite(float a) { return isfinite(a); }
 // final state unchanged bit for bit; and, through rcp and rdiv, in the FP64
 // face arithmetic: +1.7 % on the FP64 stage once the face kernels no longer
 // carry the HLLE fallback (with it inline they spilled and ran 15 % slower).
-GRHD_HD double div64(double a, double b)
+// The refinement steps of div64 from an estimate y0 of 1 / b (v_rcp_f64 on
+// the device; tests/sqrt_div_check.cpp runs them on the host from perturbed
+// estimates), for finite operands: the quotient before v_div_fixup.
+GRHD_HD double div_refine(double a, double b, double y0)
 {
-#if defined(__HIP_DEVICE_COMPILE__)
-    const double y0 = __builtin_amdgcn_rcp(b);
     const double y1 = fma(fma(-b, y0, 1.0), y0, y0);
     const double y2 = fma(fma(-b, y1, 1.0), y1, y1);
     const double q0 = a * y2;
-    return __builtin_amdgcn_div_fixup(fma(fma(-b, q0, a), y2, q0), b, a);
+    return fma(fma(-b, q0, a), y2, q0);
+}
+GRHD_HD double div64(double a, double b)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_div_fixup(div_refine(a, b, __builtin_amdgcn_rcp(b)), b, a);
 #else
     return a / b;
 #endif
@@ -204,11 +210,10 @@ GRHD_HD double rcp64(double b) { return div64(1.0, b); }
 // negative arguments give NaN as before.  No argument of the path comes near
 // that bound (velocities, sound speeds, metric determinants, discriminants of
 // conserved-state magnitudes).  The result is the IEEE square root's bit for
-// bit.  The host uses the C library's.
-GRHD_HD double sqrt64(double x)
+// bit (tests/sqrt_div_check.cpp).  The host uses the C library's.
+// sqrt_refine: the steps from an estimate y of 1 / sqrt(x).
+GRHD_HD double sqrt_refine(double x, double y)
 {
-#if defined(__HIP_DEVICE_COMPILE__)
-    const double y = __builtin_amdgcn_rsq(x);
     double g = x * y, h = y * 0.5;
     const double r = fma(-h, g, 0.5);
     g = fma(g, r, g);
@@ -216,7 +221,12 @@ GRHD_HD double sqrt64(double x)
     double d = fma(-g, g, x);
     g = fma(d, h, g);
     d = fma(-g, g, x);
-    g = fma(d, h, g);
+    return fma(d, h, g);
+}
+GRHD_HD double sqrt64(double x)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    const double g = sqrt_refine(x, __builtin_amdgcn_rsq(x));
     return __builtin_amdgcn_class(x, 0x260) ? x : g;  // +-0, +inf
 #else
     return sqrt(x);
