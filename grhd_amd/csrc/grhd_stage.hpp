@@ -349,7 +349,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
     const int nfx = P.nx + (DIR == 1), nfy = P.ny + (DIR == 2);
     const int64_t nab = (int64_t)nfx * nfy;
     const int nodes = (DIR == 1) ? P.nx : (DIR == 2) ? P.ny : P.nz;
-    const unsigned n = __hip_atomic_load(ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    // entries stop at the list's capacity (the face kernel writes none beyond)
+    const unsigned cap = (unsigned)(nab * (P.nz + (DIR == 3)));
+    const unsigned n = min(__hip_atomic_load(ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT), cap);
     for (unsigned q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
         const int64_t fo = list[q];
         const int c = (int)(fo / nab);
@@ -805,7 +807,7 @@ __global__ __launch_bounds__(256) void k_xcell_fix(Params P, const double *__res
     unsigned *ctr = P.defer_ctr;
     const unsigned *list = P.defer_list[0];
     const int64_t nab = (int64_t)(P.nx + 1) * P.ny;
-    const unsigned n = __hip_atomic_load(ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned n = min(__hip_atomic_load(ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT), (unsigned)(nab * P.nz));
     for (unsigned q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
         const int64_t t = list[q];
         const int a = (int)(t % P.nx);
