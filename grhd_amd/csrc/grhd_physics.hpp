@@ -255,6 +255,15 @@ template <typename R> GRHD_HD R quot(R a, R b, R rb)
     return fma(fma(-q, b, a), rb, q);
 }
 
+// True when x is a constant zero once the metric functions are inlined into
+// their caller: the structural zeros of a metric kind (beta^theta, gamma_r
+// theta, d_phi g, Gamma^0 entries whose every term vanishes ...).  The sums
+// over metric components drop the terms such a zero multiplies, which the
+// compiler cannot do by itself (x + 0 y is not x when y is infinite or NaN,
+// or x is -0).  For finite operands every sum keeps its value; only the sign
+// of an exact zero can differ, as with quot above.
+#define GRHD_CZERO(x) (__builtin_constant_p(x) && (x) == 0)
+
 // Reciprocal, division and reciprocal square root in the face arithmetic R.
 // FP64: the IEEE results the reference computes, through div64 on the
 // device.  FP32 (the mixed path,
@@ -562,7 +571,14 @@ GRHD_HD void metric_centre(const Params &P, const Coord &q, CentreMetric &m)
                     const double dmu = (mu == 1 || mu == 2) ? dg[rho][nu][mu - 1] : 0.0;
                     const double dnu = (nu == 1 || nu == 2) ? dg[rho][mu][nu - 1] : 0.0;
                     const double drho = (rho == 1 || rho == 2) ? dg[mu][nu][rho - 1] : 0.0;
-                    acc = acc + 0.5 * m.gm[0][rho] * (dmu + dnu - drho);
+                    if (GRHD_CZERO(m.gm[0][rho])) continue;
+                    double dsum;
+                    if (GRHD_CZERO(dmu)) dsum = dnu;
+                    else if (GRHD_CZERO(dnu)) dsum = dmu;
+                    else dsum = dmu + dnu;
+                    if (!GRHD_CZERO(drho)) dsum = dsum - drho;
+                    if (GRHD_CZERO(dsum)) continue;
+                    acc = acc + 0.5 * m.gm[0][rho] * dsum;
                 }
                 m.ch0[mu][nu] = acc;
                 m.ch0[nu][mu] = acc;
@@ -571,7 +587,9 @@ GRHD_HD void metric_centre(const Params &P, const Coord &q, CentreMetric &m)
     const double a2 = m.alpha * m.alpha, ra2 = rcp64(a2);
     for (int a = 0; a < 3; a++)
         for (int b = 0; b < 3; b++)
-            m.ginv[a][b] = m.gm[a + 1][b + 1] + quot(m.beta[a] * m.beta[b], a2, ra2);
+            m.ginv[a][b] = (GRHD_CZERO(m.beta[a]) || GRHD_CZERO(m.beta[b]))
+                               ? m.gm[a + 1][b + 1]
+                               : m.gm[a + 1][b + 1] + quot(m.beta[a] * m.beta[b], a2, ra2);
     m.sqg = sqrtr(m.det);
 }
 
@@ -609,7 +627,8 @@ GRHD_HD R vsq_clamp(const G &g, R v[3])
     const R vmax = (R)(1.0 - 1.0e-5);
     R vsq = 0;
     for (int a = 0; a < 3; a++)
-        for (int b = 0; b < 3; b++) vsq = vsq + g.gam(a, b) * v[a] * v[b];
+        for (int b = 0; b < 3; b++)
+            if (!GRHD_CZERO(g.gam(a, b))) vsq = vsq + g.gam(a, b) * v[a] * v[b];
     if (vsq >= vmax) {
         v[0] = v[0] * sqrtr(rdiv(vmax, vsq));
         v[1] = v[1] * sqrtr(rdiv(vmax, vsq));
@@ -624,7 +643,8 @@ GRHD_HD void lower(const G &g, const R v[3], R vc[3])
 {
     for (int a = 0; a < 3; a++) {
         vc[a] = 0;
-        for (int b = 0; b < 3; b++) vc[a] = vc[a] + g.gam(a, b) * v[b];
+        for (int b = 0; b < 3; b++)
+            if (!GRHD_CZERO(g.gam(a, b))) vc[a] = vc[a] + g.gam(a, b) * v[b];
     }
 }
 
@@ -665,13 +685,28 @@ GRHD_HD constexpr int sym_index(int a, int b)
     return (a <= b) ? (a == 0 ? b : (a == 1 ? 2 + b : 5)) : (b == 0 ? a : (b == 1 ? 2 + a : 5));
 }
 
+// Structural zeros of the face coframe e^(a)_i (face_frame below) along DIR.
+// gamma_{r theta} = gamma_{theta phi} = 0 for every metric kind, so the
+// Gram-Schmidt frame has one purely-theta vector (the normal for DIR = 2, the
+// first tangent for DIR = 1, the second for DIR = 3) and every other frame
+// vector has no theta component: the theta covector has only its i = theta
+// component, and the others have none there.
+template <int DIR> GRHD_HD constexpr bool cof_zero(int a, int i)
+{
+    return (a == (DIR == 1 ? 1 : (DIR == 2 ? 0 : 2))) != (i == 1);
+}
+
 template <typename R> struct FaceGeo {
     const R *col;          // field f of this face column at col[f * stride]:
     int64_t stride = 64;   // an LDS column (64 lanes) or a per-stage table
     GRHD_HD R get(int f) const { return col[f * stride]; }
     GRHD_HD R alpha() const { return get(FG_ALPHA); }
     GRHD_HD R beta(int d) const { return get(FG_BETA + d); }
-    GRHD_HD R gam(int a, int b) const { return get(FG_G + sym_index(a, b)); }
+    // gamma_{r theta} and gamma_{theta phi} vanish for every metric kind
+    GRHD_HD R gam(int a, int b) const
+    {
+        return (a + b == 1 || a + b == 3) ? (R)0 : get(FG_G + sym_index(a, b));
+    }
     GRHD_HD R sqg() const { return get(FG_SQG); }
     GRHD_HD R cof(int a, int i) const { return get(FG_COF + 3 * a + i); }
     GRHD_HD R gdd() const { return get(FG_GDD); }
@@ -898,7 +933,7 @@ GRHD_HD bool admissible(const R U[NEQ])
 // local_flux_to_coordinate (fluxes.f90:596-624).  The factor
 // sqrt(gamma) alpha sqrt(gamma^dd) depends on the face's position only and
 // comes with its geometry.
-template <typename R, typename FG>
+template <int DIR, typename R, typename FG>
 GRHD_HD bool local_to_coord(const R Ul[NEQ], const R Fl[NEQ], R w, const FG &fg, R F[NEQ])
 {
     R mf[NEQ];
@@ -916,7 +951,8 @@ GRHD_HD bool local_to_coord(const R Ul[NEQ], const R Fl[NEQ], R w, const FG &fg,
     for (int i = 0; i < 3; i++) {
         F[EQ_VX + i] = 0;
 #pragma unroll
-        for (int a = 0; a < 3; a++) F[EQ_VX + i] = F[EQ_VX + i] + fac * fg.cof(a, i) * mf[EQ_VX + a];
+        for (int a = 0; a < 3; a++)
+            if (!cof_zero<DIR>(a, i)) F[EQ_VX + i] = F[EQ_VX + i] + fac * fg.cof(a, i) * mf[EQ_VX + a];
     }
 #pragma unroll
     for (int e = 0; e < NEQ; e++) ok = ok & finite(F[e]);
@@ -944,6 +980,7 @@ GRHD_HD bool hllc_flat(R g1, R gam, const R qL[NEQ], const R qR[NEQ], const FG &
         qRh[EQ_VX + a] = 0;
 #pragma unroll
         for (int i = 0; i < 3; i++) {
+            if (cof_zero<DIR>(a, i)) continue;
             qLh[EQ_VX + a] = qLh[EQ_VX + a] + f.cof(a, i) * qL[EQ_VX + i];
             qRh[EQ_VX + a] = qRh[EQ_VX + a] + f.cof(a, i) * qR[EQ_VX + i];
         }
@@ -1047,7 +1084,7 @@ GRHD_HD bool hllc_flat(R g1, R gam, const R qL[NEQ], const R qR[NEQ], const FG &
         Us[e] = outL ? uL[e] : outR ? uR[e] : us[e];
         Fs[e] = outL ? fL[e] : outR ? fR[e] : fs[e];
     }
-    const bool okT = local_to_coord(Us, Fs, w, f, F);
+    const bool okT = local_to_coord<DIR>(Us, Fs, w, f, F);
     return ok & (outL | outR | okS) & okT;
 }
 
@@ -1070,6 +1107,7 @@ GRHD_HD bool hllc_try(R g1, R gam, const R qL[NEQ], const R qR[NEQ], const FG &f
         qLh[EQ_VX + a] = 0;
         qRh[EQ_VX + a] = 0;
         for (int i = 0; i < 3; i++) {
+            if (cof_zero<DIR>(a, i)) continue;
             qLh[EQ_VX + a] = qLh[EQ_VX + a] + f.cof(a, i) * qL[EQ_VX + i];
             qRh[EQ_VX + a] = qRh[EQ_VX + a] + f.cof(a, i) * qR[EQ_VX + i];
         }
@@ -1079,8 +1117,8 @@ GRHD_HD bool hllc_try(R g1, R gam, const R qL[NEQ], const R qR[NEQ], const FG &f
     const R ll = rmin(lLmin, lRmin), lr = rmax(lLmax, lRmax);
     const R w = f.wface();
     if (!finite(w) || lr - ll <= (R)1.0e-13) return false;
-    if (w <= ll) return local_to_coord(uL, fL, w, f, F);
-    if (w >= lr) return local_to_coord(uR, fR, w, f, F);
+    if (w <= ll) return local_to_coord<DIR>(uL, fL, w, f, F);
+    if (w >= lr) return local_to_coord<DIR>(uR, fR, w, f, F);
     R uh[NEQ], fh[NEQ];
     R den = lr - ll;
     const R rhll = rcp(den);
@@ -1160,8 +1198,8 @@ GRHD_HD bool hllc_try(R g1, R gam, const R qL[NEQ], const R qR[NEQ], const FG &f
     if (!(admissible(usL) & admissible(usR))) return false;
     for (int e = 0; e < NEQ; e++)
         if (!finite(fsL[e]) || !finite(fsR[e])) return false;
-    if (w <= ls) return local_to_coord(usL, fsL, w, f, F);
-    return local_to_coord(usR, fsR, w, f, F);
+    if (w <= ls) return local_to_coord<DIR>(usL, fsL, w, f, F);
+    return local_to_coord<DIR>(usR, fsR, w, f, F);
 }
 
 // resolve_riemann_problem (fluxes.f90:56-72).
@@ -1518,6 +1556,16 @@ GRHD_HD double root_newton(double g1, double rho_floor, double D, double tau, do
     return root_brent(g1, rho_floor, D, tau, Q);
 }
 
+// r[0] x + r[1] y + r[2] z, left to right, without the terms of structural
+// zeros of r.
+GRHD_HD double dot3(const double r[3], double x, double y, double z)
+{
+    double s = r[0] * x;
+    if (!GRHD_CZERO(r[1])) s = GRHD_CZERO(r[0]) ? r[1] * y : s + r[1] * y;
+    if (!GRHD_CZERO(r[2])) s = (GRHD_CZERO(r[0]) && GRHD_CZERO(r[1])) ? r[2] * z : s + r[2] * z;
+    return s;
+}
+
 // cons_to_prim (equations.f90:94-205).  c is repaired in place; returns true
 // when the cell was reset to the atmosphere.
 GRHD_HD bool c2p(double g1, double rho_floor, double p_floor, const double g[3][3],
@@ -1529,9 +1577,8 @@ GRHD_HD bool c2p(double g1, double rho_floor, double p_floor, const double g[3][
     bool atm = (c[EQ_DE] <= Df);
     if (!atm) {
         const double S1 = c[EQ_VX], S2 = c[EQ_VY], S3 = c[EQ_VZ];
-        const double Ssq = S1 * (gi[0][0] * S1 + gi[0][1] * S2 + gi[0][2] * S3) +
-                           S2 * (gi[1][0] * S1 + gi[1][1] * S2 + gi[1][2] * S3) +
-                           S3 * (gi[2][0] * S1 + gi[2][1] * S2 + gi[2][2] * S3);
+        const double Ssq = S1 * dot3(gi[0], S1, S2, S3) + S2 * dot3(gi[1], S1, S2, S3) +
+                           S3 * dot3(gi[2], S1, S2, S3);
         const double E = c[EQ_PR] + c[EQ_DE];
         const double Smax = rmax(0.0, E * E - c[EQ_DE] * c[EQ_DE]) * vmax;
         if (Ssq > Smax && Ssq > 1.0e-20) {
@@ -1543,7 +1590,8 @@ GRHD_HD bool c2p(double g1, double rho_floor, double p_floor, const double g[3][
         }
         double Q = 0.0;
         for (int a = 0; a < 3; a++)
-            for (int b = 0; b < 3; b++) Q = Q + gi[a][b] * c[EQ_VX + a] * c[EQ_VX + b];
+            for (int b = 0; b < 3; b++)
+                if (!GRHD_CZERO(gi[a][b])) Q = Q + gi[a][b] * c[EQ_VX + a] * c[EQ_VX + b];
         const double W = root_newton(g1, rho_floor, c[EQ_DE], c[EQ_PR], Q, brent, w_start);
         if (W < 1.0) {
             atm = true;
@@ -1556,12 +1604,13 @@ GRHD_HD bool c2p(double g1, double rho_floor, double p_floor, const double g[3][
                 q[EQ_DE] = div64(c[EQ_DE], W * sqg);
                 q[EQ_PR] = div64(V - c[EQ_PR] - c[EQ_DE], sqg);
                 const double rV = rcp64(V);
-                q[EQ_VX] = quot(gi[0][0] * c[EQ_VX] + gi[0][1] * c[EQ_VY] + gi[0][2] * c[EQ_VZ], V, rV);
-                q[EQ_VY] = quot(gi[1][0] * c[EQ_VX] + gi[1][1] * c[EQ_VY] + gi[1][2] * c[EQ_VZ], V, rV);
-                q[EQ_VZ] = quot(gi[2][0] * c[EQ_VX] + gi[2][1] * c[EQ_VY] + gi[2][2] * c[EQ_VZ], V, rV);
+                q[EQ_VX] = quot(dot3(gi[0], c[EQ_VX], c[EQ_VY], c[EQ_VZ]), V, rV);
+                q[EQ_VY] = quot(dot3(gi[1], c[EQ_VX], c[EQ_VY], c[EQ_VZ]), V, rV);
+                q[EQ_VZ] = quot(dot3(gi[2], c[EQ_VX], c[EQ_VY], c[EQ_VZ]), V, rV);
                 double vf = 0.0;
                 for (int a = 0; a < 3; a++)
-                    for (int b = 0; b < 3; b++) vf = vf + g[a][b] * q[EQ_VX + a] * q[EQ_VX + b];
+                    for (int b = 0; b < 3; b++)
+                        if (!GRHD_CZERO(g[a][b])) vf = vf + g[a][b] * q[EQ_VX + a] * q[EQ_VX + b];
                 atm = (q[EQ_DE] < rho_floor || q[EQ_PR] < p_floor || isnan(q[EQ_DE]) ||
                        isnan(q[EQ_PR]) || isnan(q[EQ_VX]) || isnan(q[EQ_VY]) || isnan(q[EQ_VZ]) ||
                        isnan(vf) || vf >= vmax);
@@ -1602,16 +1651,19 @@ GRHD_HD void sources(double g1, const CentreMetric &m, const double q[NEQ], doub
     for (int d = 0; d < 2; d++) {
         double s = 0.0;
         for (int mu = 0; mu < 4; mu++)
-            for (int nu = 0; nu < 4; nu++) s = s + 0.5 * T[mu][nu] * m.dg[mu][nu][d];
+            for (int nu = 0; nu < 4; nu++)
+                if (!GRHD_CZERO(m.dg[mu][nu][d])) s = s + 0.5 * T[mu][nu] * m.dg[mu][nu][d];
         src[EQ_VX + d] = s;
     }
     double st = 0.0;
     for (int mu = 0; mu < 4; mu++) {
-        st = st + T[mu][0] * m.dlna[mu];
-        for (int nu = 0; nu < 4; nu++) st = st - T[mu][nu] * m.ch0[mu][nu];
+        if (!GRHD_CZERO(m.dlna[mu])) st = st + T[mu][0] * m.dlna[mu];
+        for (int nu = 0; nu < 4; nu++)
+            if (!GRHD_CZERO(m.ch0[mu][nu])) st = st - T[mu][nu] * m.ch0[mu][nu];
     }
     src[EQ_PR] = st;
-    for (int e = 0; e < NEQ; e++) src[e] = src[e] * m.alpha * m.sqg;
+    for (int e = 0; e < NEQ; e++)
+        if (!GRHD_CZERO(src[e])) src[e] = src[e] * m.alpha * m.sqg;
     src[EQ_PR] = src[EQ_PR] * m.alpha;
 }
 
