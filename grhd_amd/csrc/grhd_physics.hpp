@@ -1061,7 +1061,8 @@ GRHD_HD bool hllc_flat(R g1, R gam, const R qL[NEQ], const R qR[NEQ], const FG &
         for (int e = 0; e < NEQ; e++) {
             const R uk = starL ? uR[e] : uL[e], fk = starL ? fR[e] : fL[e];
             const R pv = (e == EQ_PR) ? ls : (e == EQ_VX) ? (R)1 : (R)0;
-            uo[e] = quot(lo * uk - fk + ps * pv, dso, rdso);
+            // pv vanishes for D and the transverse momenta: no pressure term there
+            uo[e] = quot((e == EQ_PR || e == EQ_VX) ? lo * uk - fk + ps * pv : lo * uk - fk, dso, rdso);
             okO = okO & finite(fk + lo * (uo[e] - uk));
         }
         okO = okO & admissible(uo);
@@ -1072,7 +1073,7 @@ GRHD_HD bool hllc_flat(R g1, R gam, const R qL[NEQ], const R qR[NEQ], const FG &
     for (int e = 0; e < NEQ; e++) {
         const R uk = starL ? uL[e] : uR[e], fk = starL ? fL[e] : fR[e];
         const R pv = (e == EQ_PR) ? ls : (e == EQ_VX) ? (R)1 : (R)0;
-        us[e] = quot(lk * uk - fk + ps * pv, dsk, rdsk);
+        us[e] = quot((e == EQ_PR || e == EQ_VX) ? lk * uk - fk + ps * pv : lk * uk - fk, dsk, rdsk);
         fs[e] = fk + lk * (us[e] - uk);
         okS = okS & finite(fs[e]);
     }
@@ -1188,8 +1189,13 @@ GRHD_HD bool hllc_try(R g1, R gam, const R qL[NEQ], const R qR[NEQ], const FG &f
     R usL[NEQ], usR[NEQ], fsL[NEQ], fsR[NEQ];
     const R dsL = ll - ls, dsR = lr - ls, rdsL = rcp(dsL), rdsR = rcp(dsR);
     for (int e = 0; e < NEQ; e++) {
-        usL[e] = quot(ll * uL[e] - fL[e] + ps * pv[e], dsL, rdsL);
-        usR[e] = quot(lr * uR[e] - fR[e] + ps * pv[e], dsR, rdsR);
+        if (GRHD_CZERO(pv[e])) {
+            usL[e] = quot(ll * uL[e] - fL[e], dsL, rdsL);
+            usR[e] = quot(lr * uR[e] - fR[e], dsR, rdsR);
+        } else {
+            usL[e] = quot(ll * uL[e] - fL[e] + ps * pv[e], dsL, rdsL);
+            usR[e] = quot(lr * uR[e] - fR[e] + ps * pv[e], dsR, rdsR);
+        }
     }
     for (int e = 0; e < NEQ; e++) {
         fsL[e] = fL[e] + ll * (usL[e] - uL[e]);
@@ -1641,10 +1647,12 @@ GRHD_HD void sources(double g1, const CentreMetric &m, const double q[NEQ], doub
     const double W = 1.0 / sqrtr(1.0 - vsq);
     const double ra = 1.0 / m.alpha;
     u4[0] = quot(W, m.alpha, ra);
-    for (int i = 0; i < 3; i++) u4[i + 1] = W * (v[i] - quot(m.beta[i], m.alpha, ra));
+    for (int i = 0; i < 3; i++)
+        u4[i + 1] = GRHD_CZERO(m.beta[i]) ? W * v[i] : W * (v[i] - quot(m.beta[i], m.alpha, ra));
     for (int mu = 0; mu < 4; mu++)
         for (int nu = mu; nu < 4; nu++) {
-            T[mu][nu] = (q[EQ_DE] * h * u4[mu] * u4[nu]) + (q[EQ_PR] * m.gm[mu][nu]);
+            T[mu][nu] = q[EQ_DE] * h * u4[mu] * u4[nu];
+            if (!GRHD_CZERO(m.gm[mu][nu])) T[mu][nu] = T[mu][nu] + (q[EQ_PR] * m.gm[mu][nu]);
             T[nu][mu] = T[mu][nu];
         }
     for (int e = 0; e < NEQ; e++) src[e] = 0.0;
