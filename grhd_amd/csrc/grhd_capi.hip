@@ -350,7 +350,7 @@ int tune_faces(grhd_ctx *c, double dt)
 {
     c->faces_tuned = true;
     if (single_stage_fused(c->P)) return GRHD_OK;
-    static const int cand[] = {4, 2, 8, 1, 0, kFaceTableDefer};
+    static const int cand[] = {4, 2, 8, 1, 0, kFaceTableDefer, kFaceOrderZ, kFaceTableDefer | kFaceOrderZ};
     for (int d = 0; d < 3; d++) {
         if (!(d == 0 || (d == 1 && c->P.ny > 1) || (d == 2 && c->P.nz > 1))) continue;
         float best = 0.0f;
@@ -361,12 +361,14 @@ int tune_faces(grhd_ctx *c, double dt)
         // table form also fills its table
         for (int round = 0; round < 2; round++)
         for (int gw : cand) {
-            const bool table = (gw == 0 || gw == kFaceTableDefer);
+            const bool table = face_form_table(gw);
             if (table && !c->geo[d]) continue;
-            if (gw == kFaceTableDefer && !c->defer_list[d]) continue;
+            if (face_form_defers(gw) && !c->defer_list[d]) continue;
+            // the z faces walk z first in either order
+            if (d == 2 && (gw & kFaceOrderZ)) continue;
             // a forced form: only that candidate (its table filled by the
-            // warm-up launch)
-            if (c->forced_face >= 0 && gw != c->forced_face) continue;
+            // warm-up launch; the z faces take a z-first form without the flag)
+            if (c->forced_face >= 0 && gw != (d == 2 ? c->forced_face & ~kFaceOrderZ : c->forced_face)) continue;
             Params P = c->P;
             P.face_gw[d] = gw;
             P.geo_tab[d] = table ? c->geo[d] : nullptr;
@@ -383,9 +385,10 @@ int tune_faces(grhd_ctx *c, double dt)
             HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
             if (trace_tune()) {
                 std::fprintf(stderr, "grhd tune: direction %d, %s %d: %.4f ms per launch\n", d + 1,
-                             gw == kFaceTableDefer ? "geometry table, deferred faces"
-                             : gw             ? "waves per geometry group"
-                                              : "geometry table",
+                             face_form_defers(gw) ? (gw & kFaceOrderZ ? "geometry table, deferred faces, z first"
+                                                                      : "geometry table, deferred faces")
+                             : face_form_table(gw) ? (gw & kFaceOrderZ ? "geometry table, z first" : "geometry table")
+                                                   : "waves per geometry group",
                              gw, ms / 3.0);
                 std::fflush(stderr);
             }
@@ -395,7 +398,7 @@ int tune_faces(grhd_ctx *c, double dt)
             }
         }
         c->P.face_gw[d] = best_gw;
-        c->P.geo_tab[d] = (best_gw == 0 || best_gw == kFaceTableDefer) ? c->geo[d] : nullptr;
+        c->P.geo_tab[d] = face_form_table(best_gw) ? c->geo[d] : nullptr;
     }
     HIPCHK(c, hipGetLastError());
     const int rc = tune_stage_form(c, dt);
@@ -528,8 +531,8 @@ int run_steps(grhd_ctx *c, const double *dts, int n)
         for (int d = 0; d < 3; d++) {
             const bool on = (d == 0) || (d == 1 && P.ny > 1) || (d == 2 && P.nz > 1);
             if (!on || (zmarch && d == (xfused ? 0 : 2))) continue;
-            const bool table = (P.face_gw[d] == 0 || P.face_gw[d] == kFaceTableDefer);
-            per_stage += 1 + (table && !P.geo_ready) + (P.face_gw[d] == kFaceTableDefer && P.defer_list[d]);
+            const bool table = face_form_table(P.face_gw[d]);
+            per_stage += 1 + (table && !P.geo_ready) + (face_form_defers(P.face_gw[d]) && P.defer_list[d]);
         }
         if (zmarch && !P.geo_ready) per_stage += 1;
     }
@@ -1204,11 +1207,11 @@ int grhd_set_stage_form(grhd_ctx *c, int32_t zseg)
 
 int grhd_set_face_form(grhd_ctx *c, int32_t form)
 {
-    if (!c || !(form == -1 || form == 0 || form == 1 || form == 2 || form == 4 || form == 8 || form == kFaceTableDefer))
+    if (!c || !(form == -1 || form == 1 || form == 2 || form == 4 || form == 8 || face_form_table(form)))
         return GRHD_ERR_INVALID_ARGUMENT;
-    if ((form == 0 || form == kFaceTableDefer) && !c->geo[0])
+    if (face_form_table(form) && !c->geo[0])
         return fail(c, GRHD_ERR_UNSUPPORTED, "the table forms need a grid outside the fused stage form");
-    if (form == kFaceTableDefer && !c->defer_list[0])
+    if (face_form_defers(form) && !c->defer_list[0])
         return fail(c, GRHD_ERR_UNSUPPORTED, "deferred faces need the compiled MP5 + sensor + HLLC scheme");
     c->forced_face = form;
     c->faces_tuned = false;  // applied before the next step

@@ -35,8 +35,15 @@ enum MetricKind : int {
     MK_KS_LOG = 7,
 };
 
-// Params::face_gw of the geometry-table face form with deferred faces.
+// Params::face_gw of the geometry-table face form with deferred faces, and
+// the flag of the table forms' z-first block order (0 | kFaceOrderZ,
+// kFaceTableDefer | kFaceOrderZ).
 constexpr int kFaceTableDefer = 16;
+constexpr int kFaceOrderZ = 32;
+// The face forms that read the stage's geometry table, and those that defer
+// HLLC fallbacks.
+inline constexpr bool face_form_table(int f) { return (f & ~kFaceOrderZ) == 0 || (f & ~kFaceOrderZ) == kFaceTableDefer; }
+inline constexpr bool face_form_defers(int f) { return (f & ~kFaceOrderZ) == kFaceTableDefer; }
 // Params::zseg of the x-fused stage form.
 constexpr int kStageXFused = -2;
 
@@ -54,7 +61,8 @@ struct Params {
     // waves of a face kernel sharing one LDS copy of the face geometry, per
     // direction (1, 2, 4 or 8), or 0: the geometry evaluated once per stage
     // into geo_tab[d] by its own kernel (no barrier in the face kernel), or
-    // kFaceTableDefer: the table form with deferred faces (defer_list).  The
+    // kFaceTableDefer: the table form with deferred faces (defer_list); either
+    // table form with kFaceOrderZ added walks its blocks z first.  The
     // single-problem launchers' tiling, chosen by timing the face kernels on
     // the problem (tune_faces).
     int face_gw[3];

@@ -289,7 +289,12 @@ __global__ __launch_bounds__(256) void k_geo(Params P, R *__restrict__ tab)
 
 // Faces of direction DIR reading their geometry from the stage's table: the
 // blocks, lanes and planes of flux_body with groups of four, with neither the
-// geometry evaluation nor the barrier.  DEFER (scheme 1 only): faces whose
+// geometry evaluation nor the barrier.  The z faces walk their blocks along z
+// first (their stencil reaches the neighbouring planes); x and y faces walk
+// the plane first, or z first with kFaceOrderZ in their face form: each XCD
+// then keeps the table rows of its range of face columns in L2 (the x and y
+// tables of the 3D torus are 6.6 MB each, above an XCD's 4 MB), at the cost
+// of the p rows the y stencil shares between neighbouring column tiles.  DEFER (scheme 1 only): faces whose
 // HLLC solve succeeds here (flux_face_hot), the HLLC fallbacks appended to
 // the direction's list for k_flux_defer.
 template <int MK, int DIR, typename R, int SCH, bool DEFER = false>
@@ -303,8 +308,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int nct = (nfz + 3) / 4;
     const int64_t blk = xcd_block(blockIdx.x, gridDim.x);
-    const int64_t cg = (DIR == 3) ? blk / nct : blk % nchunk;
-    const int ctile = (DIR == 3) ? (int)(blk % nct) : (int)(blk / nchunk);
+    const bool zfirst = (DIR == 3) || (P.face_gw[DIR - 1] & kFaceOrderZ);
+    const int64_t cg = zfirst ? blk / nct : blk % nchunk;
+    const int ctile = zfirst ? (int)(blk % nct) : (int)(blk / nchunk);
     const int64_t ab = cg * 64 + lane;
     const int c = ctile * 4 + wave;
     if (ab >= nab || c >= nfz) return;
@@ -339,7 +345,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
 // flux_face computes for a face whose HLLC solve fails (fluxes.f90:56-72),
 // without repeating that solve.  The block that finishes last clears the
 // count and the block counter for the next stage: every block has read the
-// count before it adds itself to the block counter.
+// count before it adds itself to the block counter.  With an empty list
+// (every block reads the same count, written by the face kernel before this
+// launch) there is nothing to clear, and the blocks return at once instead
+// of queueing on the block counter.
 template <int MK, int DIR, typename R, int SCH>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_flux_defer(Params P, const double *__restrict__ p, R *__restrict__ F,
                                                     int64_t fstride, const R *__restrict__ tab)
@@ -352,6 +361,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
     // entries stop at the list's capacity (the face kernel writes none beyond)
     const unsigned cap = (unsigned)(nab * (P.nz + (DIR == 3)));
     const unsigned n = min(__hip_atomic_load(ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT), cap);
+    if (n == 0) return;
     for (unsigned q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
         const int64_t fo = list[q];
         const int c = (int)(fo / nab);
@@ -386,14 +396,14 @@ constexpr unsigned kDeferBlocks = 256;
 template <int MK, int DIR, typename R, int SCH>
 void launch_flux(const Params &P, hipStream_t s, const double *p, R *F, int64_t fstr)
 {
-    if ((P.face_gw[DIR - 1] == 0 || P.face_gw[DIR - 1] == kFaceTableDefer) && P.geo_tab[DIR - 1]) {
+    if (face_form_table(P.face_gw[DIR - 1]) && P.geo_tab[DIR - 1]) {
         R *tab = static_cast<R *>(P.geo_tab[DIR - 1]);
         const int nfx = P.nx + (DIR == 1), nfy = P.ny + (DIR == 2), nfz = P.nz + (DIR == 3);
         const int64_t nab = (int64_t)nfx * nfy;
         if (!P.geo_ready) k_geo<MK, DIR, R><<<blocks_for(nab, 256), 256, 0, s>>>(P, tab);
         const int64_t nb = ((nab + 63) / 64) * ((nfz + 3) / 4);
         if constexpr (SCH == 1) {
-            if (P.face_gw[DIR - 1] == kFaceTableDefer && P.defer_list[DIR - 1]) {
+            if (face_form_defers(P.face_gw[DIR - 1]) && P.defer_list[DIR - 1]) {
                 k_flux_tab<MK, DIR, R, SCH, true><<<(unsigned)nb, 256, 0, s>>>(P, p, F, fstr, tab);
                 k_flux_defer<MK, DIR, R, SCH><<<kDeferBlocks, 256, 0, s>>>(P, p, F, fstr, tab);
                 return;
@@ -632,10 +642,32 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
     flux_body<MK, 3, R, SCH>(P, p, Fz, fsz, L - f.nb[1], f.kt[2], f.gw[2], lds);
 }
 
+// First cell of logical block L of the cell kernels (256 cells per block).
+// A cell reads the y faces of its row and of the next row, and the z faces of
+// its plane and of the next plane.  Blocks walk tiles of 8 rows x 16 planes
+// (planes fastest), so the blocks of a tile -- about what one XCD holds at
+// once -- share those neighbouring faces in L2; in plain row order the next
+// plane's row comes 128 blocks (12 MB of streamed data on the 3D torus) after
+// its neighbour and is read from memory again.  Grids whose rows are not
+// whole blocks, or whose y and z extents are not whole tiles, keep row order.
+constexpr int kCellTileY = 8, kCellTileZ = 16;
+__device__ __forceinline__ int64_t cell_block_base(const Params &P, int64_t L)
+{
+    if (P.nx % 256 != 0 || P.ny % kCellTileY != 0 || P.nz % kCellTileZ != 0) return L * 256;
+    const int64_t spr = P.nx / 256;  // blocks per row
+    const int64_t seg = L % spr, r = L / spr;
+    const int64_t per = kCellTileY * kCellTileZ, tile = r / per, w = r % per;
+    const int ntj = P.ny / kCellTileY;
+    const int j = (int)(tile % ntj) * kCellTileY + (int)(w / kCellTileZ);
+    const int k = (int)(tile / ntj) * kCellTileZ + (int)(w % kCellTileZ);
+    return ((int64_t)k * P.ny + j) * P.nx + seg * 256;
+}
+
 // Cell update and con2prim of the same cell in one thread (stages 1..3).
 // Held to 128 VGPRs (four waves per SIMD): the kernel streams about 440 B per
 // cell, and the extra wave hides more of that latency than the few spilled
-// values cost (FP32 faces: 144 VGPRs and three waves otherwise).
+// values cost (FP32 faces: 144 VGPRs and three waves otherwise).  Launched
+// with 256-thread blocks (cell_block_base).
 template <int MK, typename R>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_cell(Params P, double *__restrict__ p, const R *__restrict__ Fx,
                                               const R *__restrict__ Fy, const R *__restrict__ Fz,
@@ -643,7 +675,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
                                               const double *__restrict__ u0, double *__restrict__ u,
                                               int stage, double dt, unsigned long long *__restrict__ counters)
 {
-    const int64_t t = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    const int64_t t = cell_block_base(P, xcd_block(blockIdx.x, gridDim.x)) + threadIdx.x;
     if (t >= (int64_t)P.nx * P.ny * P.nz) return;
     update_body<MK, R>(P, p, Fx, Fy, Fz, fsx, fsy, fsz, u0, u, stage, dt, t);
     c2p_body<MK, sizeof(R) == sizeof(float)>(P, p, u, counters, t);
@@ -808,6 +840,7 @@ __global__ __launch_bounds__(256) void k_xcell_fix(Params P, const double *__res
     const unsigned *list = P.defer_list[0];
     const int64_t nab = (int64_t)(P.nx + 1) * P.ny;
     const unsigned n = min(__hip_atomic_load(ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT), (unsigned)(nab * P.nz));
+    if (n == 0) return;  // nothing to clear (as in k_flux_defer)
     for (unsigned q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
         const int64_t t = list[q];
         const int a = (int)(t % P.nx);

@@ -254,9 +254,11 @@ int grhd_set_stage_form(grhd_ctx *ctx, int32_t zseg);
  * waves share an LDS copy of the face geometry; 0: the geometry read from a
  * per-problem table; 16: the table form with deferred faces -- for the MP5 +
  * sensor + HLLC scheme on the Kerr-Schild metrics, faces whose HLLC solve
- * falls back to HLLE are computed by a second kernel over a list of them.  Every form gives the same results bit
- * for bit.  GRHD_ERR_UNSUPPORTED for 0 or 16 where the problem has no tables
- * or no deferred-face lists. */
+ * falls back to HLLE are computed by a second kernel over a list of them;
+ * 32 and 48: forms 0 and 16 with the x- and y-face blocks walking z first
+ * (the z faces always do).  Every form gives the same results bit for bit.
+ * GRHD_ERR_UNSUPPORTED for 0, 16, 32 or 48 where the problem has no tables,
+ * and for 16 or 48 where it has no deferred-face lists. */
 int grhd_set_face_form(grhd_ctx *ctx, int32_t form);
 
 /* Device bytes grhd_create allocates for cfg: the ghosted primitives, two
