@@ -350,7 +350,8 @@ int tune_faces(grhd_ctx *c, double dt)
 {
     c->faces_tuned = true;
     if (single_stage_fused(c->P)) return GRHD_OK;
-    static const int cand[] = {4, 2, 8, 1, 0, kFaceTableDefer, kFaceOrderZ, kFaceTableDefer | kFaceOrderZ};
+    static const int cand[] = {4, 2, 8, 1, 0, kFaceTableDefer, kFaceOrderZ, kFaceTableDefer | kFaceOrderZ,
+                               kFaceOrderBand, kFaceTableDefer | kFaceOrderBand};
     for (int d = 0; d < 3; d++) {
         if (!(d == 0 || (d == 1 && c->P.ny > 1) || (d == 2 && c->P.nz > 1))) continue;
         float best = 0.0f;
@@ -364,11 +365,11 @@ int tune_faces(grhd_ctx *c, double dt)
             const bool table = face_form_table(gw);
             if (table && !c->geo[d]) continue;
             if (face_form_defers(gw) && !c->defer_list[d]) continue;
-            // the z faces walk z first in either order
-            if (d == 2 && (gw & kFaceOrderZ)) continue;
+            // the z faces walk z first in every order
+            if (d == 2 && (gw & kFaceOrders)) continue;
             // a forced form: only that candidate (its table filled by the
             // warm-up launch; the z faces take a z-first form without the flag)
-            if (c->forced_face >= 0 && gw != (d == 2 ? c->forced_face & ~kFaceOrderZ : c->forced_face)) continue;
+            if (c->forced_face >= 0 && gw != (d == 2 ? c->forced_face & ~kFaceOrders : c->forced_face)) continue;
             Params P = c->P;
             P.face_gw[d] = gw;
             P.geo_tab[d] = table ? c->geo[d] : nullptr;
@@ -385,10 +386,9 @@ int tune_faces(grhd_ctx *c, double dt)
             HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
             if (trace_tune()) {
                 std::fprintf(stderr, "grhd tune: direction %d, %s %d: %.4f ms per launch\n", d + 1,
-                             face_form_defers(gw) ? (gw & kFaceOrderZ ? "geometry table, deferred faces, z first"
-                                                                      : "geometry table, deferred faces")
-                             : face_form_table(gw) ? (gw & kFaceOrderZ ? "geometry table, z first" : "geometry table")
-                                                   : "waves per geometry group",
+                             !face_form_table(gw) ? "waves per geometry group"
+                             : face_form_defers(gw) ? "geometry table, deferred faces (+32 z first, +64 bands)"
+                                                    : "geometry table (+32 z first, +64 bands)",
                              gw, ms / 3.0);
                 std::fflush(stderr);
             }

@@ -36,14 +36,20 @@ enum MetricKind : int {
 };
 
 // Params::face_gw of the geometry-table face form with deferred faces, and
-// the flag of the table forms' z-first block order (0 | kFaceOrderZ,
-// kFaceTableDefer | kFaceOrderZ).
+// the flags of the table forms' block orders: z first (kFaceOrderZ) or in
+// bands of plane tiles, one band per XCD (kFaceOrderBand), added to 0 or
+// kFaceTableDefer.
 constexpr int kFaceTableDefer = 16;
 constexpr int kFaceOrderZ = 32;
+constexpr int kFaceOrderBand = 64;
+constexpr int kFaceOrders = kFaceOrderZ | kFaceOrderBand;
 // The face forms that read the stage's geometry table, and those that defer
 // HLLC fallbacks.
-inline constexpr bool face_form_table(int f) { return (f & ~kFaceOrderZ) == 0 || (f & ~kFaceOrderZ) == kFaceTableDefer; }
-inline constexpr bool face_form_defers(int f) { return (f & ~kFaceOrderZ) == kFaceTableDefer; }
+inline constexpr bool face_form_table(int f)
+{
+    return (f & kFaceOrders) != kFaceOrders && ((f & ~kFaceOrders) == 0 || (f & ~kFaceOrders) == kFaceTableDefer);
+}
+inline constexpr bool face_form_defers(int f) { return face_form_table(f) && (f & ~kFaceOrders) == kFaceTableDefer; }
 // Params::zseg of the x-fused stage form.
 constexpr int kStageXFused = -2;
 
@@ -62,7 +68,8 @@ struct Params {
     // direction (1, 2, 4 or 8), or 0: the geometry evaluated once per stage
     // into geo_tab[d] by its own kernel (no barrier in the face kernel), or
     // kFaceTableDefer: the table form with deferred faces (defer_list); either
-    // table form with kFaceOrderZ added walks its blocks z first.  The
+    // table form with kFaceOrderZ or kFaceOrderBand added walks its blocks z
+    // first or in bands of plane tiles.  The
     // single-problem launchers' tiling, chosen by timing the face kernels on
     // the problem (tune_faces).
     int face_gw[3];

@@ -290,11 +290,18 @@ __global__ __launch_bounds__(256) void k_geo(Params P, R *__restrict__ tab)
 // Faces of direction DIR reading their geometry from the stage's table: the
 // blocks, lanes and planes of flux_body with groups of four, with neither the
 // geometry evaluation nor the barrier.  The z faces walk their blocks along z
-// first (their stencil reaches the neighbouring planes); x and y faces walk
-// the plane first, or z first with kFaceOrderZ in their face form: each XCD
+// first (their stencil reaches the neighbouring planes).  x and y faces walk
+// the plane first, or with kFaceOrderZ in their face form z first: each XCD
 // then keeps the table rows of its range of face columns in L2 (the x and y
 // tables of the 3D torus are 6.6 MB each, above an XCD's 4 MB), at the cost
-// of the p rows the y stencil shares between neighbouring column tiles.  DEFER (scheme 1 only): faces whose
+// of the p rows the y stencil shares between neighbouring column tiles.  With
+// kFaceOrderBand they walk the plane tiles of a band, then the next column
+// tile, band after band, the bands being an eighth of the plane tiles: each
+// XCD's contiguous range of blocks (xcd_block) is then about one band, a slab
+// of planes whose p rows and table rows it reads once into its own L2.  On
+// the 3D torus (bands of 4 plane tiles, 16 planes per XCD) the x and y faces
+// measured 16 and 11 % faster than plane first; bands of 2, 3, 8 and 16 plane
+// tiles were 1.4-5 % slower per stage than bands of 4.  DEFER (scheme 1 only): faces whose
 // HLLC solve succeeds here (flux_face_hot), the HLLC fallbacks appended to
 // the direction's list for k_flux_defer.
 template <int MK, int DIR, typename R, int SCH, bool DEFER = false>
@@ -308,9 +315,23 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int nct = (nfz + 3) / 4;
     const int64_t blk = xcd_block(blockIdx.x, gridDim.x);
-    const bool zfirst = (DIR == 3) || (P.face_gw[DIR - 1] & kFaceOrderZ);
-    const int64_t cg = zfirst ? blk / nct : blk % nchunk;
-    const int ctile = zfirst ? (int)(blk % nct) : (int)(blk / nchunk);
+    const int order = (DIR == 3) ? kFaceOrderZ : (P.face_gw[DIR - 1] & kFaceOrders);
+    int64_t cg;
+    int ctile;
+    if (order == kFaceOrderZ) {
+        cg = blk / nct;
+        ctile = (int)(blk % nct);
+    } else if (order == kFaceOrderBand) {
+        const int bw = (nct + (int)kXCD - 1) / (int)kXCD;  // plane tiles per band
+        const int band = (int)(blk / (nchunk * bw));
+        const int64_t r = blk - (int64_t)band * nchunk * bw;
+        const int nb = min(bw, nct - band * bw);  // plane tiles of this band
+        cg = r / nb;
+        ctile = band * bw + (int)(r % nb);
+    } else {
+        cg = blk % nchunk;
+        ctile = (int)(blk / nchunk);
+    }
     const int64_t ab = cg * 64 + lane;
     const int c = ctile * 4 + wave;
     if (ab >= nab || c >= nfz) return;

@@ -214,7 +214,8 @@ class Solver:
     def set_face_form(self, form: int):
         """Face-kernel form (grhd_set_face_form): -1 timed, 1/2/4/8 LDS
         geometry groups, 0 geometry table, 16 table with deferred faces,
-        32/48 forms 0/16 with the x and y faces walking z first."""
+        32/48 forms 0/16 with the x and y faces walking z first, 64/80 with
+        them walking bands of plane tiles."""
         self._chk(self._L.grhd_set_face_form(self.h, int(form)))
 
     def last_timing(self):

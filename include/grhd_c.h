@@ -256,9 +256,12 @@ int grhd_set_stage_form(grhd_ctx *ctx, int32_t zseg);
  * sensor + HLLC scheme on the Kerr-Schild metrics, faces whose HLLC solve
  * falls back to HLLE are computed by a second kernel over a list of them;
  * 32 and 48: forms 0 and 16 with the x- and y-face blocks walking z first
- * (the z faces always do).  Every form gives the same results bit for bit.
- * GRHD_ERR_UNSUPPORTED for 0, 16, 32 or 48 where the problem has no tables,
- * and for 16 or 48 where it has no deferred-face lists. */
+ * (the z faces always do); 64 and 80: forms 0 and 16 with those blocks
+ * walking bands of plane tiles (an eighth of the planes, one band per XCD)
+ * per column tile.  Every form
+ * gives the same results bit for bit.  GRHD_ERR_UNSUPPORTED for the table
+ * forms (0, 16, 32, 48, 64, 80) where the problem has no tables, and for 16,
+ * 48 or 80 where it has no deferred-face lists. */
 int grhd_set_face_form(grhd_ctx *ctx, int32_t form);
 
 /* Device bytes grhd_create allocates for cfg: the ghosted primitives, two

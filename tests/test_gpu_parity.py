@@ -168,6 +168,7 @@ def _large(name):
 @pytest.mark.parametrize("name,zseg,face", [("fm3d_ks_mp5_hllc_sensor", -1, -1), ("fm3d_ks_mp5_hllc_sensor", 0, -1),
                                             ("fm3d_ks_mp5_hllc_sensor", 16, -1), ("fm3d_ks_mp5_hllc_sensor", 0, 16),
                                             ("fm3d_ks_mp5_hllc_sensor", 0, 4), ("fm3d_ks_mp5_hllc_sensor", 0, 48),
+                                            ("fm3d_ks_mp5_hllc_sensor", 0, 64),
                                             ("fm3d_ks_mp5_hllc_sensor", -2, -1),
                                             ("fm3d_ks_weno5_hlle", -1, -1), ("fm3d_ks_weno5_hlle", 8, -1),
                                             ("fm3d_ks_weno5_hlle", -2, -1)])
@@ -236,7 +237,7 @@ def test_face_forms_agree_bit_for_bit(precision):
     geometry (4), the geometry table (0) and the table with deferred faces
     (16: faces whose HLLC solve falls back to HLLE computed by a second kernel
     over a list of them), and both table forms with the x- and y-face blocks
-    walking z first (32, 48) -- give the same
+    walking z first (32, 48) or in bands of plane tiles (64, 80) -- give the same
     state and counters after three steps, bit for bit, in FP64 and with FP32
     faces, in the split, z-march and x-fused stage forms (the x-fused form
     computes its x faces itself, in the run-time cascade with the fallback
@@ -245,7 +246,7 @@ def test_face_forms_agree_bit_for_bit(precision):
     print("face forms", cfg.nx, cfg.ny, cfg.nz, "precision", precision, flush=True)
     out = {}
     for zseg in (0, 16, -2):
-        for form in (4, 0, 16, 32, 48):
+        for form in (4, 0, 16, 32, 48, 64, 80):
             with Solver(cfg) as S:
                 S.set_stage_form(zseg)
                 S.set_face_form(form)
@@ -266,15 +267,15 @@ def test_face_forms_agree_bit_for_bit(precision):
 def test_forms_agree_at_workload_size(precision):
     """The 3D torus benchmark grid (256 x 128 x 128, theta in [0.05 pi,
     0.95 pi], 1e-2 density noise): two steps in the split form with the
-    geometry table alone and with deferred faces (in both block orders), and
-    in the x-fused form, give the same state bit for bit."""
+    geometry table alone and with deferred faces (in the three block orders),
+    and in the x-fused form, give the same state bit for bit."""
     import bench
     from grhd_amd.solver import fishbone_unit_density_k
     cfg = bench.c4_config().replace(precision=precision)
     cfg.fm_polytropic_constant = fishbone_unit_density_k(cfg)
     print("stage and face forms 256x128x128 precision", precision, flush=True)
     out = {}
-    for stage, face in ((0, 0), (0, 16), (0, 32), (0, 48), (-2, 48)):
+    for stage, face in ((0, 0), (0, 16), (0, 32), (0, 48), (0, 64), (0, 80), (-2, 48)):
         with Solver(cfg) as S:
             S.set_stage_form(stage)
             S.set_face_form(face)
@@ -292,15 +293,16 @@ def test_face_form_refused_where_unsupported():
     """The table forms need a grid outside the fused stage form, deferred
     faces the MP5 + sensor + HLLC scheme."""
     with Solver(config_from_par(CASES["fm3d_ks_mp5_hllc_sensor"])) as S:
-        for form in (16, 48, 0, 32, 3):
+        for form in (16, 48, 80, 0, 32, 64, 3, 96, 112):
             with pytest.raises(Exception):
                 S.set_face_form(form)
     with Solver(_large("fm3d_ks_weno5_hlle")) as S:
-        for form in (16, 48):
+        for form in (16, 48, 80, 96):
             with pytest.raises(Exception):
                 S.set_face_form(form)
         S.set_face_form(0)
         S.set_face_form(32)
+        S.set_face_form(64)
 
 
 def test_stage_form_refused_on_small_grids():
