@@ -364,12 +364,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
 // takes list entries q = thread, thread + grid, ... below the count and
 // computes the face's HLLE flux from its reconstructed states -- what
 // flux_face computes for a face whose HLLC solve fails (fluxes.f90:56-72),
-// without repeating that solve.  The block that finishes last clears the
-// count and the block counter for the next stage: every block has read the
-// count before it adds itself to the block counter.  With an empty list
-// (every block reads the same count, written by the face kernel before this
-// launch) there is nothing to clear, and the blocks return at once instead
-// of queueing on the block counter.
+// without repeating that solve.  Every block reads the same count (written by
+// the face kernel before this launch) and so the same number of working
+// blocks, enough for one entry per thread up to the grid: the others return
+// at once, as every block does for an empty list.  The working block that
+// finishes last clears the count and the block counter for the next stage:
+// every working block has read the count before it adds itself to the block
+// counter.
 template <int MK, int DIR, typename R, int SCH>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_flux_defer(Params P, const double *__restrict__ p, R *__restrict__ F,
                                                     int64_t fstride, const R *__restrict__ tab)
@@ -382,8 +383,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
     // entries stop at the list's capacity (the face kernel writes none beyond)
     const unsigned cap = (unsigned)(nab * (P.nz + (DIR == 3)));
     const unsigned n = min(__hip_atomic_load(ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT), cap);
-    if (n == 0) return;
-    for (unsigned q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
+    const unsigned nbk = min(gridDim.x, (n + blockDim.x - 1) / blockDim.x);  // working blocks
+    if (blockIdx.x >= nbk) return;
+    for (unsigned q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += nbk * blockDim.x) {
         const int64_t fo = list[q];
         const int c = (int)(fo / nab);
         const int64_t ab = fo - (int64_t)c * nab;
@@ -400,7 +402,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
     __syncthreads();
     if (threadIdx.x == 0) {
         const unsigned fin = __hip_atomic_fetch_add(ctr + 1, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        if (fin == gridDim.x - 1) {
+        if (fin == nbk - 1) {
             __hip_atomic_store(ctr, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             __hip_atomic_store(ctr + 1, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
@@ -408,11 +410,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
 }
 
 // Blocks of k_flux_defer and k_xcell_fix: one pass over lists of up to
-// 65,536 entries (the HLLC fallbacks of the 3D torus in FP64: none of the x
-// faces, 1-1.5 % of the y and z faces); longer lists take more passes.  A
-// grid of 1024 blocks measured 15 us slower per launch: the launch pays for
-// its empty blocks.
-constexpr unsigned kDeferBlocks = 256;
+// 262,144 entries (the HLLC fallbacks of the 3D torus in FP64: none of the x
+// faces, 1-3 % of the y and z faces, 40-140 thousand per stage), four waves
+// per SIMD; longer lists take more passes.  The blocks past a list's extent
+// return at once.
+constexpr unsigned kDeferBlocks = 1024;
 
 template <int MK, int DIR, typename R, int SCH>
 void launch_flux(const Params &P, hipStream_t s, const double *p, R *F, int64_t fstr)
@@ -627,12 +629,18 @@ __device__ __forceinline__ void c2p_body(const Params &P, double *__restrict__ p
     if (atm) atomicAdd(&counters[0], 1ull);
     if (brent) atomicAdd(&counters[1], 1ull);
     double *pw = SPLIT ? pout : p;
+    // Mixed path: the store addresses are formed here, after the root solve.
+    // Computed early, two of them stayed live through it and spilled at 128
+    // VGPRs (4 % of the cell kernel's traffic; the kernel is 112 VGPRs
+    // without); the FP64 kernel measured 0.3 % faster without the barrier.
+    int64_t pcw = pc, tw = t;
+    if constexpr (WARM) asm volatile("" : "+v"(pcw), "+v"(tw));
 #pragma unroll
-    for (int v = 0; v < NEQ; v++) pw[v * P.vstride + pc] = q[v];
+    for (int v = 0; v < NEQ; v++) pw[v * P.vstride + pcw] = q[v];
     // U changes only where con2prim rescales the momentum or resets the cell
     if (repaired) {
 #pragma unroll
-        for (int v = 0; v < NEQ; v++) u[v * ncell + t] = c[v];
+        for (int v = 0; v < NEQ; v++) u[v * ncell + tw] = c[v];
     }
 }
 
@@ -848,8 +856,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
 // The cells k_xcell deferred: each thread takes list entries q = thread,
 // thread + grid, ... below the count, computes the cell's two x faces with
 // flux_face (HLLC with its HLLE fallback) and runs the cell's update and
-// con2prim as k_xcell does.  The block that finishes last clears the count and
-// the block counter (as in k_flux_defer).
+// con2prim as k_xcell does.  The working block that finishes last clears the
+// count and the block counter (as in k_flux_defer).
 template <int MK, typename R, int SCH>
 __global__ __launch_bounds__(256) void k_xcell_fix(Params P, const double *__restrict__ pin, double *__restrict__ pout,
                                                    const R *__restrict__ Fy, const R *__restrict__ Fz, int64_t fsy,
@@ -861,8 +869,9 @@ __global__ __launch_bounds__(256) void k_xcell_fix(Params P, const double *__res
     const unsigned *list = P.defer_list[0];
     const int64_t nab = (int64_t)(P.nx + 1) * P.ny;
     const unsigned n = min(__hip_atomic_load(ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT), (unsigned)(nab * P.nz));
-    if (n == 0) return;  // nothing to clear (as in k_flux_defer)
-    for (unsigned q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
+    const unsigned nbk = min(gridDim.x, (n + blockDim.x - 1) / blockDim.x);  // working blocks, as in k_flux_defer
+    if (blockIdx.x >= nbk) return;
+    for (unsigned q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += nbk * blockDim.x) {
         const int64_t t = list[q];
         const int a = (int)(t % P.nx);
         const int64_t rem = t / P.nx;
@@ -883,7 +892,7 @@ __global__ __launch_bounds__(256) void k_xcell_fix(Params P, const double *__res
     __syncthreads();
     if (threadIdx.x == 0) {
         const unsigned fin = __hip_atomic_fetch_add(ctr + 1, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        if (fin == gridDim.x - 1) {
+        if (fin == nbk - 1) {
             __hip_atomic_store(ctr, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             __hip_atomic_store(ctr + 1, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
