@@ -23,6 +23,37 @@ __device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned nb)
     return x * per + (x < rem ? x : rem) + q;
 }
 
+// Column (a, b) = (ab % nfx, ab / nfx) of face column ab = ab0 + lane, where
+// ab0 is uniform over the wave: the division is done once per wave, in 32
+// bits, and the lane offset wraps at most once when rows hold 64 or more
+// faces (a per-lane 64-bit division otherwise).
+__device__ __forceinline__ void face_column(int nfx, int64_t ab0, int lane, int &a, int &b)
+{
+    if (nfx >= 64 && ab0 + 64 < ((int64_t)1 << 31)) {
+        const unsigned u0 = (unsigned)ab0;
+        a = (int)(u0 % (unsigned)nfx) + lane;
+        b = (int)(u0 / (unsigned)nfx);
+        if (a >= nfx) {
+            a -= nfx;
+            b += 1;
+        }
+    } else {
+        const int64_t ab = ab0 + lane;
+        a = (int)(ab % nfx);
+        b = (int)(ab / nfx);
+    }
+}
+
+// Zero-based indices of an interior cell (x fastest).
+struct CellIdx {
+    int i0, j0, k0;
+};
+__device__ __forceinline__ CellIdx cell_of(const Params &P, int64_t t)
+{
+    const int64_t rem = t / P.nx;
+    return {(int)(t % P.nx), (int)(rem % P.ny), (int)(rem / P.ny)};
+}
+
 // -------------------------------------------------------------- fluxes --
 // Face arrays have shape [5][nfz][nfy][nfx] with nf_d = n_d + (d == DIR).
 
@@ -225,7 +256,8 @@ __device__ __forceinline__ void flux_body(const Params &P, const double *__restr
     const int64_t chunk = cg * groups + grp;
     const int64_t ab = chunk * 64 + lane;
     const bool active = ab < nab;
-    const int a = active ? (int)(ab % nfx) : 0, b = active ? (int)(ab / nfx) : 0;
+    int a = 0, b = 0;
+    if (active) face_column(nfx, chunk * 64, lane, a, b);
     // reference indices: faces 0..n along DIR, cells 1..n across
     const int i = a + (DIR != 1), j = b + (DIR != 2);
     R *col = lds + grp * NFG * 64 + lane;
@@ -311,31 +343,33 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
     static_assert(!DEFER || SCH == 1, "deferred faces: scheme 1 only");
     const int nfx = P.nx + (DIR == 1), nfy = P.ny + (DIR == 2), nfz = P.nz + (DIR == 3);
     const int64_t nab = (int64_t)nfx * nfy;
-    const int64_t nchunk = (nab + 63) / 64;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int nct = (nfz + 3) / 4;
-    const int64_t blk = xcd_block(blockIdx.x, gridDim.x);
+    // block -> (column tile, plane tile): uniform, in 32 bits (the grid has
+    // fewer than 2^32 blocks), which keeps the scalar prologue short
+    const unsigned nchunk = (unsigned)((nab + 63) / 64);
+    const unsigned nct = (unsigned)(nfz + 3) / 4u;
+    const unsigned blk = xcd_block(blockIdx.x, gridDim.x);
     const int order = (DIR == 3) ? kFaceOrderZ : (P.face_gw[DIR - 1] & kFaceOrders);
-    int64_t cg;
-    int ctile;
+    unsigned cg, ctile;
     if (order == kFaceOrderZ) {
         cg = blk / nct;
-        ctile = (int)(blk % nct);
+        ctile = blk % nct;
     } else if (order == kFaceOrderBand) {
-        const int bw = (nct + (int)kXCD - 1) / (int)kXCD;  // plane tiles per band
-        const int band = (int)(blk / (nchunk * bw));
-        const int64_t r = blk - (int64_t)band * nchunk * bw;
-        const int nb = min(bw, nct - band * bw);  // plane tiles of this band
+        const unsigned bw = (nct + kXCD - 1) / kXCD;  // plane tiles per band
+        const unsigned band = blk / (nchunk * bw);
+        const unsigned r = blk - band * nchunk * bw;
+        const unsigned nb = min(bw, nct - band * bw);  // plane tiles of this band
         cg = r / nb;
-        ctile = band * bw + (int)(r % nb);
+        ctile = band * bw + r % nb;
     } else {
         cg = blk % nchunk;
-        ctile = (int)(blk / nchunk);
+        ctile = blk / nchunk;
     }
-    const int64_t ab = cg * 64 + lane;
-    const int c = ctile * 4 + wave;
+    const int64_t ab = (int64_t)cg * 64 + lane;
+    const int c = (int)ctile * 4 + wave;
     if (ab >= nab || c >= nfz) return;
-    const int a = (int)(ab % nfx), b = (int)(ab / nfx);
+    int a, b;
+    face_column(nfx, (int64_t)cg * 64, lane, a, b);
     const int i = a + (DIR != 1), j = b + (DIR != 2);
     const FaceGeo<R> fg{tab + ab, nab};
     const int nodes = (DIR == 1) ? P.nx : (DIR == 2) ? P.ny : P.nz;
@@ -386,10 +420,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
     const unsigned nbk = min(gridDim.x, (n + blockDim.x - 1) / blockDim.x);  // working blocks
     if (blockIdx.x >= nbk) return;
     for (unsigned q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += nbk * blockDim.x) {
-        const int64_t fo = list[q];
-        const int c = (int)(fo / nab);
-        const int64_t ab = fo - (int64_t)c * nab;
-        const int a = (int)(ab % nfx), b = (int)(ab / nfx);
+        // face indices are below 2^32 where the lists exist
+        const unsigned fo = list[q];
+        const int c = (int)(fo / (unsigned)nab);
+        const unsigned ab = fo - (unsigned)c * (unsigned)nab;
+        const int a = (int)(ab % (unsigned)nfx), b = (int)(ab / (unsigned)nfx);
         const int i = a + (DIR != 1), j = b + (DIR != 2), k = c + (DIR != 3);
         const int fi = (DIR == 1) ? i : (DIR == 2) ? j : k;
         const FaceGeo<R> fg{tab + ab, nab};
@@ -502,12 +537,12 @@ __device__ __forceinline__ void update_body(const Params &P, const double *__res
                                             int64_t fsz, const double *__restrict__ u0,
                                             double *__restrict__ u, int stage, double dt, int64_t t,
                                             const R *zl = nullptr, const R *zr = nullptr,
-                                            const R *xl = nullptr, const R *xr = nullptr)
+                                            const R *xl = nullptr, const R *xr = nullptr,
+                                            const CellIdx *ci = nullptr)
 {
     const int64_t ncell = (int64_t)P.nx * P.ny * P.nz;
-    const int i0 = (int)(t % P.nx);
-    const int64_t rem = t / P.nx;
-    const int j0 = (int)(rem % P.ny), k0 = (int)(rem / P.ny);
+    const CellIdx c0 = ci ? *ci : cell_of(P, t);
+    const int i0 = c0.i0, j0 = c0.j0, k0 = c0.k0;
     const int i = i0 + 1, j = j0 + 1, k = k0 + 1;
     const int64_t pc = pindex(P, i, j, k);
     double q[NEQ], rhs[NEQ], src[NEQ];
@@ -604,12 +639,11 @@ template <int MK, bool WARM, bool SPLIT = false>
 __device__ __forceinline__ void c2p_body(const Params &P, double *__restrict__ p,
                                          double *__restrict__ u,
                                          unsigned long long *__restrict__ counters, int64_t t,
-                                         double *__restrict__ pout = nullptr)
+                                         double *__restrict__ pout = nullptr, const CellIdx *ci = nullptr)
 {
     const int64_t ncell = (int64_t)P.nx * P.ny * P.nz;
-    const int i = (int)(t % P.nx) + 1;
-    const int64_t rem = t / P.nx;
-    const int j = (int)(rem % P.ny) + 1, k = (int)(rem / P.ny) + 1;
+    const CellIdx c0 = ci ? *ci : cell_of(P, t);
+    const int i = c0.i0 + 1, j = c0.j0 + 1, k = c0.k0 + 1;
     double g[3][3], gi[3][3], sqg, alpha, beta[3], q[NEQ], c[NEQ];
     metric_c2p<MK>(P, coord_cc(P, i, j), g, gi, sqg, alpha, beta);
     const int64_t pc = pindex(P, i, j, k);
@@ -671,32 +705,39 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
     flux_body<MK, 3, R, SCH>(P, p, Fz, fsz, L - f.nb[1], f.kt[2], f.gw[2], lds);
 }
 
-// First cell of logical block L of the cell kernels (256 cells per block).
-// A cell reads the y faces of its row and of the next row, and the z faces of
+// Cell of thread tid of logical block L of the cell kernels (256 cells per
+// block), and its indices.  A cell reads the y faces of its row and of the next row, and the z faces of
 // its plane and of the next plane.  Blocks walk tiles of 8 rows x 16 planes
 // (planes fastest), so the blocks of a tile -- about what one XCD holds at
 // once -- share those neighbouring faces in L2; in plain row order the next
 // plane's row comes 128 blocks (12 MB of streamed data on the 3D torus) after
 // its neighbour and is read from memory again.  Grids whose rows are not
 // whole blocks, or whose y and z extents are not whole tiles, keep row order.
+// In the tiled walk a block's row and plane are uniform: the divisions are
+// done once per wave in 32 bits, not per lane in 64.
 constexpr int kCellTileY = 8, kCellTileZ = 16;
-__device__ __forceinline__ int64_t cell_block_base(const Params &P, int64_t L)
+__device__ __forceinline__ int64_t cell_block(const Params &P, unsigned L, unsigned tid, CellIdx &ci)
 {
-    if (P.nx % 256 != 0 || P.ny % kCellTileY != 0 || P.nz % kCellTileZ != 0) return L * 256;
-    const int64_t spr = P.nx / 256;  // blocks per row
-    const int64_t seg = L % spr, r = L / spr;
-    const int64_t per = kCellTileY * kCellTileZ, tile = r / per, w = r % per;
-    const int ntj = P.ny / kCellTileY;
-    const int j = (int)(tile % ntj) * kCellTileY + (int)(w / kCellTileZ);
-    const int k = (int)(tile / ntj) * kCellTileZ + (int)(w % kCellTileZ);
-    return ((int64_t)k * P.ny + j) * P.nx + seg * 256;
+    if (P.nx % 256 == 0 && P.ny % kCellTileY == 0 && P.nz % kCellTileZ == 0) {
+        const unsigned spr = (unsigned)P.nx / 256u;  // blocks per row
+        const unsigned seg = L % spr, r = L / spr;
+        const unsigned per = kCellTileY * kCellTileZ, tile = r / per, w = r % per;
+        const unsigned ntj = (unsigned)P.ny / kCellTileY;
+        ci.j0 = (int)((tile % ntj) * kCellTileY + w / kCellTileZ);
+        ci.k0 = (int)((tile / ntj) * kCellTileZ + w % kCellTileZ);
+        ci.i0 = (int)(seg * 256u + tid);
+        return ((int64_t)ci.k0 * P.ny + ci.j0) * P.nx + ci.i0;
+    }
+    const int64_t t = (int64_t)L * 256 + tid;
+    ci = cell_of(P, t);
+    return t;
 }
 
 // Cell update and con2prim of the same cell in one thread (stages 1..3).
 // Held to 128 VGPRs (four waves per SIMD): the kernel streams about 440 B per
 // cell, and the extra wave hides more of that latency than the few spilled
 // values cost (FP32 faces: 144 VGPRs and three waves otherwise).  Launched
-// with 256-thread blocks (cell_block_base).
+// with 256-thread blocks (cell_block).
 template <int MK, typename R>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_cell(Params P, double *__restrict__ p, const R *__restrict__ Fx,
                                               const R *__restrict__ Fy, const R *__restrict__ Fz,
@@ -704,10 +745,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
                                               const double *__restrict__ u0, double *__restrict__ u,
                                               int stage, double dt, unsigned long long *__restrict__ counters)
 {
-    const int64_t t = cell_block_base(P, xcd_block(blockIdx.x, gridDim.x)) + threadIdx.x;
+    CellIdx ci;
+    const int64_t t = cell_block(P, xcd_block(blockIdx.x, gridDim.x), threadIdx.x, ci);
     if (t >= (int64_t)P.nx * P.ny * P.nz) return;
-    update_body<MK, R>(P, p, Fx, Fy, Fz, fsx, fsy, fsz, u0, u, stage, dt, t);
-    c2p_body<MK, sizeof(R) == sizeof(float)>(P, p, u, counters, t);
+    update_body<MK, R>(P, p, Fx, Fy, Fz, fsx, fsy, fsz, u0, u, stage, dt, t, nullptr, nullptr, nullptr, nullptr,
+                       &ci);
+    c2p_body<MK, sizeof(R) == sizeof(float)>(P, p, u, counters, t, nullptr, &ci);
 }
 
 // z-march form of the stage (3D grids): each lane owns one (x, y) column and
